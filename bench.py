@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the per-timestep IB-LBM path on one MI355X.
+
+Headline workload (``value``): D3Q19 BGK flow past a fixed sphere on a
+256x128x128 lattice, D = 24 cells at (64, 64, 64), 1810 markers, Re = 100,
+U = 0.04, explicit direct forcing with one iteration and the 4-point
+kernel, FP32 population storage with FP32 collision and FP64 IB / force
+reductions.  A step is one coarse step of the reference (IB forcing,
+collide, stream, macros, Sum(Rho), ComputeIBForce).  ``value`` is MLUPS =
+cells * steps / wall seconds of the timed region, state resident in HBM.
+
+``roofline`` is for the dominant kernel (the fused stream-collide k_step):
+algorithmic bytes per launch = cells * 2 * Q * sizeof(population), divided
+by its average duration measured with HIP events on the level's stream.
+
+Also reported: the same case with FP64 and FP16-shifted storage, the
+periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6), a plain copy
+kernel's bandwidth, the Cd error of reduced storage against FP64, and the
+CPU oracle (FP64, OpenMP) timed on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+from ilbmrex_amd import config as cf  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "MLUPS & %HBM roofline, FP32 vs FP64; batched p50/p99 latency at Cd err<0.5%"
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def sphere_case(storage):
+    return cf.sphere_3d(nx=256, ny=128, nz=128, D=24.0, center=(64.0, 64.0, 64.0), U=0.04,
+                        Re=100.0, coupling=cf.COUPLING_EXPLICIT, loop_ns=1, delta_type=4,
+                        storage=storage)
+
+
+def periodic_case(storage, force):
+    return cf.periodic_box(cf.D3Q19, (256, 256, 256), nu=(0.6 - 0.5) / 3.0, force=force,
+                           storage=storage, name="periodic256")
+
+
+def run_level(case, init, warmup, steps, want_forces=False):
+    from ilbmrex_amd.native import Level
+    lev = Level(case, init)
+    lev.advance(warmup)
+    lev.set_timing(True)
+    lev.synchronize()
+    t0 = time.perf_counter()
+    lev.advance(steps)
+    lev.synchronize()
+    wall = time.perf_counter() - t0
+    kms, nl = lev.kernel_time()
+    info = lev.info()
+    fh = lev.force_history() if want_forces else None
+    lev.close()
+    cells = case.ncells
+    q = case.lattice.q
+    bpl = 2 * q * info["pop_bytes"]
+    mlups = cells * steps / wall / 1e6
+    kgbs = cells * bpl / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return {"mlups": mlups, "ms_per_step": 1e3 * wall / steps, "kernel_ms": kms,
+            "kernel_launches": nl, "bytes_per_lup": bpl, "kernel_gbs": kgbs,
+            "kernel_frac": kgbs / HBM_PEAK_GBS, "bytes_per_launch": cells * bpl}, fh
+
+
+def cpu_baseline(steps):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import refcpu
+    refcpu.build()
+    case = sphere_case(cf.STORE_F64)
+    init = cf.uniform_init(case, u=(0.04, 0.0, 0.0))
+    orc = refcpu.Oracle(case, init)
+    orc.advance(1)
+    t0 = time.perf_counter()
+    orc.advance(steps)
+    wall = time.perf_counter() - t0
+    orc.close()
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": case.ncells * steps / wall / 1e6, "unit": "MLUPS", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} steps (after 1 untimed) of the same 256x128x128 D3Q19 sphere case, "
+                      f"FP64, oracle/refcpu.c with OpenMP ({threads} threads)"}
+
+
+def read_traffic():
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    try:
+        with open(p) as f:
+            return json.load(f).get("k_step_sphere_fp32_bytes_per_launch")
+    except Exception:
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--variant-steps", type=int, default=400)
+    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--no-variants", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design, not a build).", file=sys.stderr)
+        sys.exit(2)
+    from ilbmrex_amd import native
+    native.load()
+
+    # headline: sphere, FP32 storage
+    c32 = sphere_case(cf.STORE_F32)
+    init = cf.uniform_init(c32, u=(0.04, 0.0, 0.0))
+    log(f"[bench] sphere 256x128x128 FP32: warmup {args.warmup}, timed {args.steps}")
+    head, fh32 = run_level(c32, init, args.warmup, args.steps, want_forces=True)
+    log(f"[bench] sphere FP32: {head['mlups']:.1f} MLUPS, k_step {head['kernel_ms']*1e3:.1f} us "
+        f"({head['kernel_gbs']:.0f} GB/s)")
+
+    variants = {}
+    if not args.no_variants:
+        vs = args.variant_steps
+        for name, case in (("sphere_f64", sphere_case(cf.STORE_F64)),
+                           ("sphere_f16s", sphere_case(cf.STORE_F16S))):
+            log(f"[bench] {name}: {vs} steps")
+            r, fh = run_level(case, init, args.warmup, args.steps if name == "sphere_f64" else vs,
+                              want_forces=True)
+            variants[name] = r
+            if name == "sphere_f64":
+                fh64 = fh
+            else:
+                fh16 = fh
+        # Cd of reduced storage against FP64 over the same steps
+        n = min(len(fh32), len(fh64))
+        cd32, cd64 = fh32[:n, 4], fh64[:n, 4]
+        tail = slice(max(0, n - 200), n)
+        variants["cd_err_f32_vs_f64"] = float(abs(cd32[tail].mean() - cd64[tail].mean()) /
+                                               abs(cd64[tail].mean()))
+        variants["cd_f64_last"] = float(cd64[-1])
+        variants["cd_f32_last"] = float(cd32[-1])
+        n16 = min(len(fh16), len(fh64))
+        t16 = slice(max(0, n16 - 200), n16)
+        variants["cd_err_f16s_vs_f64"] = float(abs(fh16[t16, 4].mean() - fh64[t16, 4].mean()) /
+                                                abs(fh64[t16, 4].mean()))
+        for name, st, force in (("periodic256_f32_F0", cf.STORE_F32, None),
+                                ("periodic256_f32_F1e-6", cf.STORE_F32, (1e-6, 0.0, 0.0)),
+                                ("periodic256_f64_F0", cf.STORE_F64, None),
+                                ("periodic256_f16s_F0", cf.STORE_F16S, None)):
+            case = periodic_case(st, force)
+            pinit = cf.random_init(case, seed=20261202)
+            log(f"[bench] {name}: {vs} steps")
+            r, _ = run_level(case, pinit, 50, vs)
+            variants[name] = r
+        copy_ms = native.load_probe(2 << 30, 20) if hasattr(native, "load_probe") else None
+        if copy_ms:
+            variants["copy_probe_gbs"] = 2 * (2 << 30) / (copy_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"[bench] CPU oracle: {args.cpu_steps} steps")
+        cpu = cpu_baseline(args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": round(head["mlups"], 2),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(head["ms_per_step"], 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (uniform inflow initial state, no dataset)",
+        "config": {
+            "workload": "D3Q19 BGK flow past a fixed sphere, 256x128x128 uniform lattice, D=24 at "
+                        "(64,64,64), 1810 markers, Re=100, U=0.04, explicit direct forcing (loop_ns=1, "
+                        "4-point delta), Guo forcing, ext_dir inflow / foextrap outflow, periodic y,z; "
+                        "FP32 population storage, FP32 collision, FP64 IB and force sums",
+            "cells": c32.ncells,
+            "markers": 1810,
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_step<D3Q19,StF32,padded>",
+            "achieved": round(head["kernel_gbs"], 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(head["kernel_frac"], 4),
+            "traffic": read_traffic(),
+            "algorithmic_bytes_per_launch": head["bytes_per_launch"],
+            "avg_launch_ms": round(head["kernel_ms"], 5),
+        },
+        "cpu_baseline": cpu,
+        "variants": variants,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

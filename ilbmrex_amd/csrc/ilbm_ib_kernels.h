@@ -1,0 +1,536 @@
+// Immersed-boundary kernels of the resident level (FP64 throughout).
+//
+// They follow IBMarkerDF::update_forcing (Source/IBM/IBMarkerDF.cpp:1522-1917)
+// stage by stage, restricted to a band box around the markers (the
+// reference allocates and sweeps the whole level every step,
+// IBMarkerDF.cpp:1532-1563, 1878-1912; outside the band every term is zero).
+//
+//   k_ib_begin     EulerVel = u(macro_old), EulerForceTotal = 0     (:1532-1566)
+//   k_ib_weights   marker cell, 5-point delta weights per dimension (:70-99)
+//   k_ib_interp    U_m = sum delta * EulerVel * dV                  (:611-739)
+//   k_ib_force_*   explicit / explicit-diag / IVC direct forcing    (:741-873, :1144-1328)
+//   k_ib_spread    EulerForceIter = sum_m delta * F_m * dv          (:1330-1512)
+//   k_ib_sums      sum_m F_m dv (marker order), moments r x F dv    (:1779-1819, :1500-1503)
+//   k_ib_correct   EulerVel += dt * Iter, Total += Iter             (:1514-1520, :1878-1879)
+//   k_ib_export    F = dt * rho * Total                             (:1889-1912)
+//
+// Spreading is a deterministic gather: every band cell walks a precomputed
+// list of (marker, stencil point) contributions in marker order, so the
+// per-cell sums are evaluated in the same order as the reference's serial
+// particle loop and there are no atomics.  Contributions that land in a
+// periodic ghost image are summed per image and added afterwards, as
+// SumBoundary does (:1511).
+#pragma once
+
+#include "ilbm_lbm_kernels.h"
+
+namespace ilbm {
+
+struct IbArgs {
+  int dim;
+  int n[3];
+  int periodic[3];
+  double plo[3], dx[3];
+  BcSpec bc;
+  int band_lo[3], band_n[3];
+  int64_t band_cells;
+  int nm;
+  double dt, dv;
+  int kernel_type;      // 0: 4-point, 1: 3-point
+  int renorm_interp;    // renormalize_delta && coupling != IVC
+  int renorm_spread;    // renormalize_delta
+  int coupling;         // 0 explicit, 1 IVC, 2 explicit-diag
+  double relax, diag_eps;
+  int geometry;         // 0 rigid, 1 box, 2 user
+  double x0, y0, z0;
+  double ub[3], om[3];
+  double box_yhi, wall_tol, lid_u[3];
+  // marker state
+  const double* xm;     // [3][nm] positions
+  const double* ut;     // [3][nm] target velocity (user geometry)
+  double* U;            // [3][nm]
+  double* F;            // [3][nm]
+  double* Mm;           // [3][nm] moments
+  int* base;            // [3][nm]
+  double* wts;          // [3][5][nm]
+  double* scale;        // [2][nm] interp / spread renormalisation
+  // band fields
+  const double* Mband;  // [4][band]
+  double* EV;           // [3][band]
+  double* EFI;          // [3][band]
+  double* EFT;          // [3][band]
+  double* Fband;        // [3][band]
+  // spreading lists
+  const int* rowptr;    // [band+1]
+  const int* ent_m;
+  const int* ent_code;
+  double* ent_w;        // per-entry delta weight
+  // IVC
+  const double* Ainv;   // [nm][nm]
+  double* rhs;          // [3][nm]
+  // reductions
+  double* sums;         // [0..2] last marker force sum, [3..5] torque
+};
+
+__device__ inline double delta_fn(double xf, double xp, double h, int type) {
+  const double rr = fabs(xf - xp) / h;
+  double value = 0.0;
+  if (type == 0) {
+    if (rr < 1.0) value = 0.125 * (3.0 - 2.0 * rr + sqrt(1.0 + 4.0 * rr - 4.0 * rr * rr));
+    else if (rr < 2.0) value = 0.125 * (5.0 - 2.0 * rr - sqrt(-7.0 + 12.0 * rr - 4.0 * rr * rr));
+  } else {
+    if (rr < 0.5) value = (1.0 + sqrt(1.0 - 3.0 * rr * rr)) / 3.0;
+    else if (rr < 1.5) value = (5.0 - 3.0 * rr - sqrt(-2.0 + 6.0 * rr - 3.0 * rr * rr)) / 6.0;
+  }
+  return value / h;
+}
+
+__device__ inline bool band_index(const IbArgs& a, const int c[3], int64_t* b) {
+  const int bi = c[0] - a.band_lo[0], bj = c[1] - a.band_lo[1], bk = c[2] - a.band_lo[2];
+  if ((unsigned)bi >= (unsigned)a.band_n[0] || (unsigned)bj >= (unsigned)a.band_n[1] ||
+      (unsigned)bk >= (unsigned)a.band_n[2])
+    return false;
+  *b = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
+  return true;
+}
+
+// FillPatchMacro value (macroBcFill, bc_fill.H:263-301, plus FilccCell and
+// periodic images) of macro component comp at any cell, from macro_old.
+__device__ double macro_fill(const IbArgs& a, const int iv[3], int comp) {
+  int c[3] = {iv[0], iv[1], iv[2]};
+  for (int iter = 0; iter < 8; ++iter) {
+    int lo_hit[3] = {0, 0, 0}, hi_hit[3] = {0, 0, 0};
+    bool any_out = false, nonper_out = false;
+    for (int d = 0; d < a.dim; ++d) {
+      if (c[d] < 0) { lo_hit[d] = 1; any_out = true; if (!a.periodic[d]) nonper_out = true; }
+      else if (c[d] > a.n[d] - 1) { hi_hit[d] = 1; any_out = true; if (!a.periodic[d]) nonper_out = true; }
+    }
+    if (!any_out) {
+      int64_t b;
+      if (!band_index(a, c, &b)) return 0.0;
+      return a.Mband[comp * a.band_cells + b];
+    }
+    if (!nonper_out) {
+      for (int d = 0; d < a.dim; ++d) {
+        if (c[d] < 0) c[d] += a.n[d];
+        else if (c[d] > a.n[d] - 1) c[d] -= a.n[d];
+      }
+      continue;
+    }
+    int bd = -1, lo_side = 1, kind = 0;
+    for (int d = 0; d < a.dim && kind == 0; ++d) {
+      if (lo_hit[d] && a.bc.lo[d] == BC_USER_2) { kind = 2; bd = d; lo_side = 1; }
+      else if (hi_hit[d] && a.bc.hi[d] == BC_USER_2) { kind = 2; bd = d; lo_side = 0; }
+    }
+    for (int d = 0; d < a.dim && kind == 0; ++d) {
+      if (lo_hit[d] && a.bc.lo[d] == BC_USER_1) { kind = 1; bd = d; lo_side = 1; }
+      else if (hi_hit[d] && a.bc.hi[d] == BC_USER_1) { kind = 1; bd = d; lo_side = 0; }
+    }
+    for (int d = 0; d < a.dim && kind == 0; ++d) {
+      if (lo_hit[d] && a.bc.lo[d] == BC_EXT_DIR) { kind = 1; bd = d; lo_side = 1; }
+      else if (hi_hit[d] && a.bc.hi[d] == BC_EXT_DIR) { kind = 1; bd = d; lo_side = 0; }
+    }
+    if (kind != 0) {
+      const double* bv = lo_side ? a.bc.lo_val[bd] : a.bc.hi_val[bd];
+      if (kind == 1) return bv[comp];
+      int mir[3] = {c[0], c[1], c[2]};
+      for (int d = 0; d < a.dim; ++d) {
+        if (mir[d] < 0) mir[d] = 0;
+        else if (mir[d] > a.n[d] - 1) mir[d] = a.n[d] - 1;
+      }
+      int64_t b;
+      if (!band_index(a, mir, &b)) return 0.0;
+      const double m = a.Mband[comp * a.band_cells + b];
+      if (comp == 0) return fmax(1.0e-20, 2.0 * bv[0] - m);
+      return 2.0 * bv[comp] - m;
+    }
+    int dsel = -1;
+    for (int d = 0; d < a.dim; ++d)
+      if ((lo_hit[d] && a.bc.lo[d] == BC_FOEXTRAP) || (hi_hit[d] && a.bc.hi[d] == BC_FOEXTRAP)) dsel = d;
+    if (dsel < 0) return 0.0;
+    c[dsel] = lo_hit[dsel] ? 0 : a.n[dsel] - 1;
+  }
+  return 0.0;
+}
+
+// EulerVel at any cell of the 2-ghost fab: corrected values inside the
+// domain and at periodic images, FillPatch values at physical ghosts.
+__device__ inline double ev_value(const IbArgs& a, const int p[3], int comp) {
+  int c[3] = {p[0], p[1], p[2]};
+  bool phys = false;
+  for (int d = 0; d < a.dim; ++d) {
+    if (c[d] < 0 || c[d] > a.n[d] - 1) {
+      if (a.periodic[d]) c[d] = (c[d] < 0) ? c[d] + a.n[d] : c[d] - a.n[d];
+      else phys = true;
+    }
+  }
+  if (phys) return macro_fill(a, p, comp + 1);
+  int64_t b;
+  if (!band_index(a, c, &b)) return 0.0;
+  return a.EV[comp * a.band_cells + b];
+}
+
+__global__ void k_ib_begin(IbArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b == 0) for (int c = 0; c < 6; ++c) a.sums[c] = 0.0;
+  if (b >= a.band_cells) return;
+  a.EV[b] = a.Mband[a.band_cells + b];
+  a.EV[a.band_cells + b] = a.Mband[2 * a.band_cells + b];
+  a.EV[2 * a.band_cells + b] = (a.dim == 3) ? a.Mband[3 * a.band_cells + b] : 0.0;
+  for (int c = 0; c < 3; ++c) a.EFT[c * a.band_cells + b] = 0.0;
+}
+
+// cell index and 5-point weights of every marker, and the partition sums
+// that the optional renormalisation uses (:640-690, :1358-1403)
+__global__ void k_ib_weights(IbArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nm) return;
+  const int nm = a.nm;
+  double t[3][5];
+  int b[3] = {0, 0, 0};
+  for (int d = 0; d < 3; ++d) {
+    if (d < a.dim) {
+      const double pos = a.xm[d * nm + m];
+      b[d] = (int)floor((pos - a.plo[d]) / a.dx[d]);
+      for (int s = 0; s < 5; ++s) {
+        const double xi = a.plo[d] + ((double)(b[d] + s - 2) + 0.5) * a.dx[d];
+        t[d][s] = delta_fn(xi, pos, a.dx[d], a.kernel_type);
+      }
+    } else {
+      for (int s = 0; s < 5; ++s) t[d][s] = 1.0;
+    }
+    a.base[d * nm + m] = b[d];
+    for (int s = 0; s < 5; ++s) a.wts[(d * 5 + s) * nm + m] = t[d][s];
+  }
+  const double dvol = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+  double part = 0.0;
+  for (int ii = 0; ii < 5; ++ii)
+    for (int jj = 0; jj < 5; ++jj) {
+      if (a.dim == 3) {
+        for (int kk = 0; kk < 5; ++kk) {
+          const int ip = b[0] + ii - 2, jp = b[1] + jj - 2, kp = b[2] + kk - 2;
+          if (ip < -2 || ip > a.n[0] + 1 || jp < -2 || jp > a.n[1] + 1 || kp < -2 || kp > a.n[2] + 1) continue;
+          part += t[0][ii] * t[1][jj] * t[2][kk] * dvol;
+        }
+      } else {
+        const int ip = b[0] + ii - 2, jp = b[1] + jj - 2;
+        if (ip < -2 || ip > a.n[0] + 1 || jp < -2 || jp > a.n[1] + 1) continue;
+        part += t[0][ii] * t[1][jj] * dvol;
+      }
+    }
+  a.scale[m] = part;                                              // interpolation divisor
+  a.scale[nm + m] = (a.renorm_spread && part > 0.0) ? 1.0 / part : 1.0;  // spreading factor
+}
+
+// One wavefront per marker: the lanes evaluate the stencil products in
+// parallel, then four lanes accumulate them in the reference's (ii, jj, kk)
+// order, so the sums are bit-identical to the serial loop (:642-681).
+constexpr int kInterpWaves = 4;
+__global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) {
+  __shared__ double prod[kInterpWaves][4][128];
+  __shared__ unsigned char valid[kInterpWaves][128];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m = blockIdx.x * kInterpWaves + wv;
+  const bool active = m < a.nm;
+  const int nm = a.nm;
+  const double d = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+  const int b0 = active ? a.base[m] : 0, b1 = active ? a.base[nm + m] : 0;
+  const int b2 = active ? a.base[2 * nm + m] : 0;
+  const int npts = (a.dim == 3) ? 125 : 25;
+  for (int t = lane; active && t < npts; t += 64) {
+    int ii, jj, kk;
+    if (a.dim == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
+    else { ii = t / 5; jj = t % 5; kk = 0; }
+    const int p[3] = {b0 + ii - 2, b1 + jj - 2, (a.dim == 3) ? b2 + kk - 2 : 0};
+    bool ok = !(p[0] < -2 || p[0] > a.n[0] + 1 || p[1] < -2 || p[1] > a.n[1] + 1);
+    if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
+    valid[wv][t] = ok ? 1 : 0;
+    if (!ok) continue;
+    const double tU = a.wts[(0 * 5 + ii) * nm + m];
+    const double tV = a.wts[(1 * 5 + jj) * nm + m];
+    const double dval = (a.dim == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
+    prod[wv][0][t] = dval * d;
+    prod[wv][1][t] = dval * ev_value(a, p, 0) * d;
+    prod[wv][2][t] = dval * ev_value(a, p, 1) * d;
+    prod[wv][3][t] = (a.dim == 3) ? dval * ev_value(a, p, 2) * d : 0.0;
+  }
+  __syncthreads();
+  if (active && lane < 4) {
+    double acc = 0.0;
+    for (int t = 0; t < npts; ++t)
+      if (valid[wv][t]) acc += prod[wv][lane][t];
+    prod[wv][lane][127] = acc;
+  }
+  __syncthreads();
+  if (active && lane == 0) {
+    const double part = prod[wv][0][127];
+    double u = prod[wv][1][127], v = prod[wv][2][127], w = prod[wv][3][127];
+    if (a.dim != 3) w = 0.0;
+    if (a.renorm_interp && part > 0.0) { u /= part; v /= part; w /= part; }
+    // update_lagrangian_marker zeroes U and F before every iteration (:599-604)
+    double U0 = 0.0, U1 = 0.0, U2 = 0.0;
+    U0 += u; U1 += v; U2 += w;
+    a.U[m] = U0; a.U[nm + m] = U1; a.U[2 * nm + m] = U2;
+    a.F[m] = 0.0; a.F[nm + m] = 0.0; a.F[2 * nm + m] = 0.0;
+  }
+}
+
+__device__ inline void target_velocity(const IbArgs& a, int m, double* ux, double* uy, double* uz) {
+  const int nm = a.nm;
+  if (a.geometry == 2) {
+    *ux = a.ut[m]; *uy = a.ut[nm + m]; *uz = a.ut[2 * nm + m];
+    return;
+  }
+  const double x = a.xm[m], y = a.xm[nm + m], z = (a.dim == 3) ? a.xm[2 * nm + m] : 0.0;
+  if (a.geometry == 1) {  // box_target_velocity (:147-161)
+    *ux = *uy = *uz = 0.0;
+    if (fabs(y - a.box_yhi) <= a.wall_tol) { *ux = a.lid_u[0]; *uy = a.lid_u[1]; *uz = a.lid_u[2]; }
+    return;
+  }
+  const double z0 = (a.dim == 3) ? a.z0 : 0.0;  // rigid_target_velocity (:163-182)
+  const double rx = x - a.x0, ry = y - a.y0, rz = z - z0;
+  const double cx = a.om[1] * rz - a.om[2] * ry;
+  const double cy = a.om[2] * rx - a.om[0] * rz;
+  const double cz = a.om[0] * ry - a.om[1] * rx;
+  *ux = a.ub[0] + cx; *uy = a.ub[1] + cy; *uz = a.ub[2] + cz;
+}
+
+__global__ void k_ib_force_explicit(IbArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nm) return;
+  const int nm = a.nm;
+  double Ux, Uy, Uz;
+  target_velocity(a, m, &Ux, &Uy, &Uz);
+  double denom = a.dt;
+  if (a.coupling == 2) {
+    const double dV = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+    const int b0 = a.base[m], b1 = a.base[nm + m], b2 = a.base[2 * nm + m];
+    double s2 = 0.0;
+    for (int ii = 0; ii < 5; ++ii) {
+      const int ip = b0 + ii - 2;
+      if (ip < 0 || ip > a.n[0] - 1) continue;
+      const double tU = a.wts[(0 * 5 + ii) * nm + m];
+      for (int jj = 0; jj < 5; ++jj) {
+        const int jp = b1 + jj - 2;
+        if (jp < 0 || jp > a.n[1] - 1) continue;
+        const double tV = a.wts[(1 * 5 + jj) * nm + m];
+        if (a.dim == 3) {
+          for (int kk = 0; kk < 5; ++kk) {
+            const int kp = b2 + kk - 2;
+            if (kp < 0 || kp > a.n[2] - 1) continue;
+            const double tW = a.wts[(2 * 5 + kk) * nm + m];
+            const double dval = tU * tV * tW;
+            s2 += dval * dval * dV;
+          }
+        } else {
+          const double dval = tU * tV;
+          s2 += dval * dval * dV;
+        }
+      }
+    }
+    const double mkk = a.dv * s2;
+    denom = a.dt * fmax(mkk, a.diag_eps);
+  }
+  a.F[m] += a.relax * (Ux - a.U[m]) / denom;
+  a.F[nm + m] += a.relax * (Uy - a.U[nm + m]) / denom;
+  a.F[2 * nm + m] += a.relax * (Uz - a.U[2 * nm + m]) / denom;
+}
+
+// IVC right-hand side b = (U_target - U) / dt (:1276-1296)
+__global__ void k_ib_ivc_rhs(IbArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nm) return;
+  const int nm = a.nm;
+  double Ux, Uy, Uz;
+  target_velocity(a, m, &Ux, &Uy, &Uz);
+  a.rhs[m] = (Ux - a.U[m]) / a.dt;
+  a.rhs[nm + m] = (Uy - a.U[nm + m]) / a.dt;
+  a.rhs[2 * nm + m] = (Uz - a.U[2 * nm + m]) / a.dt;
+}
+
+// F += A^{-1} b, one wavefront per marker row (fixed reduction tree).
+__global__ void __launch_bounds__(256) k_ib_ivc_matvec(IbArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  if (row >= a.nm) return;
+  const int nm = a.nm;
+  const double* Ar = a.Ainv + (int64_t)row * nm;
+  double sx = 0.0, sy = 0.0, sz = 0.0;
+  for (int j = lane; j < nm; j += 64) {
+    const double aij = Ar[j];
+    sx += aij * a.rhs[j];
+    sy += aij * a.rhs[nm + j];
+    sz += aij * a.rhs[2 * nm + j];
+  }
+  sx = wave_sum(sx); sy = wave_sum(sy); sz = wave_sum(sz);
+  if (lane == 0) {
+    a.F[row] += sx; a.F[nm + row] += sy; a.F[2 * nm + row] += sz;
+  }
+}
+
+// weight of every spreading entry: ((tU*tV)*tW)*scale as in :1426 / :1435
+__global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nent) return;
+  const int nm = a.nm;
+  const int code = a.ent_code[e];
+  const int m = a.ent_m[e];
+  const int ii = code & 7, jj = (code >> 3) & 7, kk = (code >> 6) & 7;
+  const double tU = a.wts[(0 * 5 + ii) * nm + m];
+  const double tV = a.wts[(1 * 5 + jj) * nm + m];
+  const double sc = a.scale[nm + m];
+  a.ent_w[e] = (a.dim == 3) ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
+}
+
+__global__ void k_ib_spread(IbArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.band_cells) return;
+  const int nm = a.nm;
+  const int e0 = a.rowptr[b], e1 = a.rowptr[b + 1];
+  double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
+  int cur = 0;
+  for (int e = e0; e < e1; ++e) {
+    const int grp = a.ent_code[e] >> 9;
+    if (grp != cur) {
+      if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+      gs[0] = gs[1] = gs[2] = 0.0;
+      cur = grp;
+    }
+    const int m = a.ent_m[e];
+    const double dval = a.ent_w[e];
+    const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
+    double* acc = (grp == 0) ? tot : gs;
+    acc[0] += dval * fx;
+    acc[1] += dval * fy;
+    if (a.dim == 3) acc[2] += dval * fz;
+  }
+  if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+  a.EFI[b] = tot[0];
+  a.EFI[a.band_cells + b] = tot[1];
+  a.EFI[2 * a.band_cells + b] = tot[2];
+}
+
+// marker force integral in marker order (:1781-1819) and moments (:1500-1503).
+// Products are formed in parallel and staged through LDS; three lanes then
+// accumulate them serially in marker order, as the reference's host loop.
+constexpr int kSumChunk = 1024;
+__global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) {
+  __shared__ double st[6][kSumChunk];
+  const int nm = a.nm;
+  const int t = threadIdx.x;
+  double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  const double z0 = (a.dim == 3) ? a.z0 : 0.0;
+  for (int c0 = 0; c0 < nm; c0 += kSumChunk) {
+    const int m = c0 + t;
+    if (m < nm) {
+      const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
+      const double rx = a.xm[m] - a.x0, ry = a.xm[nm + m] - a.y0;
+      const double rz = (a.dim == 3) ? a.xm[2 * nm + m] - z0 : 0.0;
+      const double mx = ry * fz - rz * fy, my = rz * fx - rx * fz, mz = rx * fy - ry * fx;
+      a.Mm[m] = mx; a.Mm[nm + m] = my; a.Mm[2 * nm + m] = mz;
+      st[0][t] = fx; st[1][t] = fy; st[2][t] = fz;
+      st[3][t] = mx; st[4][t] = my; st[5][t] = mz;
+    }
+    __syncthreads();
+    if (t < 6) {
+      const int cnt = (nm - c0 < kSumChunk) ? nm - c0 : kSumChunk;
+      double s = acc[t];
+      int r = 0;
+      for (; r + 8 <= cnt; r += 8) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = st[t][r + u];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) s += v[u];
+      }
+      for (; r < cnt; ++r) s += st[t][r];
+      acc[t] = s;
+    }
+    __syncthreads();
+  }
+  if (t < 3) a.sums[t] += acc[t];
+  else if (t < 6) a.sums[t] = acc[t];
+}
+
+__global__ void k_ib_correct(IbArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.band_cells) return;
+  for (int c = 0; c < 3; ++c) {
+    const int64_t o = c * a.band_cells + b;
+    a.EV[o] += a.dt * a.EFI[o];
+    a.EFT[o] += a.EFI[o];
+  }
+}
+
+__global__ void k_ib_export(IbArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.band_cells) return;
+  const double rho = a.Mband[b];
+  for (int c = 0; c < 3; ++c) a.Fband[c * a.band_cells + b] = a.dt * rho * a.EFT[c * a.band_cells + b];
+}
+
+// Sum of the exported force over the band (forcing[lev].sum, AmrCoreLBM.cpp:491-494)
+// and the ComputeIBForce record (:456-631), appended to a device history.
+struct ForceRecArgs {
+  const double* Fband;
+  int64_t band_cells;
+  const double* sums;   // marker sums [0..2]
+  double* state;        // [0] have_prev, [1..3] prev body force
+  double* hist;         // rows of ncol doubles
+  int row, ncol, dim, method;
+  double time, step, dV, dt, U0, R;
+};
+
+__global__ void __launch_bounds__(1024) k_force_record(ForceRecArgs a) {
+  __shared__ double lds[3][16];
+  double s[3] = {0.0, 0.0, 0.0};
+  for (int64_t b = threadIdx.x; b < a.band_cells; b += blockDim.x)
+    for (int c = 0; c < 3; ++c) s[c] += a.Fband[c * a.band_cells + b];
+  for (int c = 0; c < 3; ++c) {
+    const double r = wave_sum(s[c]);
+    if ((threadIdx.x & 63) == 0) lds[c][threadIdx.x >> 6] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double sum[3] = {0.0, 0.0, 0.0};
+  for (int c = 0; c < 3; ++c)
+    for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) sum[c] += lds[c][wv];
+  if (a.dim != 3) sum[2] = 0.0;
+  const double Fxe = -(sum[0] * a.dV) / a.dt, Fye = -(sum[1] * a.dV) / a.dt, Fze = -(sum[2] * a.dV) / a.dt;
+  const double Fxm = -a.sums[0], Fym = -a.sums[1], Fzm = -a.sums[2];
+  const double Fxx = Fxe, Fyx = Fye, Fzx = Fze;
+  double Fxb = Fxe, Fyb = Fye, Fzb = Fze;
+  if (a.method == 1) { Fxb = Fxm; Fyb = Fym; Fzb = Fzm; }
+  else if (a.method == 2) { Fxb = Fxx; Fyb = Fyx; Fzb = Fzx; }
+  double Fxt = Fxb, Fyt = Fyb, Fzt = Fzb;
+  if (a.state[0] != 0.0) {
+    Fxt = 0.5 * (a.state[1] + Fxb); Fyt = 0.5 * (a.state[2] + Fyb); Fzt = 0.5 * (a.state[3] + Fzb);
+  }
+  a.state[0] = 1.0; a.state[1] = Fxb; a.state[2] = Fyb; a.state[3] = Fzb;
+  const double D = 2.0 * a.R;
+  const double pi = 3.14159265358979323846;
+  const double denom = (a.dim == 3) ? 0.5 * 1.0 * a.U0 * a.U0 * (pi * a.R * a.R) : 0.5 * 1.0 * a.U0 * a.U0 * D;
+  double C[15];
+  for (int t = 0; t < 15; ++t) C[t] = 0.0;
+  if (denom > 0.0) {
+    C[0] = Fxt / denom; C[1] = Fyt / denom; C[2] = Fxb / denom; C[3] = Fyb / denom;
+    C[4] = Fxe / denom; C[5] = Fye / denom; C[6] = Fxm / denom; C[7] = Fym / denom;
+    C[8] = Fxx / denom; C[9] = Fyx / denom;
+    C[10] = Fzt / denom; C[11] = Fzb / denom; C[12] = Fze / denom; C[13] = Fzm / denom; C[14] = Fzx / denom;
+  }
+  const double tconv = (D > 0.0) ? (a.time * a.U0 / D) : 0.0;
+  double* r = a.hist + (int64_t)a.row * a.ncol;
+  int n = 0;
+  r[n++] = a.step; r[n++] = a.time;
+  r[n++] = Fxt; r[n++] = Fyt; r[n++] = C[0]; r[n++] = C[1];
+  r[n++] = Fxb; r[n++] = Fyb; r[n++] = C[2]; r[n++] = C[3];
+  r[n++] = Fxe; r[n++] = Fye; r[n++] = C[4]; r[n++] = C[5];
+  r[n++] = Fxm; r[n++] = Fym; r[n++] = C[6]; r[n++] = C[7];
+  r[n++] = Fxx; r[n++] = Fyx; r[n++] = C[8]; r[n++] = C[9]; r[n++] = tconv;
+  if (a.dim == 3) {
+    r[n++] = Fzt; r[n++] = C[10]; r[n++] = Fzb; r[n++] = C[11]; r[n++] = Fze; r[n++] = C[12];
+    r[n++] = Fzm; r[n++] = C[13]; r[n++] = Fzx; r[n++] = C[14];
+  }
+}
+
+}  // namespace ilbm

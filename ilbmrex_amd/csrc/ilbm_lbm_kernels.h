@@ -1,0 +1,651 @@
+// Fused lattice kernels of the resident MI355X level (HBM-bound, no MFMA).
+//
+// Storage.  Populations are kept POST-COLLISION, structure of arrays
+// [q][z][y][x] with x fastest.  One step of the reference
+// (Source/Core/AdvanceLBMAtLevel.cpp:223-386: collide, FillBoundary,
+// bounce-back, pull-stream, forced macros) then becomes one pass over
+// memory:
+//
+//   k_step:  f^{n+1}(x,q) = g^n(x - c_q, q)                 (pull, stream.H:9-22)
+//            rho,u = moments(f^{n+1}) + F^{n+1}/2           (calculateMacroForcing.H:9-32)
+//            g^{n+1} = f - (f - feq(rho,u))/tau [+ Guo]     (collide.H, collide_forced.H)
+//
+// reading and writing each population once (2*Q*sizeof(pop) bytes per
+// lattice update).  The immersed-boundary force of the next step is only
+// known after the IB pass, so on IB cells the Guo term is added afterwards
+// by k_guo_fixup; the reference adds it after the BGK update as well
+// (collide_forced.H:20-50), so FP64 results are unchanged.
+//
+// Ghosts.  When every dimension is periodic the arrays carry no ghost
+// cells and k_step wraps indices.  Otherwise each dimension carries one
+// ghost layer; k_fill_ghosts writes it between steps with the values the
+// reference's stream reads there: periodic images of post-collision data
+// (FillBoundary, AdvanceLBMAtLevel.cpp:244), halfway bounce-back for user_1
+// walls (:265-373), and the pre-collision FillPatch values of
+// mesoscopicBcFill / foextrap for everything else (bc_fill.H:303-347).  The
+// pre-collision values of the domain's outer layer come from a small face
+// buffer that k_step writes while it has f^{n+1} in registers.
+#pragma once
+
+#include "ilbm_lattice.h"
+
+namespace ilbm {
+
+constexpr int kBlock = 256;
+
+struct GridGeom {
+  int n[3];        // valid extents (n[2] == 1 in 2D)
+  int pad;         // ghost layers stored per dimension (0 or 1)
+  int dim;
+  int64_t js, ks;  // strides of the stored grid
+  int64_t qs;      // elements per population plane
+  int64_t off0;    // offset of valid cell (0,0,0)
+  __host__ __device__ int64_t idx(int i, int j, int k) const {
+    return off0 + (int64_t)k * ks + (int64_t)j * js + i;
+  }
+};
+
+// Face buffers of the pre-collision boundary layer: face f = 2*d + side.
+struct FaceBuf {
+  void* p[6];
+  int dims[6][2];  // the two in-plane dims of each face (-1 if unused)
+};
+
+__host__ __device__ inline int64_t face_index(const GridGeom& g, int d, int i, int j, int k) {
+  const int c[3] = {i, j, k};
+  if (d == 0) return (int64_t)c[2] * g.n[1] + c[1];
+  if (d == 1) return (int64_t)c[2] * g.n[0] + c[0];
+  return (int64_t)c[1] * g.n[0] + c[0];
+}
+
+struct StepArgs {
+  GridGeom g;
+  const void* gin;
+  void* gout;
+  double tau, inv_tau;
+  double Fmac[3];        // uniform F^{n+1} used by the macros outside the IB band
+  double Fcol[3];        // uniform F^{n+2} used by the collision (no IB)
+  int band_lo[3], band_n[3];
+  const double* Fband;   // [3][band cells] exported IB force
+  double* Mband;         // [4][band cells] rho,u,v,w written for the IB
+  int64_t band_cells;
+  double* mout;          // optional [4][cells] macro export (double)
+  double* fout;          // optional [Q][cells] f^{n+1} export (double)
+  FaceBuf faces;         // pre-collision boundary layer (non-periodic faces)
+  int nonper[3];
+  double* partial;       // per-block mass partials
+};
+
+// ---------------------------------------------------------------------------
+// Deterministic block reduction (fixed tree for a fixed block shape).
+// ---------------------------------------------------------------------------
+__device__ inline double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  return v;
+}
+
+__device__ inline double block_sum(double v, double* lds) {
+  const int t = threadIdx.x + threadIdx.y * blockDim.x;
+  const int nt = blockDim.x * blockDim.y;
+  v = wave_sum(v);
+  if ((t & 63) == 0) lds[t >> 6] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (t == 0) {
+    for (int wv = 0; wv < (nt + 63) / 64; ++wv) s += lds[wv];
+  }
+  __syncthreads();
+  return s;
+}
+
+// Equilibrium as feqFunction (Source/LBM/feq.H:9-29): the 2D form omits w.
+template <class LAT, class T>
+__device__ inline T feq_q(int q, T rho, T u, T v, T w, T uv) {
+  const T cx = T(LAT::cx[q]), cy = T(LAT::cy[q]), cz = T(LAT::cz[q]);
+  T eu;
+  if constexpr (LAT::D == 3) eu = cx * u + cy * v + cz * w;
+  else eu = cx * u + cy * v;
+  return T(LAT::w[q]) * rho * (T(1.0) + T(3.0) * eu + T(4.5) * eu * eu - T(1.5) * uv);
+}
+
+template <class LAT, class T>
+__device__ inline T usq(T u, T v, T w) {
+  if constexpr (LAT::D == 3) return u * u + v * v + w * w;
+  else return u * u + v * v;
+}
+
+// Guo forcing increment of collide_forced.H:27-50.  FP64 keeps the
+// reference's divisions by cs2; the FP32 collision multiplies by 1/cs2 = 3.
+template <class LAT, class T>
+__device__ inline T guo_q(int q, T fac, T ux, T uy, T uz, T fx, T fy, T fz) {
+  const T cx = T(LAT::cx[q]), cy = T(LAT::cy[q]), cz = T(LAT::cz[q]);
+  const T eiu = cx * ux + cy * uy + cz * uz;
+  T termx, termy, termz;
+  if constexpr (sizeof(T) == 8) {
+    const T cs2 = T(1.0) / T(3.0);
+    termx = (cx - ux) / cs2 + (eiu / (cs2 * cs2)) * cx;
+    termy = (cy - uy) / cs2 + (eiu / (cs2 * cs2)) * cy;
+    if constexpr (LAT::D == 3) termz = (cz - uz) / cs2 + (eiu / (cs2 * cs2)) * cz;
+    else termz = T(0.0);
+  } else {
+    const T e9 = eiu * T(9.0);
+    termx = (cx - ux) * T(3.0) + e9 * cx;
+    termy = (cy - uy) * T(3.0) + e9 * cy;
+    if constexpr (LAT::D == 3) termz = (cz - uz) * T(3.0) + e9 * cz;
+    else termz = T(0.0);
+  }
+  return T(LAT::w[q]) * fac * (termx * fx + termy * fy + termz * fz);
+}
+
+// u = (m + F/2) / rho: FP64 divides as the reference; FP32 multiplies by
+// the hardware reciprocal.
+template <class ST>
+__device__ inline void forced_velocity(typename ST::compute_t r, typename ST::compute_t m1,
+                                       typename ST::compute_t m2, typename ST::compute_t m3,
+                                       typename ST::compute_t fx, typename ST::compute_t fy,
+                                       typename ST::compute_t fz, typename ST::compute_t& u,
+                                       typename ST::compute_t& v, typename ST::compute_t& w) {
+  using T = typename ST::compute_t;
+  if constexpr (ST::exact) {
+    u = (m1 + T(0.5) * fx) / r;
+    v = (m2 + T(0.5) * fy) / r;
+    w = (m3 + T(0.5) * fz) / r;
+  } else {
+    const T ri = __builtin_amdgcn_rcpf(r);
+    u = (m1 + T(0.5) * fx) * ri;
+    v = (m2 + T(0.5) * fy) * ri;
+    w = (m3 + T(0.5) * fz) * ri;
+  }
+}
+
+// BGK relaxation of collide.H:17-24; FP64 keeps the reference's divide.
+template <class ST>
+__device__ inline typename ST::compute_t bgk(typename ST::compute_t f, typename ST::compute_t feq,
+                                             double tau, double inv_tau) {
+  using T = typename ST::compute_t;
+  if constexpr (ST::exact) return 1.0 * f - (f - feq) / tau;
+  else return f - (f - feq) * T(inv_tau);
+}
+
+// ---------------------------------------------------------------------------
+// k_step: pull-stream + forced macros + BGK collision, one pass.
+// ---------------------------------------------------------------------------
+template <class LAT, class ST, bool WRAP>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  __shared__ double lds[kBlock / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  const GridGeom& g = a.g;
+  const bool active = (i < g.n[0]) && (j < g.n[1]);
+  double rho_acc = 0.0;
+  if (active) {
+    const R* __restrict__ gin = static_cast<const R*>(a.gin);
+    R* __restrict__ gout = static_cast<R*>(a.gout);
+    // source offsets for c = -1, 0, +1 (source coordinate = x - c)
+    int64_t ox[3], oy[3], oz[3];
+    if constexpr (WRAP) {
+      const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
+      const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
+      ox[0] = xp; ox[1] = i; ox[2] = xm;
+      oy[0] = (int64_t)yp * g.js; oy[1] = (int64_t)j * g.js; oy[2] = (int64_t)ym * g.js;
+      if constexpr (LAT::D == 3) {
+        const int zm = (k == 0) ? g.n[2] - 1 : k - 1, zp = (k == g.n[2] - 1) ? 0 : k + 1;
+        oz[0] = (int64_t)zp * g.ks; oz[1] = (int64_t)k * g.ks; oz[2] = (int64_t)zm * g.ks;
+      } else {
+        oz[0] = oz[1] = oz[2] = 0;
+      }
+    } else {
+      ox[0] = i + 1; ox[1] = i; ox[2] = i - 1;
+      oy[0] = (int64_t)(j + 1) * g.js; oy[1] = (int64_t)j * g.js; oy[2] = (int64_t)(j - 1) * g.js;
+      if constexpr (LAT::D == 3) {
+        oz[0] = (int64_t)(k + 1) * g.ks; oz[1] = (int64_t)k * g.ks; oz[2] = (int64_t)(k - 1) * g.ks;
+      } else {
+        oz[0] = oz[1] = oz[2] = 0;
+      }
+    }
+    T f[LAT::Q];
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const int64_t src = g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+      f[q] = ST::load(gin, (int64_t)q * g.qs + src, T(LAT::w[q]));
+    }
+    // moments in the reference's order (calculateMacroForcing.H:21-27)
+    T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      r += f[q];
+      m1 += T(LAT::cx[q]) * f[q];
+      m2 += T(LAT::cy[q]) * f[q];
+      m3 += T(LAT::cz[q]) * f[q];
+    }
+    const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
+    const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
+                         (unsigned)bk < (unsigned)a.band_n[2];
+    const int64_t bidx = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
+    T Fx = T(a.Fmac[0]), Fy = T(a.Fmac[1]), Fz = T(a.Fmac[2]);
+    if (in_band) {
+      Fx = T(a.Fband[bidx]);
+      Fy = T(a.Fband[a.band_cells + bidx]);
+      Fz = T(a.Fband[2 * a.band_cells + bidx]);
+    }
+    T u, v, w;
+    forced_velocity<ST>(r, m1, m2, m3, Fx, Fy, Fz, u, v, w);
+    rho_acc = (double)r;
+    const int64_t vcell = ((int64_t)k * g.n[1] + j) * g.n[0] + i;
+    const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    if (in_band) {
+      a.Mband[bidx] = (double)r;
+      a.Mband[a.band_cells + bidx] = (double)u;
+      a.Mband[2 * a.band_cells + bidx] = (double)v;
+      a.Mband[3 * a.band_cells + bidx] = (double)w;
+    }
+    if (a.mout) {
+      a.mout[vcell] = (double)r;
+      a.mout[ncell + vcell] = (double)u;
+      a.mout[2 * ncell + vcell] = (double)v;
+      a.mout[3 * ncell + vcell] = (double)w;
+    }
+    if (a.fout) {
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q];
+    }
+    // pre-collision boundary layer for the next ghost fill
+    const int cc[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < LAT::D; ++d) {
+      if (!a.nonper[d]) continue;
+      int side = -1;
+      if (cc[d] == 0) side = 0;
+      else if (cc[d] == g.n[d] - 1) side = 1;
+      if (side < 0) continue;
+      R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
+      int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
+                                                                 : (int64_t)g.n[0] * g.n[1];
+      const int64_t fi = face_index(g, d, i, j, k);
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) ST::store(fb, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
+      // a cell sitting on the lo and hi face of a 1-cell dimension writes both
+      if (g.n[d] == 1) {
+        R* fb2 = static_cast<R*>(a.faces.p[2 * d + 1]);
+#pragma unroll
+        for (int q = 0; q < LAT::Q; ++q) ST::store(fb2, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
+      }
+    }
+    // BGK collision with the macros just computed (feq uses macro_old of the
+    // next step, AdvanceLBMAtLevel.cpp:225-238)
+    const T uv = usq<LAT, T>(u, v, w);
+    const T cfx = T(a.Fcol[0]), cfy = T(a.Fcol[1]), cfz = T(a.Fcol[2]);
+    const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
+    const T fac = T(1.0) - T(0.5) / T(a.tau);
+    const int64_t self = g.off0 + ox[1] + oy[1] + oz[1];
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
+      T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
+      if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+      ST::store_nt(gout, (int64_t)q * g.qs + self, gq, T(LAT::w[q]));
+    }
+  }
+  const double s = block_sum(rho_acc, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// k_init: f^0 = feq(M^0) as InitEquilibrium (AmrCoreLBM.cpp:2146-2161), then
+// the first BGK collision, band macros, boundary layer and exports.
+// ---------------------------------------------------------------------------
+struct InitArgs {
+  StepArgs s;           // gout = state, Fcol, band, faces, exports, partial
+  const double* macro0; // [4][cells] device
+};
+
+template <class LAT, class ST, bool WRAP>
+__global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  __shared__ double lds[kBlock / 64];
+  const StepArgs& a = ia.s;
+  const GridGeom& g = a.g;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.y * blockDim.y + threadIdx.y;
+  const int k = blockIdx.z;
+  double rho_acc = 0.0;
+  if (i < g.n[0] && j < g.n[1]) {
+    const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
+    const int64_t vcell = ((int64_t)k * g.n[1] + j) * g.n[0] + i;
+    const double rho0 = ia.macro0[vcell], u0 = ia.macro0[ncell + vcell];
+    const double v0 = ia.macro0[2 * ncell + vcell], w0 = ia.macro0[3 * ncell + vcell];
+    const double rr = fmax(rho0, 1.0e-12);
+    const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+    T f[LAT::Q];
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
+      const double fe = LAT::w[q] * rr * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+      // stored precision, as a state written in that format would hold it
+      R tmp;
+      ST::store(&tmp, 0, T(fe), T(LAT::w[q]));
+      f[q] = ST::load(&tmp, 0, T(LAT::w[q]));
+    }
+    rho_acc = rho0;
+    const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
+    const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
+                         (unsigned)bk < (unsigned)a.band_n[2];
+    const int64_t bidx = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
+    if (in_band) {
+      a.Mband[bidx] = rho0;
+      a.Mband[a.band_cells + bidx] = u0;
+      a.Mband[2 * a.band_cells + bidx] = v0;
+      a.Mband[3 * a.band_cells + bidx] = w0;
+    }
+    if (a.mout) {
+      a.mout[vcell] = rho0; a.mout[ncell + vcell] = u0;
+      a.mout[2 * ncell + vcell] = v0; a.mout[3 * ncell + vcell] = w0;
+    }
+    if (a.fout) {
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q];
+    }
+    const int cc[3] = {i, j, k};
+#pragma unroll
+    for (int d = 0; d < LAT::D; ++d) {
+      if (!a.nonper[d]) continue;
+      for (int side = 0; side < 2; ++side) {
+        if (cc[d] != (side ? g.n[d] - 1 : 0)) continue;
+        R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
+        int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
+                                                                   : (int64_t)g.n[0] * g.n[1];
+        const int64_t fi = face_index(g, d, i, j, k);
+#pragma unroll
+        for (int q = 0; q < LAT::Q; ++q) ST::store(fb, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
+      }
+    }
+    const T r = T(rho0), u = T(u0), v = T(v0), w = T(w0);
+    const T uv = usq<LAT, T>(u, v, w);
+    const T cfx = T(a.Fcol[0]), cfy = T(a.Fcol[1]), cfz = T(a.Fcol[2]);
+    const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
+    const T fac = T(1.0) - T(0.5) / T(a.tau);
+    R* gout = static_cast<R*>(a.gout);
+    const int64_t self = g.idx(i, j, k);
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
+      T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
+      if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+      ST::store(gout, (int64_t)q * g.qs + self, gq, T(LAT::w[q]));
+    }
+  }
+  const double s = block_sum(rho_acc, lds);
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
+// ---------------------------------------------------------------------------
+// k_guo_fixup: g += Guo(u^n, F^{n+1}) on IB band cells (collide_forced.H:22-50
+// evaluated after the IB force of this step is known).
+// ---------------------------------------------------------------------------
+struct FixupArgs {
+  GridGeom g;
+  void* gbuf;
+  double tau;
+  int band_lo[3], band_n[3];
+  int64_t band_cells;
+  const double* Fband;
+  const double* Mband;
+};
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_guo_fixup(FixupArgs a) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.band_cells) return;
+  const double fx = a.Fband[b], fy = a.Fband[a.band_cells + b], fz = a.Fband[2 * a.band_cells + b];
+  if (fx == 0.0 && fy == 0.0 && fz == 0.0) return;
+  const int bi = (int)(b % a.band_n[0]);
+  const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+  const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+  const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
+  const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
+  const T uz = T(a.Mband[3 * a.band_cells + b]);
+  const T fac = T(1.0) - T(0.5) / T(a.tau);
+  R* gb = static_cast<R*>(a.gbuf);
+  const int64_t self = a.g.idx(i, j, k);
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const int64_t o = (int64_t)q * a.g.qs + self;
+    T gq = ST::load(gb, o, T(LAT::w[q]));
+    gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
+    ST::store(gb, o, gq, T(LAT::w[q]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_fill_ghosts: one ghost layer of the padded layout (see header comment).
+// ---------------------------------------------------------------------------
+enum { BC_INT_DIR = 0, BC_FOEXTRAP = 2, BC_EXT_DIR = 3, BC_USER_1 = 1001, BC_USER_2 = 1002 };
+
+struct BcSpec {
+  int periodic[3];
+  int lo[3], hi[3];
+  double lo_val[3][4], hi_val[3][4];
+};
+
+struct GhostArgs {
+  GridGeom g;
+  void* gbuf;
+  FaceBuf faces;
+  BcSpec bc;
+  int64_t npad;
+};
+
+__device__ inline void pad_cell(const GridGeom& g, int64_t t, int c[3]) {
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  if (g.dim == 2) {
+    const int64_t fx = ny + 2;
+    if (t < 2 * fx) { c[0] = (t < fx) ? -1 : nx; c[1] = (int)(t % fx) - 1; c[2] = 0; return; }
+    t -= 2 * fx;
+    c[1] = (t < nx) ? -1 : ny; c[0] = (int)(t % nx); c[2] = 0;
+    return;
+  }
+  const int64_t fx = (int64_t)(ny + 2) * (nz + 2);
+  if (t < 2 * fx) {
+    c[0] = (t < fx) ? -1 : nx;
+    const int64_t r = t % fx;
+    c[1] = (int)(r % (ny + 2)) - 1; c[2] = (int)(r / (ny + 2)) - 1;
+    return;
+  }
+  t -= 2 * fx;
+  const int64_t fy = (int64_t)nx * (nz + 2);
+  if (t < 2 * fy) {
+    c[1] = (t < fy) ? -1 : ny;
+    const int64_t r = t % fy;
+    c[0] = (int)(r % nx); c[2] = (int)(r / nx) - 1;
+    return;
+  }
+  t -= 2 * fy;
+  const int64_t fz = (int64_t)nx * ny;
+  c[2] = (t < fz) ? -1 : nz;
+  const int64_t r = t % fz;
+  c[0] = (int)(r % nx); c[1] = (int)(r / nx);
+}
+
+template <class LAT, class ST>
+__device__ inline void load_face(const GhostArgs& a, const int c[3], typename ST::compute_t* out) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  const GridGeom& g = a.g;
+  for (int d = 0; d < LAT::D; ++d) {
+    if (a.bc.periodic[d]) continue;
+    int side = -1;
+    if (c[d] == 0) side = 0;
+    else if (c[d] == g.n[d] - 1) side = 1;
+    if (side < 0) continue;
+    const R* fb = static_cast<const R*>(a.faces.p[2 * d + side]);
+    const int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
+                                                                     : (int64_t)g.n[0] * g.n[1];
+    const int64_t fi = face_index(g, d, c[0], c[1], c[2]);
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) out[q] = ST::load(fb, (int64_t)q * fn + fi, T(LAT::w[q]));
+    return;
+  }
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) out[q] = T(0);  // unreachable for valid set-ups
+}
+
+// FillPatch value of a ghost cell from pre-collision data: FilccCell's
+// foextrap chain, then the mesoscopicBcFill override (user_2 > user_1 >
+// ext_dir, first matching dimension), bc_fill.H:303-347.
+template <class LAT, class ST>
+__device__ void fillpatch_value(const GhostArgs& a, const int iv[3], typename ST::compute_t* out) {
+  using T = typename ST::compute_t;
+  const GridGeom& g = a.g;
+  int c[3] = {iv[0], iv[1], iv[2]};
+  for (int iter = 0; iter < 8; ++iter) {
+    int lo_hit[3] = {0, 0, 0}, hi_hit[3] = {0, 0, 0};
+    bool any_out = false, nonper_out = false;
+    for (int d = 0; d < LAT::D; ++d) {
+      if (c[d] < 0) { lo_hit[d] = 1; any_out = true; if (!a.bc.periodic[d]) nonper_out = true; }
+      else if (c[d] > g.n[d] - 1) { hi_hit[d] = 1; any_out = true; if (!a.bc.periodic[d]) nonper_out = true; }
+    }
+    if (!any_out) { load_face<LAT, ST>(a, c, out); return; }
+    if (!nonper_out) {  // ParallelCopy of the periodic image
+      for (int d = 0; d < LAT::D; ++d) {
+        if (c[d] < 0) c[d] += g.n[d];
+        else if (c[d] > g.n[d] - 1) c[d] -= g.n[d];
+      }
+      continue;
+    }
+    int kind = 0, bd = -1, lo_side = 1;
+    const int types[3] = {BC_USER_2, BC_USER_1, BC_EXT_DIR};
+    for (int tt = 0; tt < 3 && kind == 0; ++tt) {
+      for (int d = 0; d < LAT::D; ++d) {
+        if (lo_hit[d] && a.bc.lo[d] == types[tt]) { kind = types[tt]; bd = d; lo_side = 1; break; }
+        if (hi_hit[d] && a.bc.hi[d] == types[tt]) { kind = types[tt]; bd = d; lo_side = 0; break; }
+      }
+    }
+    if (kind != 0) {
+      int mir[3] = {c[0], c[1], c[2]};
+      for (int d = 0; d < LAT::D; ++d) {
+        if (mir[d] < 0) mir[d] = 0;
+        else if (mir[d] > g.n[d] - 1) mir[d] = g.n[d] - 1;
+      }
+      T fnp[LAT::Q];
+      load_face<LAT, ST>(a, mir, fnp);
+      if (kind == BC_USER_1) {
+#pragma unroll
+        for (int q = 0; q < LAT::Q; ++q) out[q] = fnp[q];
+        return;
+      }
+      T rhonp = T(0), unp = T(0), vnp = T(0), wnp = T(0);
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) {
+        rhonp += fnp[q];
+        unp += T(LAT::cx[q]) * fnp[q];
+        vnp += T(LAT::cy[q]) * fnp[q];
+        wnp += T(LAT::cz[q]) * fnp[q];
+      }
+      rhonp = rhonp > T(1.0e-20) ? rhonp : T(1.0e-20);
+      unp /= rhonp; vnp /= rhonp; wnp /= rhonp;
+      const double* bv = lo_side ? a.bc.lo_val[bd] : a.bc.hi_val[bd];
+      const T rho_b = T(bv[0]), u_b = T(bv[1]), v_b = T(bv[2]), w_b = T(bv[3]);
+      T rg = rho_b, ug = u_b, vg = v_b, wg = w_b;
+      if (kind == BC_USER_2) {
+        const T t2 = T(2.0) * rho_b - rhonp;
+        rg = T(1.0e-20) > t2 ? T(1.0e-20) : t2;
+        ug = T(2.0) * u_b - unp; vg = T(2.0) * v_b - vnp; wg = T(2.0) * w_b - wnp;
+      }
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) {
+        const T cx = T(LAT::cx[q]), cy = T(LAT::cy[q]), cz = T(LAT::cz[q]);
+        const T eu_np = cx * unp + cy * vnp + cz * wnp;
+        const T uv_np = unp * unp + vnp * vnp + wnp * wnp;
+        const T feqnp = T(LAT::w[q]) * rhonp * (T(1.0) + T(3.0) * eu_np + T(4.5) * eu_np * eu_np - T(1.5) * uv_np);
+        const T eu_g = cx * ug + cy * vg + cz * wg;
+        const T uv_g = ug * ug + vg * vg + wg * wg;
+        const T feqg = T(LAT::w[q]) * rg * (T(1.0) + T(3.0) * eu_g + T(4.5) * eu_g * eu_g - T(1.5) * uv_g);
+        out[q] = feqg + (fnp[q] - feqnp);
+      }
+      return;
+    }
+    // FilccCell: the last foextrap dimension in which c is outside wins
+    int dsel = -1;
+    for (int d = 0; d < LAT::D; ++d) {
+      if ((lo_hit[d] && a.bc.lo[d] == BC_FOEXTRAP) || (hi_hit[d] && a.bc.hi[d] == BC_FOEXTRAP)) dsel = d;
+    }
+    if (dsel < 0) break;
+    c[dsel] = lo_hit[dsel] ? 0 : g.n[dsel] - 1;
+  }
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) out[q] = T(0);
+}
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.npad) return;
+  const GridGeom& g = a.g;
+  int iv[3];
+  pad_cell(g, t, iv);
+  R* gb = static_cast<R*>(a.gbuf);
+  bool nonper_out = false, all_user1 = true, any_out = false;
+  for (int d = 0; d < LAT::D; ++d) {
+    const bool lo = iv[d] < 0, hi = iv[d] > g.n[d] - 1;
+    if (!lo && !hi) continue;
+    any_out = true;
+    if (!a.bc.periodic[d]) nonper_out = true;
+    const int type = lo ? a.bc.lo[d] : a.bc.hi[d];
+    if (type != BC_USER_1) all_user1 = false;
+  }
+  const int64_t dst = g.idx(iv[0], iv[1], iv[2]);
+  if (!any_out) return;
+  if (!nonper_out) {  // FillBoundary after the collision
+    int s[3] = {iv[0], iv[1], iv[2]};
+    for (int d = 0; d < LAT::D; ++d) {
+      if (s[d] < 0) s[d] += g.n[d];
+      else if (s[d] > g.n[d] - 1) s[d] -= g.n[d];
+    }
+    const int64_t src = g.idx(s[0], s[1], s[2]);
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) gb[(int64_t)q * g.qs + dst] = gb[(int64_t)q * g.qs + src];
+    return;
+  }
+  T val[LAT::Q];
+  fillpatch_value<LAT, ST>(a, iv, val);
+  if (all_user1) {  // halfway bounce-back (AdvanceLBMAtLevel.cpp:340-372)
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const int it = iv[0] + LAT::cx[q], jt = iv[1] + LAT::cy[q], kt = iv[2] + LAT::cz[q];
+      if (it < 0 || it >= g.n[0] || jt < 0 || jt >= g.n[1] || kt < 0 || kt >= g.n[2]) continue;
+      constexpr int dummy = 0;
+      (void)dummy;
+      const int qr = opposite<LAT>(q);
+      val[q] = ST::load(gb, (int64_t)qr * g.qs + g.idx(it, jt, kt), T(LAT::w[qr]));
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) ST::store(gb, (int64_t)q * g.qs + dst, val[q], T(LAT::w[q]));
+}
+
+// Fixed-order reduction of block partials (Evolve's Sum(Rho)).
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t n, double* out) {
+  __shared__ double lds[16];
+  double s = 0.0;
+  for (int64_t t = threadIdx.x; t < n; t += blockDim.x) s += p[t];
+  s = wave_sum(s);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double r = 0.0;
+    for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) r += lds[wv];
+    *out = r;
+  }
+}
+
+}  // namespace ilbm

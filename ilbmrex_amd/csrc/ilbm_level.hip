@@ -1,0 +1,1125 @@
+// Resident MI355X IB-LBM level: host driver (the AdvancePhiAtLevel /
+// Evolve / IBMarkerDF roles of the reference) and the C ABI of include/ilbm.h.
+//
+// Per coarse step, all on one stream, no host synchronisation:
+//   [IB]  k_ib_*          F^{n+1} on the band from macro^n   (IBMarkerDF::update_forcing)
+//   [IB]  k_guo_fixup     Guo term of step n+1 on band cells  (collide_forced.H:22-50)
+//   [pad] k_fill_ghosts   ghost layer (FillPatch / FillBoundary / bounce-back)
+//         k_step          pull + macros + BGK                 (AdvanceLBMAtLevel.cpp:223-386)
+//         k_sum_partials  Sum(Rho)                            (AmrCoreLBM.cpp:378)
+//   [IB]  k_force_record  force.dat row                       (AmrCoreLBM.cpp:456-701)
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ilbm.h"
+#include "ilbm_ib_kernels.h"
+
+namespace ilbm {
+
+struct HipError {
+  hipError_t e;
+};
+#define HIPCHK(x)                                   \
+  do {                                              \
+    hipError_t e_ = (x);                            \
+    if (e_ != hipSuccess) throw ::ilbm::HipError{e_}; \
+  } while (0)
+
+struct StatusError {
+  int status;
+  std::string msg;
+};
+
+template <class T>
+static T* dev_alloc(size_t count) {
+  if (count == 0) count = 1;
+  void* p = nullptr;
+  if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) throw StatusError{ILBM_ERR_ALLOC, "hipMalloc"};
+  return static_cast<T*>(p);
+}
+static void* dev_alloc_bytes(size_t bytes) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 1;
+  if (hipMalloc(&p, bytes) != hipSuccess) throw StatusError{ILBM_ERR_ALLOC, "hipMalloc"};
+  return p;
+}
+static void dev_free(void* p) {
+  if (p) (void)hipFree(p);
+}
+
+static constexpr double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------------------
+// Marker geometry on the host (IBMarkerDF::build_markers,
+// Source/IBM/IBMarkerDF.cpp:201-497, and the user hooks of
+// Examples/Pitching_airfoil/IBMUserDefinedGeometry.H:59-175).
+// ---------------------------------------------------------------------------
+struct MarkerGeom {
+  int nm = 0;
+  double dv = 0.0, wall_tol = 0.0;
+  std::vector<double> phi, theta;     // 1-based
+  std::vector<double> xref, yref, zref; // 1-based
+};
+
+static double naca_thickness(double x, double chord, double thick) {
+  const double xc = std::min(std::max(x / chord, 0.0), 1.0);
+  const double sqrt_x = std::sqrt(std::max(xc, 0.0));
+  return 5.0 * thick * chord *
+         (0.2969 * sqrt_x - 0.1260 * xc - 0.3516 * xc * xc + 0.2843 * xc * xc * xc -
+          0.1036 * xc * xc * xc * xc);
+}
+
+static MarkerGeom build_marker_geometry(ilbm_case_desc& c) {
+  MarkerGeom G;
+  const double h = c.dx[0];
+  int Ml = c.n_marker;
+  if (c.geometry_type == 1) {
+    if (c.dim != 2) throw StatusError{ILBM_ERR_UNSUPPORTED, "box marker geometry is 2D only"};
+    const double phx = c.plo[0] + c.dx[0] * c.n[0], phy = c.plo[1] + c.dx[1] * c.n[1];
+    double xlo = c.box_lo[0], xhi = c.box_hi[0], ylo = c.box_lo[1], yhi = c.box_hi[1];
+    if (xhi <= xlo) { xlo = c.plo[0]; xhi = phx; }
+    if (yhi <= ylo) { ylo = c.plo[1]; yhi = phy; }
+    const double epsx = std::max(1.0e-6 * c.dx[0], 1.0e-12), epsy = std::max(1.0e-6 * c.dx[1], 1.0e-12);
+    xlo = std::max(xlo, c.plo[0] + epsx); xhi = std::min(xhi, phx - epsx);
+    ylo = std::max(ylo, c.plo[1] + epsy); yhi = std::min(yhi, phy - epsy);
+    if (xhi <= xlo || yhi <= ylo) throw StatusError{ILBM_ERR_INVALID, "invalid in-domain box bounds"};
+    c.box_lo[0] = xlo; c.box_hi[0] = xhi; c.box_lo[1] = ylo; c.box_hi[1] = yhi;
+    double ds = (c.box_ds > 0.0) ? c.box_ds : h;
+    ds = std::max(ds, 1.0e-12);
+    const int nx = std::max(2, (int)std::round((xhi - xlo) / ds) + 1);
+    const int ny = std::max(2, (int)std::round((yhi - ylo) / ds) + 1);
+    const double dsx = (xhi - xlo) / (double)(nx - 1), dsy = (yhi - ylo) / (double)(ny - 1);
+    G.xref.push_back(0.0); G.yref.push_back(0.0); G.zref.push_back(0.0);
+    auto add = [&](double x, double y) { G.xref.push_back(x); G.yref.push_back(y); G.zref.push_back(0.0); };
+    if (c.box_lid_only) {
+      for (int i = 0; i < nx; ++i) add(xlo + (double)i * dsx, yhi);
+    } else {
+      for (int i = 0; i < nx; ++i) add(xlo + (double)i * dsx, ylo);
+      for (int i = 0; i < nx; ++i) add(xlo + (double)i * dsx, yhi);
+      for (int j = 1; j < ny - 1; ++j) {
+        const double y = ylo + (double)j * dsy;
+        add(xlo, y);
+        add(xhi, y);
+      }
+    }
+    Ml = (int)G.xref.size() - 1;
+    const double perim = c.box_lid_only ? (xhi - xlo) : 2.0 * ((xhi - xlo) + (yhi - ylo));
+    G.dv = (perim / (double)Ml) * h;
+    G.wall_tol = (c.box_wall_tol > 0.0) ? c.box_wall_tol : std::max(0.25 * std::min(dsx, dsy), 1.0e-12);
+  } else if (c.geometry_type == 2) {
+    if (c.dim != 2) throw StatusError{ILBM_ERR_UNSUPPORTED, "pitching section geometry is 2D only"};
+    const double chord = std::max(c.airfoil_chord, 1.0e-12);
+    const double thick = std::max(c.airfoil_thickness, 1.0e-12);
+    const int n_half = (c.n_marker > 0) ? std::max(c.n_marker / 2, 32)
+                                        : std::max((int)std::ceil(1.5 * chord / h), 48);
+    G.xref.push_back(0.0); G.yref.push_back(0.0); G.zref.push_back(0.0);
+    for (int i = 0; i <= n_half; ++i) {
+      const double beta = kPi * (double)i / (double)n_half;
+      const double x = 0.5 * chord * (1.0 - std::cos(beta));
+      G.xref.push_back(x); G.yref.push_back(naca_thickness(x, chord, thick)); G.zref.push_back(0.0);
+    }
+    for (int i = n_half - 1; i >= 1; --i) {
+      const double beta = kPi * (double)i / (double)n_half;
+      const double x = 0.5 * chord * (1.0 - std::cos(beta));
+      G.xref.push_back(x); G.yref.push_back(-naca_thickness(x, chord, thick)); G.zref.push_back(0.0);
+    }
+    Ml = (int)G.xref.size() - 1;
+    double perim = 0.0;
+    for (int i = 1; i <= Ml; ++i) {
+      const int inext = (i == Ml) ? 1 : (i + 1);
+      const double dxs = G.xref[inext] - G.xref[i], dys = G.yref[inext] - G.yref[i];
+      perim += std::sqrt(dxs * dxs + dys * dys);
+    }
+    G.dv = (perim / (double)Ml) * h;
+  } else if (c.dim == 2) {
+    if (Ml <= 0) Ml = std::max((int)std::ceil(2.0 * kPi * c.R / h), 16);
+    G.dv = (2.0 * kPi * c.R * h) / (double)Ml;
+    G.phi.assign(Ml + 1, 0.0);
+    G.theta.assign(Ml + 1, 0.5 * kPi);
+    for (int i = 1; i <= Ml; ++i) G.phi[i] = 2.0 * kPi * (double)(i - 1) / (double)Ml;
+  } else {
+    const double r1 = c.R - (c.rd - 0.5) * h, r2 = c.R - (c.rd + 0.5) * h;
+    const double a1 = r1 * r1 * r1, a2 = r2 * r2 * r2;
+    if (Ml <= 0) {
+      const double denom = 3.0 * h * h * h / 4.0 / kPi;
+      Ml = std::max((int)((a1 - a2) / denom), 32);
+    }
+    G.dv = (a1 - a2) / (3.0 * (double)Ml / 4.0 / kPi);
+    G.phi.assign(Ml + 1, 0.0);
+    G.theta.assign(Ml + 1, 0.0);
+    double phi = 0.0;
+    for (int mi = 0; mi < Ml; ++mi) {
+      const double Hk = -1.0 + 2.0 * (double)mi / ((double)Ml - 1.0);
+      const double theta = std::acos(Hk);
+      if (mi == 0 || mi == Ml - 1) phi = 0.0;
+      else phi = std::fmod(phi + 3.809 / std::sqrt((double)Ml) / std::sqrt(1.0 - Hk * Hk), 2.0 * kPi);
+      G.phi[mi + 1] = phi;
+      G.theta[mi + 1] = theta;
+    }
+  }
+  if (Ml <= 0) throw StatusError{ILBM_ERR_INVALID, "no markers"};
+  G.nm = Ml;
+  return G;
+}
+
+// Positions (and user target velocities) at `time`, update_lagrangian_marker
+// (IBMarkerDF.cpp:544-605) and marker_state (IBMUserDefinedGeometry.H:144-175,
+// with the pivot offset along the chord as a parameter).
+static void marker_positions(const ilbm_case_desc& c, const MarkerGeom& G, double time,
+                             std::vector<double>& xm, std::vector<double>& ut) {
+  const int nm = G.nm;
+  xm.assign(3 * (size_t)nm, 0.0);
+  ut.assign(3 * (size_t)nm, 0.0);
+  const double z0 = (c.dim == 3) ? c.z0 : 0.0;
+  for (int m = 0; m < nm; ++m) {
+    const int idx = m + 1;
+    double x, y, z = 0.0;
+    if (c.geometry_type == 1) {
+      x = G.xref[idx]; y = G.yref[idx]; z = (c.dim == 3) ? G.zref[idx] : 0.0;
+    } else if (c.geometry_type == 2) {
+      const double twopi = 2.0 * kPi;
+      const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
+      const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
+      const double omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
+      const double px = c.x0 + c.pivot_x, py = c.y0;
+      const double xrr = G.xref[idx] - c.pivot_x, yr = G.yref[idx];
+      const double cs = std::cos(angle), sn = std::sin(angle);
+      x = px + cs * xrr - sn * yr;
+      y = py + sn * xrr + cs * yr;
+      z = (c.dim == 3) ? c.z0 + G.zref[idx] : 0.0;
+      ut[m] = -omega * (y - py);
+      ut[nm + m] = omega * (x - px);
+      ut[2 * nm + m] = 0.0;
+    } else {
+      x = c.x0 + c.R * std::sin(G.theta[idx]) * std::cos(G.phi[idx]);
+      y = c.y0 + c.R * std::sin(G.theta[idx]) * std::sin(G.phi[idx]);
+      if (c.dim == 3) z = z0 + c.R * std::cos(G.theta[idx]);
+    }
+    xm[m] = x; xm[nm + m] = y; xm[2 * (size_t)nm + m] = z;
+  }
+}
+
+// Peskin kernels and the IVC operator entries (IBMarkerDF.cpp:70-145)
+static double delta_host(double xf, double xp, double h, int type) {
+  const double rr = std::fabs(xf - xp) / h;
+  double value = 0.0;
+  if (type == 0) {
+    if (rr < 1.0) value = 0.125 * (3.0 - 2.0 * rr + std::sqrt(1.0 + 4.0 * rr - 4.0 * rr * rr));
+    else if (rr < 2.0) value = 0.125 * (5.0 - 2.0 * rr - std::sqrt(-7.0 + 12.0 * rr - 4.0 * rr * rr));
+  } else {
+    if (rr < 0.5) value = (1.0 + std::sqrt(1.0 - 3.0 * rr * rr)) / 3.0;
+    else if (rr < 1.5) value = (5.0 - 3.0 * rr - std::sqrt(-2.0 + 6.0 * rr - 3.0 * rr * rr)) / 6.0;
+  }
+  return value / h;
+}
+
+static double overlap_1d(double xa, double xb, double h, int kt, double plo, int ilo, int ihi) {
+  const int ia = (int)std::floor((xa - plo) / h), ib = (int)std::floor((xb - plo) / h);
+  int lo = std::max(ia - 2, ib - 2), hi = std::min(ia + 2, ib + 2);
+  lo = std::max(lo, ilo);
+  hi = std::min(hi, ihi);
+  if (lo > hi) return 0.0;
+  double sum = 0.0;
+  for (int i = lo; i <= hi; ++i) {
+    const double xc = plo + ((double)i + 0.5) * h;
+    sum += delta_host(xc, xa, h, kt) * delta_host(xc, xb, h, kt) * h;
+  }
+  return sum;
+}
+
+// Partial-pivot LU and solve of IBMarkerDF::lu_factorize / lu_solve (:1015-1079)
+static bool lu_factorize(std::vector<double>& a, std::vector<int>& piv, int n, double tiny) {
+  piv.assign(n, 0);
+  for (int k = 0; k < n; ++k) {
+    int p = k;
+    double maxv = std::fabs(a[(size_t)k * n + k]);
+    for (int i = k + 1; i < n; ++i) {
+      const double v = std::fabs(a[(size_t)i * n + k]);
+      if (v > maxv) { maxv = v; p = i; }
+    }
+    if (maxv <= tiny) return false;
+    piv[k] = p;
+    if (p != k)
+      for (int j = 0; j < n; ++j) std::swap(a[(size_t)k * n + j], a[(size_t)p * n + j]);
+    const double akk = a[(size_t)k * n + k];
+    for (int i = k + 1; i < n; ++i) {
+      a[(size_t)i * n + k] /= akk;
+      const double lik = a[(size_t)i * n + k];
+      double* ai = &a[(size_t)i * n];
+      const double* ak = &a[(size_t)k * n];
+      for (int j = k + 1; j < n; ++j) ai[j] -= lik * ak[j];
+    }
+  }
+  return true;
+}
+
+static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, std::vector<double>& x, int n) {
+  for (int k = 0; k < n; ++k)
+    if (piv[k] != k) std::swap(x[k], x[piv[k]]);
+  for (int i = 1; i < n; ++i) {
+    double s = x[i];
+    for (int j = 0; j < i; ++j) s -= a[(size_t)i * n + j] * x[j];
+    x[i] = s;
+  }
+  for (int i = n - 1; i >= 0; --i) {
+    double s = x[i];
+    for (int j = i + 1; j < n; ++j) s -= a[(size_t)i * n + j] * x[j];
+    x[i] = s / a[(size_t)i * n + i];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level
+// ---------------------------------------------------------------------------
+using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&);
+using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
+using FixFn = void (*)(dim3, dim3, hipStream_t, const FixupArgs&);
+using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
+
+template <class LAT, class ST, bool W>
+static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a) {
+  hipLaunchKernelGGL((k_step<LAT, ST, W>), g, b, 0, s, a);
+}
+template <class LAT, class ST, bool W>
+static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
+  hipLaunchKernelGGL((k_init<LAT, ST, W>), g, b, 0, s, a);
+}
+template <class LAT, class ST>
+static void launch_fix(dim3 g, dim3 b, hipStream_t s, const FixupArgs& a) {
+  hipLaunchKernelGGL((k_guo_fixup<LAT, ST>), g, b, 0, s, a);
+}
+template <class LAT, class ST>
+static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
+  hipLaunchKernelGGL((k_fill_ghosts<LAT, ST>), g, b, 0, s, a);
+}
+
+template <class LAT, class ST>
+static void pick(bool wrap, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf) {
+  sf = wrap ? launch_step<LAT, ST, true> : launch_step<LAT, ST, false>;
+  inf = wrap ? launch_init<LAT, ST, true> : launch_init<LAT, ST, false>;
+  ff = launch_fix<LAT, ST>;
+  gf = launch_ghost<LAT, ST>;
+}
+
+template <class LAT>
+static bool lattice_matches(const ilbm_case_desc& c) {
+  if (c.q != LAT::Q || c.dim != LAT::D) return false;
+  for (int q = 0; q < LAT::Q; ++q) {
+    if (c.cx[q] != (double)LAT::cx[q] || c.cy[q] != (double)LAT::cy[q] || c.cz[q] != (double)LAT::cz[q])
+      return false;
+    if (c.w[q] != LAT::w[q]) return false;
+  }
+  return true;
+}
+
+struct Level {
+  ilbm_case_desc c{};
+  int lat = -1, st = -1;
+  bool wrap = true;
+  GridGeom g{};
+  int64_t ncells = 0;
+  size_t popbytes = 8;
+  void* gbuf[2] = {nullptr, nullptr};
+  int cur = 0;
+  FaceBuf faces{};
+  int nonper[3] = {0, 0, 0};
+  double* partial = nullptr;
+  int64_t nblocks = 0;
+  dim3 grid, block;
+  StepFn stepfn = nullptr;
+  InitFn initfn = nullptr;
+  FixFn fixfn = nullptr;
+  GhostFn ghostfn = nullptr;
+  double tau = 0, dt = 0, t_new = 0, t_old = -1.0e100;
+  int istep = 0;
+  hipStream_t stream = nullptr;
+  // exports
+  double* mout = nullptr;
+  double* fout = nullptr;
+  bool state_valid = false;
+  // histories
+  double* mass_dev = nullptr;
+  int mass_cap = 0;
+  std::vector<double> mass_host;
+  double* force_dev = nullptr;
+  int force_cap = 0, ncol = 0;
+  std::vector<double> force_host;
+  // IB
+  bool ib = false;
+  MarkerGeom mg;
+  int nm = 0;
+  int band_lo[3] = {0, 0, 0}, band_n[3] = {0, 0, 0};
+  int64_t band_cells = 0;
+  double *Mband = nullptr, *Fband = nullptr, *EV = nullptr, *EFI = nullptr, *EFT = nullptr;
+  double *xm_d = nullptr, *ut_d = nullptr, *U_d = nullptr, *F_d = nullptr, *Mm_d = nullptr;
+  int* base_d = nullptr;
+  double *wts_d = nullptr, *scale_d = nullptr, *rhs_d = nullptr, *sums_d = nullptr;
+  double* Ainv_d = nullptr;
+  bool ivc_ready = false;
+  std::vector<double> ivc_xc;  // positions the operator was built for
+  int* rowptr_d = nullptr;
+  int* entm_d = nullptr;
+  int* entc_d = nullptr;
+  double* entw_d = nullptr;
+  int64_t nent = 0, ent_cap = 0;
+  double* frstate_d = nullptr;
+  std::vector<double> xm_h, ut_h;
+  bool moving = false;
+  // timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_start, ev_stop;
+  double last_avg_ms = 0.0;
+  int last_launches = 0;
+
+  ~Level() {
+    for (auto e : ev_start) (void)hipEventDestroy(e);
+    for (auto e : ev_stop) (void)hipEventDestroy(e);
+    dev_free(gbuf[0]); dev_free(gbuf[1]);
+    for (int f = 0; f < 6; ++f) dev_free(faces.p[f]);
+    dev_free(partial); dev_free(mout); dev_free(fout); dev_free(mass_dev); dev_free(force_dev);
+    dev_free(Mband); dev_free(Fband); dev_free(EV); dev_free(EFI); dev_free(EFT);
+    dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
+    dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
+    dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void validate() {
+    if (c.dim != 2 && c.dim != 3) throw StatusError{ILBM_ERR_INVALID, "dim must be 2 or 3"};
+    if (c.dim == 2) { c.n[2] = 1; c.periodic[2] = 1; c.bc_lo[2] = c.bc_hi[2] = 0; }
+    for (int d = 0; d < 3; ++d)
+      if (c.n[d] < 1) throw StatusError{ILBM_ERR_INVALID, "n must be positive"};
+    for (int d = 0; d < c.dim; ++d) {
+      const int ok[5] = {0, 2, 3, 1001, 1002};
+      for (int side = 0; side < 2; ++side) {
+        const int b = side ? c.bc_hi[d] : c.bc_lo[d];
+        if (std::find(ok, ok + 5, b) == ok + 5) throw StatusError{ILBM_ERR_UNSUPPORTED, "BC type"};
+        if (c.periodic[d] && b != 0) throw StatusError{ILBM_ERR_INVALID, "periodic dim needs int_dir"};
+        if (!c.periodic[d] && b == 0) throw StatusError{ILBM_ERR_INVALID, "int_dir on non-periodic dim"};
+      }
+    }
+    if (lattice_matches<D2Q9>(c)) lat = 0;
+    else if (lattice_matches<D3Q19>(c)) lat = 1;
+    else if (lattice_matches<D3Q27>(c)) lat = 2;
+    else throw StatusError{ILBM_ERR_UNSUPPORTED, "lattice table is not D2Q9/D3Q19/D3Q27 in the reference order"};
+    if (c.storage < 0 || c.storage > 2) throw StatusError{ILBM_ERR_INVALID, "storage"};
+    st = c.storage;
+    if (c.nu <= 0.0) throw StatusError{ILBM_ERR_INVALID, "nu must be positive"};
+  }
+
+  void setup_layout() {
+    wrap = true;
+    for (int d = 0; d < c.dim; ++d)
+      if (!c.periodic[d]) { wrap = false; nonper[d] = 1; }
+    g.dim = c.dim;
+    for (int d = 0; d < 3; ++d) g.n[d] = c.n[d];
+    g.pad = wrap ? 0 : 1;
+    if (wrap) {
+      g.js = c.n[0];
+      g.ks = (int64_t)c.n[0] * c.n[1];
+      g.off0 = 0;
+      g.qs = ((int64_t)c.n[0] * c.n[1] * c.n[2] + 127) / 128 * 128;
+    } else {
+      // rows start on 64-element boundaries; the x ghosts sit in the row
+      // slack (i = -1 at the end of the previous slot, i = n at its start)
+      g.js = (c.n[0] >= 64) ? (c.n[0] + 2 + 63) / 64 * 64 : c.n[0] + 2;
+      g.ks = g.js * (int64_t)(c.n[1] + 2);
+      g.off0 = g.js + ((c.dim == 3) ? g.ks : 0) + 64;
+      const int64_t top = g.off0 + (int64_t)((c.dim == 3) ? c.n[2] : 0) * g.ks + (int64_t)c.n[1] * g.js + c.n[0] + 1;
+      g.qs = (top + 127) / 128 * 128;
+    }
+    ncells = (int64_t)c.n[0] * c.n[1] * c.n[2];
+    popbytes = (st == 0) ? 8 : (st == 1) ? 4 : 2;
+    const int Q = c.q;
+    for (int b = 0; b < 2; ++b) {
+      gbuf[b] = dev_alloc_bytes(popbytes * (size_t)g.qs * Q);
+      HIPCHK(hipMemsetAsync(gbuf[b], 0, popbytes * (size_t)g.qs * Q, stream));
+    }
+    for (int f = 0; f < 6; ++f) faces.p[f] = nullptr;
+    for (int d = 0; d < c.dim; ++d) {
+      if (!nonper[d]) continue;
+      const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
+                                                                       : (int64_t)c.n[0] * c.n[1];
+      for (int side = 0; side < 2; ++side) faces.p[2 * d + side] = dev_alloc_bytes(popbytes * (size_t)fn * Q);
+    }
+    const int bx = std::min(256, (c.n[0] + 63) / 64 * 64);
+    const int by = 256 / bx;
+    block = dim3(bx, by, 1);
+    grid = dim3((c.n[0] + bx - 1) / bx, (c.n[1] + by - 1) / by, c.n[2]);
+    nblocks = (int64_t)grid.x * grid.y * grid.z;
+    partial = dev_alloc<double>(nblocks);
+    auto pk = [&](auto latt) {
+      using L = decltype(latt);
+      if (st == 0) pick<L, StF64>(wrap, stepfn, initfn, fixfn, ghostfn);
+      else if (st == 1) pick<L, StF32>(wrap, stepfn, initfn, fixfn, ghostfn);
+      else pick<L, StF16S>(wrap, stepfn, initfn, fixfn, ghostfn);
+    };
+    if (lat == 0) pk(D2Q9{});
+    else if (lat == 1) pk(D3Q19{});
+    else pk(D3Q27{});
+    tau = 3.0 * c.nu / c.dx[0] + 0.5;  // AmrCoreLBM.cpp:225
+    dt = c.dx[0];                       // AmrCoreLBM.cpp:228
+  }
+
+  // ---------------- immersed boundary set-up ----------------
+  void setup_ib() {
+    ib = c.use_ib != 0;
+    if (!ib) return;
+    if (c.coupling_method < 0 || c.coupling_method > 2) throw StatusError{ILBM_ERR_INVALID, "coupling_method"};
+    mg = build_marker_geometry(c);
+    nm = mg.nm;
+    moving = (c.geometry_type == 2);
+    for (int d = 0; d < c.dim; ++d)
+      if (c.periodic[d] && c.n[d] < 5) throw StatusError{ILBM_ERR_UNSUPPORTED, "periodic extent < 5 with IB"};
+    // band: bounding box of the 5-point stencils (over the motion for a moving body)
+    int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+    for (int d = 0; d < 3; ++d) { lo[d] = 1 << 30; hi[d] = -(1 << 30); }
+    auto absorb = [&](const std::vector<double>& xm, int margin) {
+      for (int m = 0; m < nm; ++m)
+        for (int d = 0; d < c.dim; ++d) {
+          const int b = (int)std::floor((xm[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]);
+          lo[d] = std::min(lo[d], b - 2 - margin);
+          hi[d] = std::max(hi[d], b + 2 + margin);
+        }
+    };
+    std::vector<double> ut;
+    if (moving && c.pitch_frequency != 0.0) {
+      const double period = 1.0 / std::fabs(c.pitch_frequency);
+      for (int s = 0; s < 2048; ++s) {
+        marker_positions(c, mg, period * (double)s / 2048.0, xm_h, ut);
+        absorb(xm_h, 1);
+      }
+    }
+    marker_positions(c, mg, 0.0, xm_h, ut_h);
+    absorb(xm_h, moving ? 1 : 0);
+    for (int d = 0; d < 3; ++d) {
+      if (d >= c.dim) { band_lo[d] = 0; band_n[d] = 1; continue; }
+      if (c.periodic[d] && (lo[d] < 0 || hi[d] > c.n[d] - 1)) { band_lo[d] = 0; band_n[d] = c.n[d]; continue; }
+      const int l = std::max(lo[d], 0), h = std::min(hi[d], c.n[d] - 1);
+      if (h < l) throw StatusError{ILBM_ERR_INVALID, "markers outside the domain"};
+      band_lo[d] = l;
+      band_n[d] = h - l + 1;
+    }
+    band_cells = (int64_t)band_n[0] * band_n[1] * band_n[2];
+    Mband = dev_alloc<double>(4 * band_cells);
+    Fband = dev_alloc<double>(3 * band_cells);
+    EV = dev_alloc<double>(3 * band_cells);
+    EFI = dev_alloc<double>(3 * band_cells);
+    EFT = dev_alloc<double>(3 * band_cells);
+    HIPCHK(hipMemsetAsync(Fband, 0, sizeof(double) * 3 * band_cells, stream));
+    HIPCHK(hipMemsetAsync(Mband, 0, sizeof(double) * 4 * band_cells, stream));
+    xm_d = dev_alloc<double>(3 * nm);
+    ut_d = dev_alloc<double>(3 * nm);
+    U_d = dev_alloc<double>(3 * nm);
+    F_d = dev_alloc<double>(3 * nm);
+    Mm_d = dev_alloc<double>(3 * nm);
+    base_d = dev_alloc<int>(3 * nm);
+    wts_d = dev_alloc<double>(15 * nm);
+    scale_d = dev_alloc<double>(2 * nm);
+    rhs_d = dev_alloc<double>(3 * nm);
+    sums_d = dev_alloc<double>(8);
+    frstate_d = dev_alloc<double>(4);
+    HIPCHK(hipMemsetAsync(frstate_d, 0, sizeof(double) * 4, stream));
+    HIPCHK(hipMemsetAsync(sums_d, 0, sizeof(double) * 8, stream));
+    HIPCHK(hipMemsetAsync(F_d, 0, sizeof(double) * 3 * nm, stream));
+    HIPCHK(hipMemsetAsync(U_d, 0, sizeof(double) * 3 * nm, stream));
+    HIPCHK(hipMemsetAsync(Mm_d, 0, sizeof(double) * 3 * nm, stream));
+    rowptr_d = dev_alloc<int>(band_cells + 1);
+    upload_markers();
+    ncol = (c.dim == 3) ? 33 : 23;
+  }
+
+  // positions, spreading lists and (for IVC) the inverse operator
+  void upload_markers() {
+    HIPCHK(hipMemcpyAsync(xm_d, xm_h.data(), sizeof(double) * 3 * nm, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipMemcpyAsync(ut_d, ut_h.data(), sizeof(double) * 3 * nm, hipMemcpyHostToDevice, stream));
+    build_spread_lists();
+    if (c.coupling_method == 1) build_ivc();
+    // delta weights depend on positions only: once for a fixed body, every
+    // step for a moving one
+    const IbArgs a = ib_args();
+    hipLaunchKernelGGL(k_ib_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, a);
+    if (nent)
+      hipLaunchKernelGGL(k_ib_entry_weights, dim3((unsigned)((nent + 255) / 256)), dim3(256), 0, stream, a, nent);
+    HIPCHK(hipGetLastError());
+  }
+
+  void build_spread_lists() {
+    struct Ent { int64_t b; int grp; int m; int code; };
+    std::vector<Ent> ents;
+    ents.reserve((size_t)nm * (c.dim == 3 ? 125 : 25));
+    for (int m = 0; m < nm; ++m) {
+      int b[3] = {0, 0, 0};
+      for (int d = 0; d < c.dim; ++d) b[d] = (int)std::floor((xm_h[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]);
+      const int kmax = (c.dim == 3) ? 5 : 1;
+      for (int ii = 0; ii < 5; ++ii)
+        for (int jj = 0; jj < 5; ++jj)
+          for (int kk = 0; kk < kmax; ++kk) {
+            int p[3] = {b[0] + ii - 2, b[1] + jj - 2, (c.dim == 3) ? b[2] + kk - 2 : 0};
+            bool drop = false;
+            int t[3] = {0, 0, 0};
+            for (int d = 0; d < c.dim; ++d) {
+              if (p[d] < -2 || p[d] > c.n[d] + 1) drop = true;
+              if (p[d] < 0 || p[d] > c.n[d] - 1) {
+                if (!c.periodic[d]) drop = true;
+                else if (p[d] < 0) { t[d] = -1; p[d] += c.n[d]; }
+                else { t[d] = 1; p[d] -= c.n[d]; }
+              }
+            }
+            if (drop) continue;
+            const int bi = p[0] - band_lo[0], bj = p[1] - band_lo[1], bk = p[2] - band_lo[2];
+            if (bi < 0 || bi >= band_n[0] || bj < 0 || bj >= band_n[1] || bk < 0 || bk >= band_n[2])
+              throw StatusError{ILBM_ERR_STATE, "marker stencil left the IB band"};
+            const int64_t bidx = ((int64_t)bk * band_n[1] + bj) * band_n[0] + bi;
+            int grp = 0;
+            if (t[0] || t[1] || t[2]) {
+              const int code = (t[2] + 1) * 9 + (t[1] + 1) * 3 + (t[0] + 1);
+              grp = (code < 13) ? code + 1 : code;
+            }
+            ents.push_back({bidx, grp, m, ii | (jj << 3) | ((c.dim == 3 ? kk : 0) << 6) | (grp << 9)});
+          }
+    }
+    std::stable_sort(ents.begin(), ents.end(), [](const Ent& x, const Ent& y) {
+      if (x.b != y.b) return x.b < y.b;
+      if (x.grp != y.grp) return x.grp < y.grp;
+      return x.m < y.m;
+    });
+    std::vector<int> rowptr(band_cells + 1, 0), em(ents.size()), ec(ents.size());
+    for (size_t e = 0; e < ents.size(); ++e) {
+      rowptr[ents[e].b + 1]++;
+      em[e] = ents[e].m;
+      ec[e] = ents[e].code;
+    }
+    for (int64_t b = 0; b < band_cells; ++b) rowptr[b + 1] += rowptr[b];
+    nent = (int64_t)ents.size();
+    if (nent > ent_cap) {
+      dev_free(entm_d); dev_free(entc_d); dev_free(entw_d);
+      ent_cap = std::max<int64_t>(nent, 1) * 2;
+      entm_d = dev_alloc<int>(ent_cap);
+      entc_d = dev_alloc<int>(ent_cap);
+      entw_d = dev_alloc<double>(ent_cap);
+    }
+    HIPCHK(hipMemcpyAsync(rowptr_d, rowptr.data(), sizeof(int) * (band_cells + 1), hipMemcpyHostToDevice, stream));
+    if (nent) {
+      HIPCHK(hipMemcpyAsync(entm_d, em.data(), sizeof(int) * nent, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(entc_d, ec.data(), sizeof(int) * nent, hipMemcpyHostToDevice, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));  // host vectors go out of scope
+  }
+
+  // build_ivc_operator + lu_factorize (IBMarkerDF.cpp:1081-1142); the solve
+  // is applied on the device as F += A^{-1} b.
+  void build_ivc() {
+    const int n = nm;
+    bool changed = ivc_xc.size() != (size_t)(3 * n);
+    if (!changed)
+      for (int i = 0; i < 3 * n && !changed; ++i)
+        if (std::fabs(xm_h[i] - ivc_xc[i]) > 1.0e-12) changed = true;
+    if (!changed && ivc_ready && c.ivc_rebuild_matrix <= 0) return;
+    const int kt = (c.delta_type == 3 || c.delta_type == 1) ? 1 : 0;
+    std::vector<double> A((size_t)n * n, 0.0);
+    for (int i = 0; i < n; ++i)
+      for (int j = i; j < n; ++j) {
+        const double sx = overlap_1d(xm_h[i], xm_h[j], c.dx[0], kt, c.plo[0], 0, c.n[0] - 1);
+        const double sy = overlap_1d(xm_h[n + i], xm_h[n + j], c.dx[1], kt, c.plo[1], 0, c.n[1] - 1);
+        const double sz = (c.dim == 3)
+                              ? overlap_1d(xm_h[2 * n + i], xm_h[2 * n + j], c.dx[2], kt, c.plo[2], 0, c.n[2] - 1)
+                              : 1.0;
+        double val = mg.dv * sx * sy * sz;
+        if (i == j) val += c.ivc_diag_reg;
+        A[(size_t)i * n + j] = val;
+        A[(size_t)j * n + i] = val;
+      }
+    const double tiny = std::max(c.ivc_diag_reg * 1.0e-4, 1.0e-30);
+    std::vector<int> piv;
+    ivc_ready = lu_factorize(A, piv, n, tiny);
+    ivc_xc = xm_h;
+    if (!ivc_ready) return;  // explicit forcing fallback (:1253-1259)
+    std::vector<double> Ainv((size_t)n * n), col(n);
+    for (int j = 0; j < n; ++j) {
+      std::fill(col.begin(), col.end(), 0.0);
+      col[j] = 1.0;
+      lu_solve(A, piv, col, n);
+      for (int i = 0; i < n; ++i) Ainv[(size_t)i * n + j] = col[i];
+    }
+    if (!Ainv_d) Ainv_d = dev_alloc<double>((size_t)n * n);
+    HIPCHK(hipMemcpyAsync(Ainv_d, Ainv.data(), sizeof(double) * (size_t)n * n, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  IbArgs ib_args() const {
+    IbArgs a{};
+    a.dim = c.dim;
+    for (int d = 0; d < 3; ++d) {
+      a.n[d] = c.n[d]; a.periodic[d] = c.periodic[d]; a.plo[d] = c.plo[d]; a.dx[d] = c.dx[d];
+      a.band_lo[d] = band_lo[d]; a.band_n[d] = band_n[d];
+    }
+    a.bc = bc_spec();
+    a.band_cells = band_cells;
+    a.nm = nm;
+    a.dt = dt;
+    a.dv = mg.dv;
+    a.kernel_type = (c.delta_type == 3 || c.delta_type == 1) ? 1 : 0;
+    a.renorm_interp = (c.renormalize_delta != 0) && (c.coupling_method != 1);
+    a.renorm_spread = (c.renormalize_delta != 0);
+    a.coupling = c.coupling_method;
+    a.relax = std::max(0.0, std::min(1.0, c.mdf_relax));
+    a.diag_eps = std::max(c.explicit_diag_eps, 1.0e-30);
+    a.geometry = c.geometry_type;
+    a.x0 = c.x0; a.y0 = c.y0; a.z0 = c.z0;
+    a.ub[0] = c.ubx; a.ub[1] = c.uby; a.ub[2] = c.ubz;
+    a.om[0] = c.omx; a.om[1] = c.omy; a.om[2] = c.omz;
+    a.box_yhi = c.box_hi[1]; a.wall_tol = mg.wall_tol;
+    for (int d = 0; d < 3; ++d) a.lid_u[d] = c.box_lid_u[d];
+    a.xm = xm_d; a.ut = ut_d; a.U = U_d; a.F = F_d; a.Mm = Mm_d;
+    a.base = base_d; a.wts = wts_d; a.scale = scale_d;
+    a.Mband = Mband; a.EV = EV; a.EFI = EFI; a.EFT = EFT; a.Fband = Fband;
+    a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
+    a.Ainv = Ainv_d; a.rhs = rhs_d; a.sums = sums_d;
+    return a;
+  }
+
+  BcSpec bc_spec() const {
+    BcSpec b{};
+    for (int d = 0; d < 3; ++d) {
+      b.periodic[d] = (d < c.dim) ? c.periodic[d] : 1;
+      b.lo[d] = c.bc_lo[d]; b.hi[d] = c.bc_hi[d];
+      for (int k = 0; k < 4; ++k) { b.lo_val[d][k] = c.bc_lo_val[d][k]; b.hi_val[d][k] = c.bc_hi_val[d][k]; }
+    }
+    return b;
+  }
+
+  // IBMarkerDF::update_forcing (IBMarkerDF.cpp:1522-1917) on the band
+  void ib_update_forcing(double time) {
+    if (moving) {
+      marker_positions(c, mg, time, xm_h, ut_h);
+      upload_markers();
+    }
+    const IbArgs a = ib_args();
+    const int tb = 256;
+    const unsigned gb = (unsigned)((band_cells + tb - 1) / tb);
+    const unsigned gm = (unsigned)((nm + tb - 1) / tb);
+    hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
+    const bool use_ivc = (c.coupling_method == 1);
+    const int nloop = use_ivc ? 1 : std::max(c.loop_ns, 1);
+    for (int it = 0; it < nloop; ++it) {
+      hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
+                         stream, a);
+      if (use_ivc && ivc_ready) {
+        hipLaunchKernelGGL(k_ib_ivc_rhs, dim3(gm), dim3(tb), 0, stream, a);
+        hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
+      } else {
+        hipLaunchKernelGGL(k_ib_force_explicit, dim3(gm), dim3(tb), 0, stream, a);
+      }
+      hipLaunchKernelGGL(k_ib_spread, dim3(gb), dim3(tb), 0, stream, a);
+      hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
+      hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
+    }
+    hipLaunchKernelGGL(k_ib_export, dim3(gb), dim3(tb), 0, stream, a);
+    HIPCHK(hipGetLastError());
+  }
+
+  // ---------------- lattice ----------------
+  StepArgs step_args(bool want_state) {
+    StepArgs a{};
+    a.g = g;
+    a.gin = gbuf[cur];
+    a.gout = gbuf[cur ^ 1];
+    a.tau = tau;
+    a.inv_tau = 1.0 / tau;
+    for (int d = 0; d < 3; ++d) {
+      const double pf = (c.use_prescribed_force && !ib) ? c.prescribed_force[d] : 0.0;
+      a.Fmac[d] = pf;
+      a.Fcol[d] = pf;
+      a.band_lo[d] = ib ? band_lo[d] : 0;
+      a.band_n[d] = ib ? band_n[d] : 0;
+      a.nonper[d] = nonper[d];
+    }
+    a.Fband = Fband;
+    a.Mband = Mband;
+    a.band_cells = ib ? band_cells : 0;
+    a.mout = want_state ? mout : nullptr;
+    a.fout = want_state ? fout : nullptr;
+    a.faces = faces;
+    a.partial = partial;
+    return a;
+  }
+
+  void ensure_exports() {
+    if (!mout) mout = dev_alloc<double>(4 * ncells);
+    if (!fout) fout = dev_alloc<double>((size_t)c.q * ncells);
+  }
+
+  void init(const double* macro_host) {
+    double* m0 = dev_alloc<double>(4 * ncells);
+    HIPCHK(hipMemcpyAsync(m0, macro_host, sizeof(double) * 4 * ncells, hipMemcpyHostToDevice, stream));
+    const bool small = (int64_t)c.q * ncells <= (int64_t)1 << 27;
+    if (small) ensure_exports();
+    InitArgs ia{};
+    ia.s = step_args(small);
+    ia.s.gout = gbuf[cur];
+    ia.macro0 = m0;
+    initfn(grid, block, stream, ia);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(stream));
+    dev_free(m0);
+    state_valid = small;
+    t_new = 0.0;
+    t_old = -1.0e100;
+    istep = 0;
+  }
+
+  void grow_histories(int need_mass, int need_force) {
+    if (need_mass > mass_cap) {
+      dev_free(mass_dev);
+      mass_cap = std::max(need_mass, 2 * mass_cap);
+      mass_dev = dev_alloc<double>(mass_cap);
+    }
+    if (ib && need_force > force_cap) {
+      dev_free(force_dev);
+      force_cap = std::max(need_force, 2 * force_cap);
+      force_dev = dev_alloc<double>((size_t)force_cap * ncol);
+    }
+  }
+
+  void advance(int nsteps, bool want_state) {
+    if (nsteps < 0) throw StatusError{ILBM_ERR_INVALID, "nsteps < 0"};
+    if (nsteps == 0) return;
+    if (want_state) ensure_exports();
+    grow_histories(nsteps, nsteps);
+    if (timing) {
+      while ((int)ev_start.size() < nsteps) {
+        hipEvent_t a, b;
+        HIPCHK(hipEventCreate(&a));
+        HIPCHK(hipEventCreate(&b));
+        ev_start.push_back(a);
+        ev_stop.push_back(b);
+      }
+    }
+    state_valid = false;
+    const int tb = 256;
+    for (int s = 0; s < nsteps; ++s) {
+      const double time = t_new;  // timeStepWithSubcycling, AmrCoreLBM.cpp:1730-1733
+      t_old = t_new;
+      t_new += dt;
+      if (ib) {
+        ib_update_forcing(time);
+        FixupArgs fa{};
+        fa.g = g; fa.gbuf = gbuf[cur]; fa.tau = tau;
+        for (int d = 0; d < 3; ++d) { fa.band_lo[d] = band_lo[d]; fa.band_n[d] = band_n[d]; }
+        fa.band_cells = band_cells; fa.Fband = Fband; fa.Mband = Mband;
+        fixfn(dim3((unsigned)((band_cells + tb - 1) / tb)), dim3(tb), stream, fa);
+      }
+      if (!wrap) {
+        GhostArgs ga{};
+        ga.g = g; ga.gbuf = gbuf[cur]; ga.faces = faces; ga.bc = bc_spec();
+        const int64_t P0 = c.n[0] + 2, P1 = c.n[1] + 2, P2 = (c.dim == 3) ? c.n[2] + 2 : 1;
+        ga.npad = P0 * P1 * P2 - ncells;
+        ghostfn(dim3((unsigned)((ga.npad + tb - 1) / tb)), dim3(tb), stream, ga);
+      }
+      const bool last = (s == nsteps - 1);
+      StepArgs sa = step_args(want_state && last);
+      if (timing) HIPCHK(hipEventRecord(ev_start[s], stream));
+      stepfn(grid, block, stream, sa);
+      if (timing) HIPCHK(hipEventRecord(ev_stop[s], stream));
+      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, stream, (const double*)partial, nblocks,
+                         mass_dev + s);
+      cur ^= 1;
+      ++istep;
+      if (ib) {
+        ForceRecArgs fr{};
+        fr.Fband = Fband; fr.band_cells = band_cells; fr.sums = sums_d; fr.state = frstate_d;
+        fr.hist = force_dev; fr.row = s; fr.ncol = ncol; fr.dim = c.dim; fr.method = c.force_eval_method;
+        fr.time = time + dt; fr.step = (double)istep;
+        fr.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
+        fr.dt = dt; fr.U0 = c.U0; fr.R = c.R;
+        hipLaunchKernelGGL(k_force_record, dim3(1), dim3(1024), 0, stream, fr);
+      }
+      HIPCHK(hipGetLastError());
+    }
+    std::vector<double> mh(nsteps);
+    HIPCHK(hipMemcpyAsync(mh.data(), mass_dev, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
+    std::vector<double> fh;
+    if (ib) {
+      fh.resize((size_t)nsteps * ncol);
+      HIPCHK(hipMemcpyAsync(fh.data(), force_dev, sizeof(double) * nsteps * ncol, hipMemcpyDeviceToHost, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    mass_host.insert(mass_host.end(), mh.begin(), mh.end());
+    force_host.insert(force_host.end(), fh.begin(), fh.end());
+    if (timing) {
+      double tot = 0.0;
+      for (int s = 0; s < nsteps; ++s) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev_start[s], ev_stop[s]));
+        tot += ms;
+      }
+      last_avg_ms = tot / nsteps;
+      last_launches = nsteps;
+    }
+    state_valid = want_state;
+  }
+};
+
+}  // namespace ilbm
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct ilbm_level {
+  ilbm::Level L;
+};
+
+static thread_local std::string g_last_error;
+
+template <class F>
+static int guarded(F&& f) {
+  try {
+    f();
+    return ILBM_OK;
+  } catch (const ilbm::StatusError& e) {
+    g_last_error = e.msg;
+    return e.status;
+  } catch (const ilbm::HipError& e) {
+    g_last_error = hipGetErrorString(e.e);
+    return ILBM_ERR_HIP;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation";
+    return ILBM_ERR_ALLOC;
+  }
+}
+
+extern "C" {
+
+int ilbm_abi_version(void) { return ILBM_ABI_VERSION; }
+
+const char* ilbm_status_string(int status) {
+  switch (status) {
+    case ILBM_OK: return "ok";
+    case ILBM_ERR_INVALID: return "invalid argument";
+    case ILBM_ERR_UNSUPPORTED: return "unsupported";
+    case ILBM_ERR_HIP: return "HIP runtime error";
+    case ILBM_ERR_ALLOC: return "allocation failed";
+    case ILBM_ERR_STATE: return "invalid state";
+    default: return "unknown";
+  }
+}
+
+const char* ilbm_last_error(void) { return g_last_error.c_str(); }
+
+int ilbm_level_create(const ilbm_case_desc* desc, const double* init_macro, ilbm_level** out) {
+  if (!desc || !init_macro || !out) return ILBM_ERR_INVALID;
+  *out = nullptr;
+  ilbm_level* h = new (std::nothrow) ilbm_level();
+  if (!h) return ILBM_ERR_ALLOC;
+  const int st = guarded([&] {
+    ilbm::Level& L = h->L;
+    L.c = *desc;
+    L.validate();
+    HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+    L.setup_layout();
+    L.setup_ib();
+    L.init(init_macro);
+  });
+  if (st != ILBM_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return ILBM_OK;
+}
+
+int ilbm_level_destroy(ilbm_level* h) {
+  if (!h) return ILBM_ERR_INVALID;
+  if (h->L.stream) (void)hipStreamSynchronize(h->L.stream);
+  delete h;
+  return ILBM_OK;
+}
+
+int ilbm_level_advance(ilbm_level* h, int nsteps, int want_state) {
+  if (!h) return ILBM_ERR_INVALID;
+  return guarded([&] { h->L.advance(nsteps, want_state != 0); });
+}
+
+int ilbm_level_synchronize(ilbm_level* h) {
+  if (!h) return ILBM_ERR_INVALID;
+  return guarded([&] { HIPCHK(hipStreamSynchronize(h->L.stream)); });
+}
+
+int ilbm_level_get_f(ilbm_level* h, double* out) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  if (!h->L.state_valid) return ILBM_ERR_STATE;
+  return guarded([&] {
+    HIPCHK(hipMemcpy(out, h->L.fout, sizeof(double) * h->L.c.q * h->L.ncells, hipMemcpyDeviceToHost));
+  });
+}
+
+int ilbm_level_get_macro(ilbm_level* h, double* out) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  if (!h->L.state_valid) return ILBM_ERR_STATE;
+  return guarded([&] {
+    HIPCHK(hipMemcpy(out, h->L.mout, sizeof(double) * 4 * h->L.ncells, hipMemcpyDeviceToHost));
+  });
+}
+
+int ilbm_level_get_forcing(ilbm_level* h, double* out) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    ilbm::Level& L = h->L;
+    std::fill(out, out + 3 * L.ncells, 0.0);
+    if (!L.ib) return;
+    std::vector<double> fb(3 * L.band_cells);
+    HIPCHK(hipMemcpy(fb.data(), L.Fband, sizeof(double) * 3 * L.band_cells, hipMemcpyDeviceToHost));
+    for (int bk = 0; bk < L.band_n[2]; ++bk)
+      for (int bj = 0; bj < L.band_n[1]; ++bj)
+        for (int bi = 0; bi < L.band_n[0]; ++bi) {
+          const int64_t b = ((int64_t)bk * L.band_n[1] + bj) * L.band_n[0] + bi;
+          const int i = L.band_lo[0] + bi, j = L.band_lo[1] + bj, k = L.band_lo[2] + bk;
+          const int64_t v = ((int64_t)k * L.c.n[1] + j) * L.c.n[0] + i;
+          for (int cc = 0; cc < 3; ++cc) out[cc * L.ncells + v] = fb[cc * L.band_cells + b];
+        }
+  });
+}
+
+int ilbm_level_mass_history(ilbm_level* h, double* out, int max_rows, int* n_out) {
+  if (!h || !n_out) return ILBM_ERR_INVALID;
+  const auto& v = h->L.mass_host;
+  *n_out = (int)v.size();
+  if (out) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), (size_t)std::max(max_rows, 0)), out);
+  return ILBM_OK;
+}
+
+int ilbm_level_force_history(ilbm_level* h, double* out, int max_rows, int* n_rows, int* n_cols) {
+  if (!h || !n_rows || !n_cols) return ILBM_ERR_INVALID;
+  const ilbm::Level& L = h->L;
+  *n_cols = L.ncol;
+  *n_rows = L.ncol ? (int)(L.force_host.size() / L.ncol) : 0;
+  if (out && L.ncol) {
+    const size_t n = std::min<size_t>(L.force_host.size(), (size_t)std::max(max_rows, 0) * L.ncol);
+    std::copy(L.force_host.begin(), L.force_host.begin() + n, out);
+  }
+  return ILBM_OK;
+}
+
+int ilbm_level_num_markers(ilbm_level* h, int* out) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  *out = h->L.ib ? h->L.nm : 0;
+  return ILBM_OK;
+}
+
+int ilbm_level_marker_dv(ilbm_level* h, double* out) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  *out = h->L.ib ? h->L.mg.dv : 0.0;
+  return ILBM_OK;
+}
+
+int ilbm_level_marker_state(ilbm_level* h, double* xyz, double* uvw, double* fxyz) {
+  if (!h) return ILBM_ERR_INVALID;
+  ilbm::Level& L = h->L;
+  if (!L.ib) return ILBM_ERR_STATE;
+  return guarded([&] {
+    const int nm = L.nm;
+    std::vector<double> t(3 * nm);
+    auto tr = [&](const double* dev, double* out) {
+      if (!out) return;
+      HIPCHK(hipMemcpy(t.data(), dev, sizeof(double) * 3 * nm, hipMemcpyDeviceToHost));
+      for (int m = 0; m < nm; ++m)
+        for (int d = 0; d < 3; ++d) out[3 * m + d] = t[d * nm + m];
+    };
+    HIPCHK(hipStreamSynchronize(L.stream));
+    tr(L.xm_d, xyz);
+    tr(L.U_d, uvw);
+    tr(L.F_d, fxyz);
+  });
+}
+
+int ilbm_level_marker_force_sum(ilbm_level* h, double out[3]) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  if (!h->L.ib) return ILBM_ERR_STATE;
+  return guarded([&] {
+    double s[8];
+    HIPCHK(hipStreamSynchronize(h->L.stream));
+    HIPCHK(hipMemcpy(s, h->L.sums_d, sizeof(double) * 8, hipMemcpyDeviceToHost));
+    out[0] = s[0]; out[1] = s[1]; out[2] = s[2];
+  });
+}
+
+int ilbm_level_marker_torque(ilbm_level* h, double out[3]) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  if (!h->L.ib) return ILBM_ERR_STATE;
+  return guarded([&] {
+    double s[8];
+    HIPCHK(hipStreamSynchronize(h->L.stream));
+    HIPCHK(hipMemcpy(s, h->L.sums_d, sizeof(double) * 8, hipMemcpyDeviceToHost));
+    out[0] = s[3]; out[1] = s[4]; out[2] = s[5];
+  });
+}
+
+int ilbm_level_info(ilbm_level* h, double out[8]) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  const ilbm::Level& L = h->L;
+  out[0] = L.tau; out[1] = L.dt; out[2] = L.t_new; out[3] = L.istep;
+  out[4] = (double)L.band_cells; out[5] = (double)L.popbytes; out[6] = L.wrap ? 0.0 : 1.0;
+  out[7] = (double)L.nblocks;
+  return ILBM_OK;
+}
+
+int ilbm_level_kernel_time(ilbm_level* h, double* avg_ms, int* launches) {
+  if (!h || !avg_ms || !launches) return ILBM_ERR_INVALID;
+  *avg_ms = h->L.last_avg_ms;
+  *launches = h->L.last_launches;
+  return ILBM_OK;
+}
+
+int ilbm_level_set_timing(ilbm_level* h, int enable) {
+  if (!h) return ILBM_ERR_INVALID;
+  h->L.timing = enable != 0;
+  return ILBM_OK;
+}
+
+// Plain device-to-device copy, used only to measure the achievable HBM
+// bandwidth and to calibrate rocprof byte counters against a kernel whose
+// traffic is known (bytes read = bytes written = `bytes`).
+typedef float v4f __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(256) k_copy_probe(const v4f* __restrict__ src,
+                                                    v4f* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride)
+    __builtin_nontemporal_store(src[t], dst + t);
+}
+
+int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
+  if (bytes <= 0 || iters <= 0 || !avg_ms) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    const int64_t n = bytes / 16;
+    void* a = ilbm::dev_alloc_bytes((size_t)n * 16);
+    void* b = ilbm::dev_alloc_bytes((size_t)n * 16);
+    hipEvent_t e0, e1;
+    HIPCHK(hipMemset(a, 0, (size_t)n * 16));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    const unsigned grid = 256u * 32u;
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, nullptr, (const v4f*)a, (v4f*)b, n);
+    HIPCHK(hipEventRecord(e0, nullptr));
+    for (int it = 0; it < iters; ++it)
+      hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, nullptr, (const v4f*)a, (v4f*)b, n);
+    HIPCHK(hipEventRecord(e1, nullptr));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *avg_ms = ms / iters;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    ilbm::dev_free(a);
+    ilbm::dev_free(b);
+  });
+}
+
+}  // extern "C"

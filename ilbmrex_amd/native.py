@@ -1,0 +1,272 @@
+"""ctypes binding of libilbm_hip.so (the C ABI declared in include/ilbm.h).
+
+The HIP library is the only compute path: there is no CPU fallback.  If the
+library has not been built, importing ``Level`` raises immediately.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import config as cf
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG_DIR, "_lib", "libilbm_hip.so")
+
+D3 = C.c_double * 3
+I3 = C.c_int * 3
+D27 = C.c_double * 27
+
+
+class CaseDesc(C.Structure):
+    """ilbm_case_desc (include/ilbm.h)."""
+    _fields_ = [
+        ("dim", C.c_int), ("n", I3), ("plo", D3), ("dx", D3), ("periodic", I3),
+        ("bc_lo", I3), ("bc_hi", I3),
+        ("bc_lo_val", (C.c_double * 4) * 3), ("bc_hi_val", (C.c_double * 4) * 3),
+        ("q", C.c_int), ("cx", D27), ("cy", D27), ("cz", D27), ("w", D27),
+        ("nu", C.c_double), ("U0", C.c_double), ("nmac", C.c_int),
+        ("use_prescribed_force", C.c_int), ("prescribed_force", D3),
+        ("use_ib", C.c_int), ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+        ("R", C.c_double), ("coupling_method", C.c_int), ("delta_type", C.c_int),
+        ("loop_ns", C.c_int), ("mdf_relax", C.c_double), ("explicit_diag_eps", C.c_double),
+        ("geometry_type", C.c_int), ("n_marker", C.c_int), ("rd", C.c_double),
+        ("ubx", C.c_double), ("uby", C.c_double), ("ubz", C.c_double),
+        ("omx", C.c_double), ("omy", C.c_double), ("omz", C.c_double),
+        ("ivc_diag_reg", C.c_double), ("ivc_rebuild_matrix", C.c_int),
+        ("renormalize_delta", C.c_int), ("force_eval_method", C.c_int),
+        ("airfoil_chord", C.c_double), ("airfoil_thickness", C.c_double),
+        ("pitch_mean", C.c_double), ("pitch_amplitude", C.c_double),
+        ("pitch_frequency", C.c_double), ("pitch_phase", C.c_double),
+        ("pivot_x", C.c_double),
+        ("box_lo", D3), ("box_hi", D3), ("box_ds", C.c_double), ("box_lid_only", C.c_int),
+        ("box_lid_u", D3), ("box_wall_tol", C.c_double),
+        ("storage", C.c_int),
+    ]
+
+
+EXPORTED = [
+    "ilbm_level_create", "ilbm_level_destroy", "ilbm_level_advance", "ilbm_level_synchronize",
+    "ilbm_level_get_f", "ilbm_level_get_macro", "ilbm_level_get_forcing",
+    "ilbm_level_mass_history", "ilbm_level_force_history", "ilbm_level_num_markers",
+    "ilbm_level_marker_dv", "ilbm_level_marker_state", "ilbm_level_marker_force_sum",
+    "ilbm_level_marker_torque", "ilbm_level_info", "ilbm_level_kernel_time",
+    "ilbm_level_set_timing", "ilbm_copy_probe", "ilbm_status_string", "ilbm_last_error",
+    "ilbm_abi_version",
+]
+
+_lib = None
+
+
+class IlbmError(RuntimeError):
+    pass
+
+
+def load():
+    """Loads the in-tree HIP library; raises if it is missing."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise IlbmError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    L.ilbm_level_create.argtypes = [P(CaseDesc), P(C.c_double), P(vp)]
+    L.ilbm_level_destroy.argtypes = [vp]
+    L.ilbm_level_advance.argtypes = [vp, C.c_int, C.c_int]
+    L.ilbm_level_synchronize.argtypes = [vp]
+    for nm in ("ilbm_level_get_f", "ilbm_level_get_macro", "ilbm_level_get_forcing"):
+        getattr(L, nm).argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_mass_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int)]
+    L.ilbm_level_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
+    L.ilbm_level_num_markers.argtypes = [vp, P(C.c_int)]
+    L.ilbm_level_marker_dv.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_marker_state.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_double)]
+    L.ilbm_level_marker_force_sum.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_marker_torque.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_info.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int)]
+    L.ilbm_level_set_timing.argtypes = [vp, C.c_int]
+    L.ilbm_status_string.restype = C.c_char_p
+    L.ilbm_status_string.argtypes = [C.c_int]
+    L.ilbm_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        L = load()
+        msg = L.ilbm_status_string(rc).decode()
+        detail = L.ilbm_last_error().decode()
+        raise IlbmError(f"{what}: {msg} ({detail})")
+
+
+def to_desc(case: "cf.Case") -> CaseDesc:
+    d = CaseDesc()
+    lat = case.lattice
+    d.dim = lat.dim
+    d.n = I3(*case.n)
+    d.plo = D3(*case.plo)
+    d.dx = D3(*case.dx)
+    d.periodic = I3(*case.periodic)
+    d.bc_lo = I3(*case.bc_lo)
+    d.bc_hi = I3(*case.bc_hi)
+    for k in range(3):
+        for m in range(4):
+            d.bc_lo_val[k][m] = case.bc_lo_val[k][m]
+            d.bc_hi_val[k][m] = case.bc_hi_val[k][m]
+    d.q = lat.q
+    w = lat.w
+    for q in range(lat.q):
+        d.cx[q], d.cy[q], d.cz[q], d.w[q] = float(lat.cx[q]), float(lat.cy[q]), float(lat.cz[q]), w[q]
+    d.nu, d.U0, d.nmac = case.nu, case.U0, case.nmac
+    if case.prescribed_force is not None:
+        d.use_prescribed_force = 1
+        d.prescribed_force = D3(*case.prescribed_force)
+    d.use_ib = int(case.use_ib)
+    d.x0, d.y0, d.z0, d.R = case.x0, case.y0, case.z0, case.R
+    d.coupling_method, d.delta_type, d.loop_ns = case.coupling_method, case.delta_type, case.loop_ns
+    d.mdf_relax, d.explicit_diag_eps = case.mdf_relax, case.explicit_diag_eps
+    d.geometry_type, d.n_marker, d.rd = case.geometry_type, case.n_marker, case.rd
+    d.ubx, d.uby, d.ubz = case.ub
+    d.omx, d.omy, d.omz = case.om
+    d.ivc_diag_reg, d.ivc_rebuild_matrix = case.ivc_diag_reg, case.ivc_rebuild_matrix
+    d.renormalize_delta, d.force_eval_method = case.renormalize_delta, case.force_eval_method
+    d.airfoil_chord, d.airfoil_thickness = case.airfoil_chord, case.airfoil_thickness
+    d.pitch_mean, d.pitch_amplitude = case.pitch_mean, case.pitch_amplitude
+    d.pitch_frequency, d.pitch_phase, d.pivot_x = case.pitch_frequency, case.pitch_phase, case.pivot_x
+    d.box_lo, d.box_hi = D3(*case.box_lo), D3(*case.box_hi)
+    d.box_ds, d.box_lid_only = case.box_ds, case.box_lid_only
+    d.box_lid_u, d.box_wall_tol = D3(*case.box_lid_u), case.box_wall_tol
+    d.storage = case.storage
+    return d
+
+
+class Level:
+    """One resident level on the GPU (AmrCoreLBM level 0 + IBMarkerDF)."""
+
+    def __init__(self, case: "cf.Case", init_macro: Optional[np.ndarray] = None):
+        case.validate()
+        self.case = case
+        L = load()
+        if init_macro is None:
+            init_macro = cf.uniform_init(case)
+        m = np.ascontiguousarray(init_macro, dtype=np.float64).reshape(4, -1)
+        if m.shape[1] != case.ncells:
+            raise ValueError("init_macro must be [4][cells]")
+        self._desc = to_desc(case)
+        h = C.c_void_p()
+        _check(L.ilbm_level_create(C.byref(self._desc), _dp(m), C.byref(h)), "ilbm_level_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ilbm_level_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def advance(self, nsteps: int, want_state: bool = False):
+        _check(load().ilbm_level_advance(self._h, int(nsteps), int(want_state)), "advance")
+
+    def synchronize(self):
+        _check(load().ilbm_level_synchronize(self._h), "synchronize")
+
+    def f(self) -> np.ndarray:
+        out = np.empty(self.case.lattice.q * self.case.ncells)
+        _check(load().ilbm_level_get_f(self._h, _dp(out)), "get_f")
+        return out.reshape(self.case.lattice.q, -1)
+
+    def macro(self) -> np.ndarray:
+        out = np.empty(4 * self.case.ncells)
+        _check(load().ilbm_level_get_macro(self._h, _dp(out)), "get_macro")
+        return out.reshape(4, -1)
+
+    def forcing(self) -> np.ndarray:
+        out = np.empty(3 * self.case.ncells)
+        _check(load().ilbm_level_get_forcing(self._h, _dp(out)), "get_forcing")
+        return out.reshape(3, -1)
+
+    def mass_history(self) -> np.ndarray:
+        n = C.c_int(0)
+        _check(load().ilbm_level_mass_history(self._h, None, 0, C.byref(n)), "mass_history")
+        out = np.empty(max(n.value, 1))
+        _check(load().ilbm_level_mass_history(self._h, _dp(out), n.value, C.byref(n)), "mass_history")
+        return out[:n.value]
+
+    def force_history(self) -> np.ndarray:
+        nr, nc = C.c_int(0), C.c_int(0)
+        _check(load().ilbm_level_force_history(self._h, None, 0, C.byref(nr), C.byref(nc)), "force_history")
+        out = np.empty(max(nr.value * nc.value, 1))
+        _check(load().ilbm_level_force_history(self._h, _dp(out), nr.value, C.byref(nr), C.byref(nc)),
+               "force_history")
+        return out[:nr.value * nc.value].reshape(nr.value, max(nc.value, 1))
+
+    def num_markers(self) -> int:
+        n = C.c_int(0)
+        _check(load().ilbm_level_num_markers(self._h, C.byref(n)), "num_markers")
+        return n.value
+
+    def dv(self) -> float:
+        v = C.c_double(0)
+        _check(load().ilbm_level_marker_dv(self._h, C.byref(v)), "marker_dv")
+        return v.value
+
+    def markers(self):
+        n = self.num_markers()
+        xyz, uvw, f = np.zeros(3 * n), np.zeros(3 * n), np.zeros(3 * n)
+        _check(load().ilbm_level_marker_state(self._h, _dp(xyz), _dp(uvw), _dp(f)), "marker_state")
+        return xyz.reshape(n, 3), uvw.reshape(n, 3), f.reshape(n, 3)
+
+    def last_marker_force_sum(self) -> np.ndarray:
+        out = np.zeros(3)
+        _check(load().ilbm_level_marker_force_sum(self._h, _dp(out)), "marker_force_sum")
+        return out
+
+    def torque(self) -> np.ndarray:
+        out = np.zeros(3)
+        _check(load().ilbm_level_marker_torque(self._h, _dp(out)), "marker_torque")
+        return out
+
+    def info(self) -> dict:
+        out = np.zeros(8)
+        _check(load().ilbm_level_info(self._h, _dp(out)), "info")
+        return {"tau": out[0], "dt": out[1], "time": out[2], "steps": int(out[3]),
+                "band_cells": int(out[4]), "pop_bytes": int(out[5]), "padded": bool(out[6]),
+                "blocks": int(out[7])}
+
+    def set_timing(self, enable: bool):
+        _check(load().ilbm_level_set_timing(self._h, int(enable)), "set_timing")
+
+    def kernel_time(self):
+        ms, n = C.c_double(0), C.c_int(0)
+        _check(load().ilbm_level_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
+        return ms.value, n.value
+
+
+def load_probe(nbytes: int, iters: int) -> float:
+    """Average milliseconds of a plain device copy of nbytes (ilbm_copy_probe)."""
+    L = load()
+    L.ilbm_copy_probe.argtypes = [C.c_int64, C.c_int, C.POINTER(C.c_double)]
+    ms = C.c_double(0.0)
+    _check(L.ilbm_copy_probe(int(nbytes), int(iters), C.byref(ms)), "copy_probe")
+    return ms.value

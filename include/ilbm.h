@@ -1,0 +1,145 @@
+/*
+ * ilbm.h -- C ABI of the MI355X IB-LBM level (libilbm_hip.so).
+ *
+ * Plain pointers and sizes only.  Device work is ordered on one HIP stream
+ * per level; every entry point returns an ilbm_status (0 = ok) and never
+ * aborts.  One host thread per handle.
+ *
+ * Correspondence with the reference's C++ call surface (iLBMReX, which has
+ * no FFI of its own; these are the members a maintainer would route here):
+ *
+ *   ilbm_level_create        AmrCoreLBM::AmrCoreLBM + ReadParameters +
+ *                            MakeNewLevelFromScratch + InitEquilibrium
+ *                            (Source/Core/AmrCoreLBM.cpp:190-356, 931-984,
+ *                            1188-1507, 2117-2164) and the IBMarkerDF ctor /
+ *                            build_markers (Source/IBM/IBMarkerDF.cpp:186-497)
+ *   ilbm_level_advance       Evolve's loop body for max_level = 0:
+ *                            timeStepWithSubcycling -> AdvancePhiAtLevel
+ *                            (Source/Core/AdvanceLBMAtLevel.cpp:20-417), the
+ *                            Sum(Rho) reduction (AmrCoreLBM.cpp:378) and
+ *                            ComputeIBForce (AmrCoreLBM.cpp:407-410, 456-701)
+ *   ilbm_level_get_f /       f_new[lev], macro_new[lev] comps 0..3 and
+ *   ilbm_level_get_macro /   forcing[lev] on the valid box (AmrCoreLBM.H:175-180)
+ *   ilbm_level_get_forcing
+ *   ilbm_level_mass_history  "Sum(Rho)" printed per coarse step (AmrCoreLBM.cpp:412-414)
+ *   ilbm_level_force_history force.dat rows (AmrCoreLBM.cpp:636-700)
+ *   ilbm_level_num_markers   IBMarkerDF::num_markers (IBMarkerDF.H:222)
+ *   ilbm_level_marker_force_sum  IBMarkerDF::last_marker_force_sum (IBMarkerDF.H:223)
+ *   ilbm_level_marker_state  marker positions / U / F (IBMarkerDF.H:182-195)
+ */
+#ifndef ILBM_H
+#define ILBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ILBM_ABI_VERSION 1
+
+typedef enum {
+  ILBM_OK = 0,
+  ILBM_ERR_INVALID = 1,     /* bad argument or case description */
+  ILBM_ERR_UNSUPPORTED = 2, /* lattice table / BC type / option not built */
+  ILBM_ERR_HIP = 3,         /* HIP runtime error */
+  ILBM_ERR_ALLOC = 4,       /* device allocation failed */
+  ILBM_ERR_STATE = 5        /* call not valid in the current state */
+} ilbm_status;
+
+typedef enum { ILBM_STORE_F64 = 0, ILBM_STORE_F32 = 1, ILBM_STORE_F16S = 2 } ilbm_storage;
+
+/* One level of an iLBMReX run; field names follow the reference's inputs
+ * keys (geometry.*, amrbc.*, lbm2d/lbm3d.*, lbmPhysicalParameters.*, ibm.*)
+ * and MarkerIBParams (Source/IBM/MarkerIBParams.H:8-132). */
+typedef struct {
+  int dim;                 /* 2 or 3 */
+  int n[3];                /* valid cells (n[2] = 1 in 2D) */
+  double plo[3];           /* geometry.prob_lo */
+  double dx[3];            /* cell size */
+  int periodic[3];         /* geometry.is_periodic */
+  int bc_lo[3], bc_hi[3];  /* AMReX BCType: 0 int_dir, 2 foextrap, 3 ext_dir, 1001 user_1, 1002 user_2 */
+  double bc_lo_val[3][4];  /* rho, ux, uy, uz */
+  double bc_hi_val[3][4];
+  int q;                   /* lbm*.ndir */
+  double cx[27], cy[27], cz[27], w[27]; /* lbm*.dir{x,y,z}, wis/wim */
+  double nu, U0;
+  int nmac;
+  int use_prescribed_force;
+  double prescribed_force[3];
+  int use_ib;
+  double x0, y0, z0, R;
+  int coupling_method;     /* 0 explicit, 1 ivc (wu_shu), 2 explicit_diag */
+  int delta_type;          /* 0/4: 4-point, 1/3: 3-point */
+  int loop_ns;
+  double mdf_relax, explicit_diag_eps;
+  int geometry_type;       /* 0 cylinder/sphere, 1 box (2D), 2 user pitching section */
+  int n_marker;
+  double rd;
+  double ubx, uby, ubz, omx, omy, omz;
+  double ivc_diag_reg;
+  int ivc_rebuild_matrix;
+  int renormalize_delta;
+  int force_eval_method;   /* 0 eulerian, 1 marker, 2 momentum_exchange (falls back to eulerian) */
+  double airfoil_chord, airfoil_thickness;
+  double pitch_mean, pitch_amplitude, pitch_frequency, pitch_phase, pivot_x;
+  double box_lo[3], box_hi[3], box_ds;
+  int box_lid_only;
+  double box_lid_u[3];
+  double box_wall_tol;
+  int storage;             /* ilbm_storage */
+} ilbm_case_desc;
+
+typedef struct ilbm_level ilbm_level;
+
+/* init_macro: host array [4][n2][n1][n0] (rho, ux, uy, uz; x fastest),
+ * the initdata values of the valid region. */
+int ilbm_level_create(const ilbm_case_desc* desc, const double* init_macro,
+                      ilbm_level** out);
+int ilbm_level_destroy(ilbm_level* L);
+
+/* Advances nsteps coarse steps.  With want_state != 0 the last step also
+ * writes f_new and macro_new for ilbm_level_get_f / get_macro. */
+int ilbm_level_advance(ilbm_level* L, int nsteps, int want_state);
+int ilbm_level_synchronize(ilbm_level* L);
+
+int ilbm_level_get_f(ilbm_level* L, double* host_out);       /* [q][cells] */
+int ilbm_level_get_macro(ilbm_level* L, double* host_out);   /* [4][cells] */
+int ilbm_level_get_forcing(ilbm_level* L, double* host_out); /* [3][cells] */
+
+/* Sum(Rho) after each step since creation; *n_out = steps recorded. */
+int ilbm_level_mass_history(ilbm_level* L, double* host_out, int max_rows, int* n_out);
+/* force.dat rows (step, time, 21 columns in 2D / 31 in 3D). */
+int ilbm_level_force_history(ilbm_level* L, double* host_out, int max_rows,
+                             int* n_rows, int* n_cols);
+
+int ilbm_level_num_markers(ilbm_level* L, int* out);
+int ilbm_level_marker_dv(ilbm_level* L, double* out);
+int ilbm_level_marker_state(ilbm_level* L, double* xyz, double* uvw, double* fxyz);
+int ilbm_level_marker_force_sum(ilbm_level* L, double out[3]);
+int ilbm_level_marker_torque(ilbm_level* L, double out[3]);
+
+/* Level facts: [0] tau, [1] dt, [2] time, [3] steps, [4] band cells,
+ * [5] storage bytes per population, [6] padded (1) or wrapped (0) layout. */
+int ilbm_level_info(ilbm_level* L, double out[8]);
+
+/* Timing hooks for measurement: average device time (ms) of the fused
+ * stream-collide kernel over the last advance() call, measured with HIP
+ * events on the level's stream, and the number of launches it covers. */
+int ilbm_level_kernel_time(ilbm_level* L, double* avg_ms, int* launches);
+int ilbm_level_set_timing(ilbm_level* L, int enable);
+
+/* Measurement helper (no counterpart in the reference): average time of a
+ * plain device copy of `bytes` bytes over `iters` launches, used to state
+ * achievable HBM bandwidth and to calibrate profiler byte counters. */
+int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms);
+
+const char* ilbm_status_string(int status);
+/* Text of the last error raised on the calling thread. */
+const char* ilbm_last_error(void);
+int ilbm_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
