@@ -113,6 +113,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="also run the 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only; --gpus {args.gpus} is not supported "
@@ -163,9 +165,12 @@ def main():
             log(f"[bench] {name}: {vs} steps")
             r, _ = run_level(case, pinit, 50, vs)
             variants[name] = r
-        copy_ms = native.load_probe(2 << 30, 20) if hasattr(native, "load_probe") else None
-        if copy_ms:
-            variants["copy_probe_gbs"] = 2 * (2 << 30) / (copy_ms * 1e-3) / 1e9
+        copy_ms = native.load_probe(2 << 30, 20)
+        variants["copy_probe_gbs"] = 2 * (2 << 30) / (copy_ms * 1e-3) / 1e9
+
+    if args.probe:
+        ms = native.load_probe(1 << 30, 3)
+        log(f"[bench] copy probe 1 GiB: {ms:.3f} ms ({2 * (1 << 30) / (ms * 1e-3) / 1e9:.0f} GB/s)")
 
     cpu = None
     if not args.no_cpu:
@@ -196,7 +201,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_step<D3Q19,StF32,padded>",
+            "kernel": "k_step<D3Q19,StF32,slabs> (fused pull-stream + macros + BGK)",
             "achieved": round(head["kernel_gbs"], 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -70,6 +70,13 @@ struct IbArgs {
   double* rhs;          // [3][nm]
   // reductions
   double* sums;         // [0..2] last marker force sum, [3..5] torque
+  // single-iteration fast path (loop_ns == 1 or IVC): EulerVel is the
+  // macro velocity and EulerForceTotal is the one iteration's force
+  int ev_from_macro;
+  int fuse_explicit;    // explicit / diag force evaluated by k_ib_interp
+  int first_iter;
+  const int* ne_cells;  // band cells that receive spreading entries
+  int n_ne;
 };
 
 __device__ inline double delta_fn(double xf, double xp, double h, int type) {
@@ -167,12 +174,12 @@ __device__ inline double ev_value(const IbArgs& a, const int p[3], int comp) {
   if (phys) return macro_fill(a, p, comp + 1);
   int64_t b;
   if (!band_index(a, c, &b)) return 0.0;
+  if (a.ev_from_macro) return (comp == 2 && a.dim != 3) ? 0.0 : a.Mband[(comp + 1) * a.band_cells + b];
   return a.EV[comp * a.band_cells + b];
 }
 
 __global__ void k_ib_begin(IbArgs a) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b == 0) for (int c = 0; c < 6; ++c) a.sums[c] = 0.0;
   if (b >= a.band_cells) return;
   a.EV[b] = a.Mband[a.band_cells + b];
   a.EV[a.band_cells + b] = a.Mband[2 * a.band_cells + b];
@@ -222,6 +229,67 @@ __global__ void k_ib_weights(IbArgs a) {
   a.scale[nm + m] = (a.renorm_spread && part > 0.0) ? 1.0 / part : 1.0;  // spreading factor
 }
 
+__device__ inline void target_velocity(const IbArgs& a, int m, double* ux, double* uy, double* uz) {
+  const int nm = a.nm;
+  if (a.geometry == 2) {
+    *ux = a.ut[m]; *uy = a.ut[nm + m]; *uz = a.ut[2 * nm + m];
+    return;
+  }
+  const double x = a.xm[m], y = a.xm[nm + m], z = (a.dim == 3) ? a.xm[2 * nm + m] : 0.0;
+  if (a.geometry == 1) {  // box_target_velocity (:147-161)
+    *ux = *uy = *uz = 0.0;
+    if (fabs(y - a.box_yhi) <= a.wall_tol) { *ux = a.lid_u[0]; *uy = a.lid_u[1]; *uz = a.lid_u[2]; }
+    return;
+  }
+  const double z0 = (a.dim == 3) ? a.z0 : 0.0;  // rigid_target_velocity (:163-182)
+  const double rx = x - a.x0, ry = y - a.y0, rz = z - z0;
+  const double cx = a.om[1] * rz - a.om[2] * ry;
+  const double cy = a.om[2] * rx - a.om[0] * rz;
+  const double cz = a.om[0] * ry - a.om[1] * rx;
+  *ux = a.ub[0] + cx; *uy = a.ub[1] + cy; *uz = a.ub[2] + cz;
+}
+
+// compute_lagrangian_force_explicit (:741-873): F += relax*(U_t - U)/denom
+__device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, double Uy_m, double Uz_m,
+                                      double& F0, double& F1, double& F2) {
+  const int nm = a.nm;
+  double Ux, Uy, Uz;
+  target_velocity(a, m, &Ux, &Uy, &Uz);
+  double denom = a.dt;
+  if (a.coupling == 2) {
+    const double dV = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+    const int b0 = a.base[m], b1 = a.base[nm + m], b2 = a.base[2 * nm + m];
+    double s2 = 0.0;
+    for (int ii = 0; ii < 5; ++ii) {
+      const int ip = b0 + ii - 2;
+      if (ip < 0 || ip > a.n[0] - 1) continue;
+      const double tU = a.wts[(0 * 5 + ii) * nm + m];
+      for (int jj = 0; jj < 5; ++jj) {
+        const int jp = b1 + jj - 2;
+        if (jp < 0 || jp > a.n[1] - 1) continue;
+        const double tV = a.wts[(1 * 5 + jj) * nm + m];
+        if (a.dim == 3) {
+          for (int kk = 0; kk < 5; ++kk) {
+            const int kp = b2 + kk - 2;
+            if (kp < 0 || kp > a.n[2] - 1) continue;
+            const double tW = a.wts[(2 * 5 + kk) * nm + m];
+            const double dval = tU * tV * tW;
+            s2 += dval * dval * dV;
+          }
+        } else {
+          const double dval = tU * tV;
+          s2 += dval * dval * dV;
+        }
+      }
+    }
+    const double mkk = a.dv * s2;
+    denom = a.dt * fmax(mkk, a.diag_eps);
+  }
+  F0 += a.relax * (Ux - Ux_m) / denom;
+  F1 += a.relax * (Uy - Uy_m) / denom;
+  F2 += a.relax * (Uz - Uz_m) / denom;
+}
+
 // One wavefront per marker: the lanes evaluate the stencil products in
 // parallel, then four lanes accumulate them in the reference's (ii, jj, kk)
 // order, so the sums are bit-identical to the serial loop (:642-681).
@@ -257,8 +325,16 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) {
   __syncthreads();
   if (active && lane < 4) {
     double acc = 0.0;
-    for (int t = 0; t < npts; ++t)
-      if (valid[wv][t]) acc += prod[wv][lane][t];
+    int t = 0;
+    for (; t + 5 <= npts; t += 5) {
+      double v[5];
+      unsigned char ok[5];
+#pragma unroll
+      for (int u = 0; u < 5; ++u) { v[u] = prod[wv][lane][t + u]; ok[u] = valid[wv][t + u]; }
+#pragma unroll
+      for (int u = 0; u < 5; ++u)
+        if (ok[u]) acc += v[u];
+    }
     prod[wv][lane][127] = acc;
   }
   __syncthreads();
@@ -271,69 +347,19 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) {
     double U0 = 0.0, U1 = 0.0, U2 = 0.0;
     U0 += u; U1 += v; U2 += w;
     a.U[m] = U0; a.U[nm + m] = U1; a.U[2 * nm + m] = U2;
-    a.F[m] = 0.0; a.F[nm + m] = 0.0; a.F[2 * nm + m] = 0.0;
+    double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+    if (a.fuse_explicit) explicit_force(a, m, U0, U1, U2, F0, F1, F2);
+    a.F[m] = F0; a.F[nm + m] = F1; a.F[2 * nm + m] = F2;
   }
-}
-
-__device__ inline void target_velocity(const IbArgs& a, int m, double* ux, double* uy, double* uz) {
-  const int nm = a.nm;
-  if (a.geometry == 2) {
-    *ux = a.ut[m]; *uy = a.ut[nm + m]; *uz = a.ut[2 * nm + m];
-    return;
-  }
-  const double x = a.xm[m], y = a.xm[nm + m], z = (a.dim == 3) ? a.xm[2 * nm + m] : 0.0;
-  if (a.geometry == 1) {  // box_target_velocity (:147-161)
-    *ux = *uy = *uz = 0.0;
-    if (fabs(y - a.box_yhi) <= a.wall_tol) { *ux = a.lid_u[0]; *uy = a.lid_u[1]; *uz = a.lid_u[2]; }
-    return;
-  }
-  const double z0 = (a.dim == 3) ? a.z0 : 0.0;  // rigid_target_velocity (:163-182)
-  const double rx = x - a.x0, ry = y - a.y0, rz = z - z0;
-  const double cx = a.om[1] * rz - a.om[2] * ry;
-  const double cy = a.om[2] * rx - a.om[0] * rz;
-  const double cz = a.om[0] * ry - a.om[1] * rx;
-  *ux = a.ub[0] + cx; *uy = a.ub[1] + cy; *uz = a.ub[2] + cz;
 }
 
 __global__ void k_ib_force_explicit(IbArgs a) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.nm) return;
   const int nm = a.nm;
-  double Ux, Uy, Uz;
-  target_velocity(a, m, &Ux, &Uy, &Uz);
-  double denom = a.dt;
-  if (a.coupling == 2) {
-    const double dV = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
-    const int b0 = a.base[m], b1 = a.base[nm + m], b2 = a.base[2 * nm + m];
-    double s2 = 0.0;
-    for (int ii = 0; ii < 5; ++ii) {
-      const int ip = b0 + ii - 2;
-      if (ip < 0 || ip > a.n[0] - 1) continue;
-      const double tU = a.wts[(0 * 5 + ii) * nm + m];
-      for (int jj = 0; jj < 5; ++jj) {
-        const int jp = b1 + jj - 2;
-        if (jp < 0 || jp > a.n[1] - 1) continue;
-        const double tV = a.wts[(1 * 5 + jj) * nm + m];
-        if (a.dim == 3) {
-          for (int kk = 0; kk < 5; ++kk) {
-            const int kp = b2 + kk - 2;
-            if (kp < 0 || kp > a.n[2] - 1) continue;
-            const double tW = a.wts[(2 * 5 + kk) * nm + m];
-            const double dval = tU * tV * tW;
-            s2 += dval * dval * dV;
-          }
-        } else {
-          const double dval = tU * tV;
-          s2 += dval * dval * dV;
-        }
-      }
-    }
-    const double mkk = a.dv * s2;
-    denom = a.dt * fmax(mkk, a.diag_eps);
-  }
-  a.F[m] += a.relax * (Ux - a.U[m]) / denom;
-  a.F[nm + m] += a.relax * (Uy - a.U[nm + m]) / denom;
-  a.F[2 * nm + m] += a.relax * (Uz - a.U[2 * nm + m]) / denom;
+  double F0 = a.F[m], F1 = a.F[nm + m], F2 = a.F[2 * nm + m];
+  explicit_force(a, m, a.U[m], a.U[nm + m], a.U[2 * nm + m], F0, F1, F2);
+  a.F[m] = F0; a.F[nm + m] = F1; a.F[2 * nm + m] = F2;
 }
 
 // IVC right-hand side b = (U_target - U) / dt (:1276-1296)
@@ -382,74 +408,98 @@ __global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
   a.ent_w[e] = (a.dim == 3) ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
 }
 
-__global__ void k_ib_spread(IbArgs a) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= a.band_cells) return;
+// One wavefront per band cell that receives contributions: lanes form the
+// products, lane 0 accumulates them in list order (marker order within each
+// periodic image group, direct contributions first), bit-identical to the
+// serial scatter followed by SumBoundary.
+constexpr int kSpreadWaves = 4;
+constexpr int kSpreadChunk = 128;
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread(IbArgs a) {
+  __shared__ double pr[kSpreadWaves][3][kSpreadChunk];
+  __shared__ int gp[kSpreadWaves][kSpreadChunk];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blockIdx.x * kSpreadWaves + wv;
+  const bool active = ci < a.n_ne;
   const int nm = a.nm;
-  const int e0 = a.rowptr[b], e1 = a.rowptr[b + 1];
+  const int b = active ? a.ne_cells[ci] : 0;
+  const int e0 = active ? a.rowptr[b] : 0, e1 = active ? a.rowptr[b + 1] : 0;
   double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
   int cur = 0;
-  for (int e = e0; e < e1; ++e) {
-    const int grp = a.ent_code[e] >> 9;
-    if (grp != cur) {
-      if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
-      gs[0] = gs[1] = gs[2] = 0.0;
-      cur = grp;
+  // every wave of the block runs the same number of chunk rounds
+  int my_rounds = (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
+  __shared__ int rounds[kSpreadWaves];
+  if (lane == 0) rounds[wv] = my_rounds;
+  __syncthreads();
+  int nr = 0;
+  for (int w2 = 0; w2 < kSpreadWaves; ++w2) nr = rounds[w2] > nr ? rounds[w2] : nr;
+  for (int r = 0; r < nr; ++r) {
+    const int base = e0 + r * kSpreadChunk;
+    for (int t = lane; t < kSpreadChunk; t += 64) {
+      const int e = base + t;
+      if (e < e1) {
+        const int m = a.ent_m[e];
+        const double dval = a.ent_w[e];
+        pr[wv][0][t] = dval * (a.F[m] * a.dv);
+        pr[wv][1][t] = dval * (a.F[nm + m] * a.dv);
+        pr[wv][2][t] = dval * (a.F[2 * nm + m] * a.dv);
+        gp[wv][t] = a.ent_code[e] >> 9;
+      }
     }
-    const int m = a.ent_m[e];
-    const double dval = a.ent_w[e];
-    const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
-    double* acc = (grp == 0) ? tot : gs;
-    acc[0] += dval * fx;
-    acc[1] += dval * fy;
-    if (a.dim == 3) acc[2] += dval * fz;
+    __syncthreads();
+    if (active && lane == 0) {
+      const int cnt = (e1 - base < kSpreadChunk) ? e1 - base : kSpreadChunk;
+      for (int t = 0; t < cnt; ++t) {
+        const int grp = gp[wv][t];
+        if (grp != cur) {
+          if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+          gs[0] = gs[1] = gs[2] = 0.0;
+          cur = grp;
+        }
+        double* acc = (grp == 0) ? tot : gs;
+        acc[0] += pr[wv][0][t];
+        acc[1] += pr[wv][1][t];
+        if (a.dim == 3) acc[2] += pr[wv][2][t];
+      }
+    }
+    __syncthreads();
   }
-  if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
-  a.EFI[b] = tot[0];
-  a.EFI[a.band_cells + b] = tot[1];
-  a.EFI[2 * a.band_cells + b] = tot[2];
+  if (active && lane == 0) {
+    if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+    a.EFI[b] = tot[0];
+    a.EFI[a.band_cells + b] = tot[1];
+    a.EFI[2 * a.band_cells + b] = tot[2];
+  }
 }
 
-// marker force integral in marker order (:1781-1819) and moments (:1500-1503).
-// Products are formed in parallel and staged through LDS; three lanes then
-// accumulate them serially in marker order, as the reference's host loop.
-constexpr int kSumChunk = 1024;
+// Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
+// iterations) and the moments r x (F dv) (:1500-1503), with their sum as
+// the body torque.  One block, fixed reduction tree.
 __global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) {
-  __shared__ double st[6][kSumChunk];
+  __shared__ double lds[6][16];
   const int nm = a.nm;
   const int t = threadIdx.x;
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const double z0 = (a.dim == 3) ? a.z0 : 0.0;
-  for (int c0 = 0; c0 < nm; c0 += kSumChunk) {
-    const int m = c0 + t;
-    if (m < nm) {
-      const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
-      const double rx = a.xm[m] - a.x0, ry = a.xm[nm + m] - a.y0;
-      const double rz = (a.dim == 3) ? a.xm[2 * nm + m] - z0 : 0.0;
-      const double mx = ry * fz - rz * fy, my = rz * fx - rx * fz, mz = rx * fy - ry * fx;
-      a.Mm[m] = mx; a.Mm[nm + m] = my; a.Mm[2 * nm + m] = mz;
-      st[0][t] = fx; st[1][t] = fy; st[2][t] = fz;
-      st[3][t] = mx; st[4][t] = my; st[5][t] = mz;
-    }
-    __syncthreads();
-    if (t < 6) {
-      const int cnt = (nm - c0 < kSumChunk) ? nm - c0 : kSumChunk;
-      double s = acc[t];
-      int r = 0;
-      for (; r + 8 <= cnt; r += 8) {
-        double v[8];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) v[u] = st[t][r + u];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) s += v[u];
-      }
-      for (; r < cnt; ++r) s += st[t][r];
-      acc[t] = s;
-    }
-    __syncthreads();
+  for (int m = t; m < nm; m += blockDim.x) {
+    const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
+    const double rx = a.xm[m] - a.x0, ry = a.xm[nm + m] - a.y0;
+    const double rz = (a.dim == 3) ? a.xm[2 * nm + m] - z0 : 0.0;
+    const double mx = ry * fz - rz * fy, my = rz * fx - rx * fz, mz = rx * fy - ry * fx;
+    a.Mm[m] = mx; a.Mm[nm + m] = my; a.Mm[2 * nm + m] = mz;
+    acc[0] += fx; acc[1] += fy; acc[2] += fz;
+    acc[3] += mx; acc[4] += my; acc[5] += mz;
   }
-  if (t < 3) a.sums[t] += acc[t];
-  else if (t < 6) a.sums[t] = acc[t];
+  for (int c = 0; c < 6; ++c) {
+    const double r = wave_sum(acc[c]);
+    if ((t & 63) == 0) lds[c][t >> 6] = r;
+  }
+  __syncthreads();
+  if (t < 6) {
+    double s = 0.0;
+    for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) s += lds[t][wv];
+    if (t < 3) a.sums[t] = a.first_iter ? s : a.sums[t] + s;
+    else a.sums[t] = s;
+  }
 }
 
 __global__ void k_ib_correct(IbArgs a) {
@@ -469,47 +519,111 @@ __global__ void k_ib_export(IbArgs a) {
   for (int c = 0; c < 3; ++c) a.Fband[c * a.band_cells + b] = a.dt * rho * a.EFT[c * a.band_cells + b];
 }
 
-// Sum of the exported force over the band (forcing[lev].sum, AmrCoreLBM.cpp:491-494)
-// and the ComputeIBForce record (:456-631), appended to a device history.
-struct ForceRecArgs {
-  const double* Fband;
-  int64_t band_cells;
+// ComputeIBForce inputs (AmrCoreLBM.cpp:456-631) for the fused record.
+struct RecArgs {
+  double* partial;      // [3][blocks] band partial sums of the exported force
   const double* sums;   // marker sums [0..2]
-  double* state;        // [0] have_prev, [1..3] prev body force
-  double* hist;         // rows of ncol doubles
-  int row, ncol, dim, method;
-  double time, step, dV, dt, U0, R;
+  double* state;        // [0] have_prev, [1..3] previous body force
+  double* hist;         // history rows of ncol doubles
+  int row, ncol, method;
+  double time, step, dV, U0, R;
 };
 
-__global__ void __launch_bounds__(1024) k_force_record(ForceRecArgs a) {
-  __shared__ double lds[3][16];
-  double s[3] = {0.0, 0.0, 0.0};
-  for (int64_t b = threadIdx.x; b < a.band_cells; b += blockDim.x)
-    for (int c = 0; c < 3; ++c) s[c] += a.Fband[c * a.band_cells + b];
+__device__ void write_force_record(const IbArgs& a, const RecArgs& r, const double sum[3]);
+
+// Export F = dt * rho * EulerForceTotal (:1889-1912) fused with the Guo
+// term of the coming collision on the same cell (collide_forced.H:22-50),
+// which the stream-collide kernel left out because F was not yet known.
+// Each block also sums its exported force for the step's force record.
+template <class LAT, class ST>
+__global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, void* gbuf, double tau, RecArgs rec) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  __shared__ double lds[3][4];
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (b < a.band_cells) {
+    const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
+    const double rho = a.Mband[b];
+    fx = a.dt * rho * tot[b];
+    fy = a.dt * rho * tot[a.band_cells + b];
+    fz = a.dt * rho * tot[2 * a.band_cells + b];
+    a.Fband[b] = fx;
+    a.Fband[a.band_cells + b] = fy;
+    a.Fband[2 * a.band_cells + b] = fz;
+    if (!(fx == 0.0 && fy == 0.0 && fz == 0.0)) {
+      const int bi = (int)(b % a.band_n[0]);
+      const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+      const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+      const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
+      const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
+      const T uz = T(a.Mband[3 * a.band_cells + b]);
+      const T fac = T(1.0) - T(0.5) / T(tau);
+      R* gb = static_cast<R*>(gbuf);
+      const int64_t self = g.idx(i, j, k);
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) {
+        const int64_t o = (int64_t)q * g.qs + self;
+        T gq = ST::load(gb, o, T(LAT::w[q]));
+        gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
+        ST::store(gb, o, gq, T(LAT::w[q]));
+      }
+    }
+  }
+  const double v[3] = {fx, fy, fz};
   for (int c = 0; c < 3; ++c) {
-    const double r = wave_sum(s[c]);
+    const double r = wave_sum(v[c]);
     if ((threadIdx.x & 63) == 0) lds[c][threadIdx.x >> 6] = r;
   }
   __syncthreads();
-  if (threadIdx.x != 0) return;
-  double sum[3] = {0.0, 0.0, 0.0};
-  for (int c = 0; c < 3; ++c)
-    for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) sum[c] += lds[c][wv];
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 3; ++c) {
+      double s = 0.0;
+      for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) s += lds[c][wv];
+      rec.partial[(int64_t)c * gridDim.x + blockIdx.x] = s;
+    }
+  }
+}
+
+// Adds the export kernel's block sums in block order and writes the
+// force.dat row of the step.
+__global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) {
+  __shared__ double lds[3][4];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int t = threadIdx.x; t < nblk; t += blockDim.x)
+    for (int c = 0; c < 3; ++c) acc[c] += rec.partial[(int64_t)c * nblk + t];
+  for (int c = 0; c < 3; ++c) {
+    const double r = wave_sum(acc[c]);
+    if ((threadIdx.x & 63) == 0) lds[c][threadIdx.x >> 6] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c)
+      for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) sum[c] += lds[c][wv];
+    write_force_record(a, rec, sum);
+  }
+}
+
+// The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of
+// the exported force and the marker sums, appended to the history.
+__device__ void write_force_record(const IbArgs& a, const RecArgs& rc, const double sum_in[3]) {
+  double sum[3] = {sum_in[0], sum_in[1], sum_in[2]};
   if (a.dim != 3) sum[2] = 0.0;
-  const double Fxe = -(sum[0] * a.dV) / a.dt, Fye = -(sum[1] * a.dV) / a.dt, Fze = -(sum[2] * a.dV) / a.dt;
-  const double Fxm = -a.sums[0], Fym = -a.sums[1], Fzm = -a.sums[2];
+  const double Fxe = -(sum[0] * rc.dV) / a.dt, Fye = -(sum[1] * rc.dV) / a.dt, Fze = -(sum[2] * rc.dV) / a.dt;
+  const double Fxm = -rc.sums[0], Fym = -rc.sums[1], Fzm = -rc.sums[2];
   const double Fxx = Fxe, Fyx = Fye, Fzx = Fze;
   double Fxb = Fxe, Fyb = Fye, Fzb = Fze;
-  if (a.method == 1) { Fxb = Fxm; Fyb = Fym; Fzb = Fzm; }
-  else if (a.method == 2) { Fxb = Fxx; Fyb = Fyx; Fzb = Fzx; }
+  if (rc.method == 1) { Fxb = Fxm; Fyb = Fym; Fzb = Fzm; }
+  else if (rc.method == 2) { Fxb = Fxx; Fyb = Fyx; Fzb = Fzx; }
   double Fxt = Fxb, Fyt = Fyb, Fzt = Fzb;
-  if (a.state[0] != 0.0) {
-    Fxt = 0.5 * (a.state[1] + Fxb); Fyt = 0.5 * (a.state[2] + Fyb); Fzt = 0.5 * (a.state[3] + Fzb);
+  if (rc.state[0] != 0.0) {
+    Fxt = 0.5 * (rc.state[1] + Fxb); Fyt = 0.5 * (rc.state[2] + Fyb); Fzt = 0.5 * (rc.state[3] + Fzb);
   }
-  a.state[0] = 1.0; a.state[1] = Fxb; a.state[2] = Fyb; a.state[3] = Fzb;
-  const double D = 2.0 * a.R;
+  rc.state[0] = 1.0; rc.state[1] = Fxb; rc.state[2] = Fyb; rc.state[3] = Fzb;
+  const double D = 2.0 * rc.R;
   const double pi = 3.14159265358979323846;
-  const double denom = (a.dim == 3) ? 0.5 * 1.0 * a.U0 * a.U0 * (pi * a.R * a.R) : 0.5 * 1.0 * a.U0 * a.U0 * D;
+  const double denom = (a.dim == 3) ? 0.5 * 1.0 * rc.U0 * rc.U0 * (pi * rc.R * rc.R) : 0.5 * 1.0 * rc.U0 * rc.U0 * D;
   double C[15];
   for (int t = 0; t < 15; ++t) C[t] = 0.0;
   if (denom > 0.0) {
@@ -518,10 +632,10 @@ __global__ void __launch_bounds__(1024) k_force_record(ForceRecArgs a) {
     C[8] = Fxx / denom; C[9] = Fyx / denom;
     C[10] = Fzt / denom; C[11] = Fzb / denom; C[12] = Fze / denom; C[13] = Fzm / denom; C[14] = Fzx / denom;
   }
-  const double tconv = (D > 0.0) ? (a.time * a.U0 / D) : 0.0;
-  double* r = a.hist + (int64_t)a.row * a.ncol;
+  const double tconv = (D > 0.0) ? (rc.time * rc.U0 / D) : 0.0;
+  double* r = rc.hist + (int64_t)rc.row * rc.ncol;
   int n = 0;
-  r[n++] = a.step; r[n++] = a.time;
+  r[n++] = rc.step; r[n++] = rc.time;
   r[n++] = Fxt; r[n++] = Fyt; r[n++] = C[0]; r[n++] = C[1];
   r[n++] = Fxb; r[n++] = Fyb; r[n++] = C[2]; r[n++] = C[3];
   r[n++] = Fxe; r[n++] = Fye; r[n++] = C[4]; r[n++] = C[5];

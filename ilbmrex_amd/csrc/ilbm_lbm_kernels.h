@@ -16,15 +16,15 @@
 // by k_guo_fixup; the reference adds it after the BGK update as well
 // (collide_forced.H:20-50), so FP64 results are unchanged.
 //
-// Ghosts.  When every dimension is periodic the arrays carry no ghost
-// cells and k_step wraps indices.  Otherwise each dimension carries one
-// ghost layer; k_fill_ghosts writes it between steps with the values the
-// reference's stream reads there: periodic images of post-collision data
-// (FillBoundary, AdvanceLBMAtLevel.cpp:244), halfway bounce-back for user_1
-// walls (:265-373), and the pre-collision FillPatch values of
-// mesoscopicBcFill / foextrap for everything else (bc_fill.H:303-347).  The
-// pre-collision values of the domain's outer layer come from a small face
-// buffer that k_step writes while it has f^{n+1} in registers.
+// Ghosts.  The arrays carry no ghost cells.  Periodic dimensions wrap
+// (FillBoundary of post-collision data, AdvanceLBMAtLevel.cpp:244).  For
+// non-periodic faces k_fill_ghosts writes, between steps, dense per-face
+// slabs holding what the reference's stream reads in its ghost cells:
+// halfway bounce-back for user_1 walls (:265-373) and the pre-collision
+// FillPatch values of mesoscopicBcFill / foextrap elsewhere
+// (bc_fill.H:303-347).  The pre-collision values of the domain's outer
+// layer come from a face buffer that k_step writes while it has f^{n+1} in
+// registers.
 #pragma once
 
 #include "ilbm_lattice.h"
@@ -58,8 +58,35 @@ __host__ __device__ inline int64_t face_index(const GridGeom& g, int d, int i, i
   return (int64_t)c[1] * g.n[0] + c[0];
 }
 
+// Ghost cells of the non-periodic faces, stored densely per face slab.  A
+// cell outside the domain in non-periodic dimensions d1 < d2 belongs to the
+// slab of d1; in the other dimensions a slab spans [-1, n] (or [0, n) for a
+// lower non-periodic dimension).
+struct Slabs {
+  int nface;
+  int slab_of[3][2];       // face slab index of (dim, side), -1 if periodic
+  int face_d[6], face_side[6];
+  int face_lo[6][3], face_n[6][3];
+  int64_t face_off[7];     // prefix sums of slab sizes; face_off[nface] = total
+};
+
+__host__ __device__ inline int64_t slab_index(const Slabs& S, const int c[3], int dim, const int n[3]) {
+  int d = -1;
+  for (int e = 0; e < dim; ++e)
+    if (S.slab_of[e][0] >= 0 && (c[e] < 0 || c[e] > n[e] - 1)) { d = e; break; }
+  const int f = S.slab_of[d][c[d] < 0 ? 0 : 1];
+  int64_t t = S.face_off[f], mul = 1;
+  for (int e = 0; e < 3; ++e) {
+    if (e == d) continue;
+    t += (int64_t)(c[e] - S.face_lo[f][e]) * mul;
+    mul *= S.face_n[f][e];
+  }
+  return t;
+}
+
 struct StepArgs {
   GridGeom g;
+  Slabs slabs;
   const void* gin;
   void* gout;
   double tau, inv_tau;
@@ -167,6 +194,16 @@ __device__ inline typename ST::compute_t bgk(typename ST::compute_t f, typename 
   else return f - (f - feq) * T(inv_tau);
 }
 
+// Per-block partial of Sum(Rho); k_sum_partials adds them in block order.
+// (A last-block-done reduction would need an agent-scope release per block,
+// which on a multi-XCD device writes back the XCD's L2 -- far costlier than
+// one more small kernel.)
+__device__ inline void finish_mass(const StepArgs& a, double s, double* lds) {
+  (void)lds;
+  if (threadIdx.x == 0 && threadIdx.y == 0)
+    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+}
+
 // ---------------------------------------------------------------------------
 // k_step: pull-stream + forced macros + BGK collision, one pass.
 // ---------------------------------------------------------------------------
@@ -184,9 +221,10 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   if (active) {
     const R* __restrict__ gin = static_cast<const R*>(a.gin);
     R* __restrict__ gout = static_cast<R*>(a.gout);
-    // source offsets for c = -1, 0, +1 (source coordinate = x - c)
+    // source offsets for c = -1, 0, +1 (source coordinate = x - c); periodic
+    // dimensions wrap
     int64_t ox[3], oy[3], oz[3];
-    if constexpr (WRAP) {
+    {
       const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
       const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
       ox[0] = xp; ox[1] = i; ox[2] = xm;
@@ -197,20 +235,54 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
       } else {
         oz[0] = oz[1] = oz[2] = 0;
       }
-    } else {
-      ox[0] = i + 1; ox[1] = i; ox[2] = i - 1;
-      oy[0] = (int64_t)(j + 1) * g.js; oy[1] = (int64_t)j * g.js; oy[2] = (int64_t)(j - 1) * g.js;
-      if constexpr (LAT::D == 3) {
-        oz[0] = (int64_t)(k + 1) * g.ks; oz[1] = (int64_t)k * g.ks; oz[2] = (int64_t)(k - 1) * g.ks;
-      } else {
-        oz[0] = oz[1] = oz[2] = 0;
+    }
+    // A source outside a non-periodic face is a ghost cell; its value (the
+    // pre-collision FillPatch value, or bounce-back) sits in the slabs that
+    // follow the Q population planes of the same buffer.  The BC functor
+    // clamps the mirror in every dimension (bc_fill.H:28-40), so a ghost
+    // that is also outside a periodic dimension is not wrapped.  All loads
+    // are issued unconditionally; only their offsets are selected.
+    bool outx[3] = {false, false, false}, outy[3] = {false, false, false}, outz[3] = {false, false, false};
+    int64_t tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    if constexpr (!WRAP) {
+      const int cc[3] = {i, j, k};
+      bool* outs[3] = {outx, outy, outz};
+#pragma unroll
+      for (int d = 0; d < LAT::D; ++d) {
+        if (!a.nonper[d]) continue;
+        outs[d][0] = (cc[d] == g.n[d] - 1);
+        outs[d][2] = (cc[d] == 0);
+        const int sd = (cc[d] == 0) ? 0 : 1;
+        const int f = a.slabs.slab_of[d][sd];
+        int64_t t = a.slabs.face_off[f], mul = 1;
+#pragma unroll
+        for (int e = 0; e < 3; ++e) {
+          if (e == d) continue;
+          t += (int64_t)(cc[e] - a.slabs.face_lo[f][e]) * mul;
+          mu[d][e] = mul;
+          mul *= a.slabs.face_n[f][e];
+        }
+        tb[d] = t;
       }
     }
+    const int64_t gofs = (int64_t)LAT::Q * g.qs;          // start of the slabs
+    const int64_t npad = a.slabs.face_off[a.slabs.nface];
     T f[LAT::Q];
 #pragma unroll
     for (int q = 0; q < LAT::Q; ++q) {
-      const int64_t src = g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
-      f[q] = ST::load(gin, (int64_t)q * g.qs + src, T(LAT::w[q]));
+      constexpr int dummy = 0;
+      (void)dummy;
+      int64_t off = (int64_t)q * g.qs + g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+      if constexpr (!WRAP) {
+        const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
+        // slab of the lowest non-periodic dimension the source leaves
+        const int64_t t0 = tb[0] - LAT::cy[q] * mu[0][1] - LAT::cz[q] * mu[0][2];
+        const int64_t t1 = tb[1] - LAT::cx[q] * mu[1][0] - LAT::cz[q] * mu[1][2];
+        const int64_t t2 = tb[2] - LAT::cx[q] * mu[2][0] - LAT::cy[q] * mu[2][1];
+        const int64_t t = o0 ? t0 : (o1 ? t1 : t2);
+        off = (o0 | o1 | o2) ? gofs + (int64_t)q * npad + t : off;
+      }
+      f[q] = ST::load(gin, off, T(LAT::w[q]));
     }
     // moments in the reference's order (calculateMacroForcing.H:21-27)
     T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
@@ -290,8 +362,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
   }
   const double s = block_sum(rho_acc, lds);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+  finish_mass(a, s, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -380,48 +451,7 @@ __global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
     }
   }
   const double s = block_sum(rho_acc, lds);
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
-}
-
-// ---------------------------------------------------------------------------
-// k_guo_fixup: g += Guo(u^n, F^{n+1}) on IB band cells (collide_forced.H:22-50
-// evaluated after the IB force of this step is known).
-// ---------------------------------------------------------------------------
-struct FixupArgs {
-  GridGeom g;
-  void* gbuf;
-  double tau;
-  int band_lo[3], band_n[3];
-  int64_t band_cells;
-  const double* Fband;
-  const double* Mband;
-};
-
-template <class LAT, class ST>
-__global__ void __launch_bounds__(kBlock) k_guo_fixup(FixupArgs a) {
-  using T = typename ST::compute_t;
-  using R = typename ST::raw;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= a.band_cells) return;
-  const double fx = a.Fband[b], fy = a.Fband[a.band_cells + b], fz = a.Fband[2 * a.band_cells + b];
-  if (fx == 0.0 && fy == 0.0 && fz == 0.0) return;
-  const int bi = (int)(b % a.band_n[0]);
-  const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
-  const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
-  const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
-  const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
-  const T uz = T(a.Mband[3 * a.band_cells + b]);
-  const T fac = T(1.0) - T(0.5) / T(a.tau);
-  R* gb = static_cast<R*>(a.gbuf);
-  const int64_t self = a.g.idx(i, j, k);
-#pragma unroll
-  for (int q = 0; q < LAT::Q; ++q) {
-    const int64_t o = (int64_t)q * a.g.qs + self;
-    T gq = ST::load(gb, o, T(LAT::w[q]));
-    gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
-    ST::store(gb, o, gq, T(LAT::w[q]));
-  }
+  finish_mass(a, s, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -437,41 +467,23 @@ struct BcSpec {
 
 struct GhostArgs {
   GridGeom g;
-  void* gbuf;
+  void* gbuf;            // post-collision populations; slabs follow the Q planes
   FaceBuf faces;
   BcSpec bc;
-  int64_t npad;
+  Slabs slabs;
 };
 
-__device__ inline void pad_cell(const GridGeom& g, int64_t t, int c[3]) {
-  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
-  if (g.dim == 2) {
-    const int64_t fx = ny + 2;
-    if (t < 2 * fx) { c[0] = (t < fx) ? -1 : nx; c[1] = (int)(t % fx) - 1; c[2] = 0; return; }
-    t -= 2 * fx;
-    c[1] = (t < nx) ? -1 : ny; c[0] = (int)(t % nx); c[2] = 0;
-    return;
+__device__ inline void slab_cell(const GhostArgs& a, int64_t t, int c[3]) {
+  const Slabs& S = a.slabs;
+  int f = 0;
+  while (f + 1 < S.nface && t >= S.face_off[f + 1]) ++f;
+  int64_t r = t - S.face_off[f];
+  for (int e = 0; e < 3; ++e) {
+    if (e == S.face_d[f]) { c[e] = S.face_side[f] ? a.g.n[e] : -1; continue; }
+    const int n = S.face_n[f][e];
+    c[e] = S.face_lo[f][e] + (int)(r % n);
+    r /= n;
   }
-  const int64_t fx = (int64_t)(ny + 2) * (nz + 2);
-  if (t < 2 * fx) {
-    c[0] = (t < fx) ? -1 : nx;
-    const int64_t r = t % fx;
-    c[1] = (int)(r % (ny + 2)) - 1; c[2] = (int)(r / (ny + 2)) - 1;
-    return;
-  }
-  t -= 2 * fx;
-  const int64_t fy = (int64_t)nx * (nz + 2);
-  if (t < 2 * fy) {
-    c[1] = (t < fy) ? -1 : ny;
-    const int64_t r = t % fy;
-    c[0] = (int)(r % nx); c[2] = (int)(r / nx) - 1;
-    return;
-  }
-  t -= 2 * fy;
-  const int64_t fz = (int64_t)nx * ny;
-  c[2] = (t < fz) ? -1 : nz;
-  const int64_t r = t % fz;
-  c[0] = (int)(r % nx); c[1] = (int)(r / nx);
 }
 
 template <class LAT, class ST>
@@ -588,33 +600,20 @@ template <class LAT, class ST>
 __global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
+  const int64_t npad = a.slabs.face_off[a.slabs.nface];
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= a.npad) return;
+  if (t >= npad) return;
   const GridGeom& g = a.g;
   int iv[3];
-  pad_cell(g, t, iv);
-  R* gb = static_cast<R*>(a.gbuf);
-  bool nonper_out = false, all_user1 = true, any_out = false;
+  slab_cell(a, t, iv);
+  const R* gb = static_cast<const R*>(a.gbuf);
+  R* out = static_cast<R*>(a.gbuf) + (int64_t)LAT::Q * g.qs;
+  bool all_user1 = true;
   for (int d = 0; d < LAT::D; ++d) {
     const bool lo = iv[d] < 0, hi = iv[d] > g.n[d] - 1;
     if (!lo && !hi) continue;
-    any_out = true;
-    if (!a.bc.periodic[d]) nonper_out = true;
     const int type = lo ? a.bc.lo[d] : a.bc.hi[d];
     if (type != BC_USER_1) all_user1 = false;
-  }
-  const int64_t dst = g.idx(iv[0], iv[1], iv[2]);
-  if (!any_out) return;
-  if (!nonper_out) {  // FillBoundary after the collision
-    int s[3] = {iv[0], iv[1], iv[2]};
-    for (int d = 0; d < LAT::D; ++d) {
-      if (s[d] < 0) s[d] += g.n[d];
-      else if (s[d] > g.n[d] - 1) s[d] -= g.n[d];
-    }
-    const int64_t src = g.idx(s[0], s[1], s[2]);
-#pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) gb[(int64_t)q * g.qs + dst] = gb[(int64_t)q * g.qs + src];
-    return;
   }
   T val[LAT::Q];
   fillpatch_value<LAT, ST>(a, iv, val);
@@ -623,14 +622,12 @@ __global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
     for (int q = 0; q < LAT::Q; ++q) {
       const int it = iv[0] + LAT::cx[q], jt = iv[1] + LAT::cy[q], kt = iv[2] + LAT::cz[q];
       if (it < 0 || it >= g.n[0] || jt < 0 || jt >= g.n[1] || kt < 0 || kt >= g.n[2]) continue;
-      constexpr int dummy = 0;
-      (void)dummy;
       const int qr = opposite<LAT>(q);
       val[q] = ST::load(gb, (int64_t)qr * g.qs + g.idx(it, jt, kt), T(LAT::w[qr]));
     }
   }
 #pragma unroll
-  for (int q = 0; q < LAT::Q; ++q) ST::store(gb, (int64_t)q * g.qs + dst, val[q], T(LAT::w[q]));
+  for (int q = 0; q < LAT::Q; ++q) ST::store(out, (int64_t)q * npad + t, val[q], T(LAT::w[q]));
 }
 
 // Fixed-order reduction of block partials (Evolve's Sum(Rho)).

@@ -2,12 +2,13 @@
 // Evolve / IBMarkerDF roles of the reference) and the C ABI of include/ilbm.h.
 //
 // Per coarse step, all on one stream, no host synchronisation:
-//   [IB]  k_ib_*          F^{n+1} on the band from macro^n   (IBMarkerDF::update_forcing)
-//   [IB]  k_guo_fixup     Guo term of step n+1 on band cells  (collide_forced.H:22-50)
-//   [pad] k_fill_ghosts   ghost layer (FillPatch / FillBoundary / bounce-back)
-//         k_step          pull + macros + BGK                 (AdvanceLBMAtLevel.cpp:223-386)
-//         k_sum_partials  Sum(Rho)                            (AmrCoreLBM.cpp:378)
-//   [IB]  k_force_record  force.dat row                       (AmrCoreLBM.cpp:456-701)
+//   [IB]   k_ib_*             F^{n+1} on the band from macro^n (IBMarkerDF::update_forcing)
+//   [IB]   k_ib_export_fixup  export F, Guo term of step n+1 on band cells
+//                             (collide_forced.H:22-50), force.dat row
+//                             (AmrCoreLBM.cpp:456-701)
+//   [wall] k_fill_ghosts      ghost slabs (FillPatch / bounce-back)
+//          k_step             pull + macros + BGK (AdvanceLBMAtLevel.cpp:223-386)
+//                             and Sum(Rho) (AmrCoreLBM.cpp:378)
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -280,7 +281,7 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
 // ---------------------------------------------------------------------------
 using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&);
 using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
-using FixFn = void (*)(dim3, dim3, hipStream_t, const FixupArgs&);
+using FixFn = void (*)(dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 
 template <class LAT, class ST, bool W>
@@ -292,8 +293,9 @@ static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
   hipLaunchKernelGGL((k_init<LAT, ST, W>), g, b, 0, s, a);
 }
 template <class LAT, class ST>
-static void launch_fix(dim3 g, dim3 b, hipStream_t s, const FixupArgs& a) {
-  hipLaunchKernelGGL((k_guo_fixup<LAT, ST>), g, b, 0, s, a);
+static void launch_fix(dim3 g, dim3 b, hipStream_t s, const IbArgs& a, const GridGeom& gg, void* buf,
+                       double tau, const RecArgs& r) {
+  hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST>), g, b, 0, s, a, gg, buf, tau, r);
 }
 template <class LAT, class ST>
 static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
@@ -340,6 +342,8 @@ struct Level {
   double tau = 0, dt = 0, t_new = 0, t_old = -1.0e100;
   int istep = 0;
   hipStream_t stream = nullptr;
+  Slabs slabs{};
+  double* fpartial = nullptr;    // [3][export blocks]
   // exports
   double* mout = nullptr;
   double* fout = nullptr;
@@ -365,6 +369,8 @@ struct Level {
   bool ivc_ready = false;
   std::vector<double> ivc_xc;  // positions the operator was built for
   int* rowptr_d = nullptr;
+  int* ne_d = nullptr;
+  int n_ne = 0;
   int* entm_d = nullptr;
   int* entc_d = nullptr;
   double* entw_d = nullptr;
@@ -388,6 +394,7 @@ struct Level {
     dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
+    dev_free(ne_d); dev_free(fpartial);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
@@ -403,6 +410,7 @@ struct Level {
         if (std::find(ok, ok + 5, b) == ok + 5) throw StatusError{ILBM_ERR_UNSUPPORTED, "BC type"};
         if (c.periodic[d] && b != 0) throw StatusError{ILBM_ERR_INVALID, "periodic dim needs int_dir"};
         if (!c.periodic[d] && b == 0) throw StatusError{ILBM_ERR_INVALID, "int_dir on non-periodic dim"};
+        if (!c.periodic[d] && c.n[d] < 2) throw StatusError{ILBM_ERR_UNSUPPORTED, "non-periodic extent < 2"};
       }
     }
     if (lattice_matches<D2Q9>(c)) lat = 0;
@@ -420,27 +428,44 @@ struct Level {
       if (!c.periodic[d]) { wrap = false; nonper[d] = 1; }
     g.dim = c.dim;
     for (int d = 0; d < 3; ++d) g.n[d] = c.n[d];
-    g.pad = wrap ? 0 : 1;
-    if (wrap) {
-      g.js = c.n[0];
-      g.ks = (int64_t)c.n[0] * c.n[1];
-      g.off0 = 0;
-      g.qs = ((int64_t)c.n[0] * c.n[1] * c.n[2] + 127) / 128 * 128;
-    } else {
-      // rows start on 64-element boundaries; the x ghosts sit in the row
-      // slack (i = -1 at the end of the previous slot, i = n at its start)
-      g.js = (c.n[0] >= 64) ? (c.n[0] + 2 + 63) / 64 * 64 : c.n[0] + 2;
-      g.ks = g.js * (int64_t)(c.n[1] + 2);
-      g.off0 = g.js + ((c.dim == 3) ? g.ks : 0) + 64;
-      const int64_t top = g.off0 + (int64_t)((c.dim == 3) ? c.n[2] : 0) * g.ks + (int64_t)c.n[1] * g.js + c.n[0] + 1;
-      g.qs = (top + 127) / 128 * 128;
+    g.pad = 0;
+    g.js = c.n[0];
+    g.ks = (int64_t)c.n[0] * c.n[1];
+    g.off0 = 0;
+    g.qs = ((int64_t)c.n[0] * c.n[1] * c.n[2] + 127) / 128 * 128;
+    // ghost slabs of the non-periodic faces
+    for (int d = 0; d < 3; ++d) slabs.slab_of[d][0] = slabs.slab_of[d][1] = -1;
+    int nf = 0;
+    int64_t off = 0;
+    for (int d = 0; d < c.dim; ++d) {
+      if (!nonper[d]) continue;
+      for (int sd = 0; sd < 2; ++sd) {
+        slabs.slab_of[d][sd] = nf;
+        slabs.face_d[nf] = d;
+        slabs.face_side[nf] = sd;
+        int64_t cnt = 1;
+        for (int e = 0; e < 3; ++e) {
+          if (e == d || e >= c.dim) { slabs.face_lo[nf][e] = 0; slabs.face_n[nf][e] = 1; continue; }
+          if (nonper[e] && e < d) { slabs.face_lo[nf][e] = 0; slabs.face_n[nf][e] = c.n[e]; }
+          else { slabs.face_lo[nf][e] = -1; slabs.face_n[nf][e] = c.n[e] + 2; }
+          cnt *= slabs.face_n[nf][e];
+        }
+        slabs.face_off[nf] = off;
+        off += cnt;
+        ++nf;
+      }
     }
+    slabs.nface = nf;
+    for (int f = nf; f < 7; ++f) slabs.face_off[f] = off;
     ncells = (int64_t)c.n[0] * c.n[1] * c.n[2];
     popbytes = (st == 0) ? 8 : (st == 1) ? 4 : 2;
     const int Q = c.q;
+    const int64_t npad = slabs.face_off[slabs.nface];
     for (int b = 0; b < 2; ++b) {
-      gbuf[b] = dev_alloc_bytes(popbytes * (size_t)g.qs * Q);
-      HIPCHK(hipMemsetAsync(gbuf[b], 0, popbytes * (size_t)g.qs * Q, stream));
+      // Q population planes followed by the Q ghost slabs
+      const size_t bytes = popbytes * ((size_t)g.qs * Q + (size_t)npad * Q);
+      gbuf[b] = dev_alloc_bytes(bytes);
+      HIPCHK(hipMemsetAsync(gbuf[b], 0, bytes, stream));
     }
     for (int f = 0; f < 6; ++f) faces.p[f] = nullptr;
     for (int d = 0; d < c.dim; ++d) {
@@ -455,6 +480,7 @@ struct Level {
     grid = dim3((c.n[0] + bx - 1) / bx, (c.n[1] + by - 1) / by, c.n[2]);
     nblocks = (int64_t)grid.x * grid.y * grid.z;
     partial = dev_alloc<double>(nblocks);
+
     auto pk = [&](auto latt) {
       using L = decltype(latt);
       if (st == 0) pick<L, StF64>(wrap, stepfn, initfn, fixfn, ghostfn);
@@ -514,6 +540,7 @@ struct Level {
     EFI = dev_alloc<double>(3 * band_cells);
     EFT = dev_alloc<double>(3 * band_cells);
     HIPCHK(hipMemsetAsync(Fband, 0, sizeof(double) * 3 * band_cells, stream));
+    HIPCHK(hipMemsetAsync(EFI, 0, sizeof(double) * 3 * band_cells, stream));
     HIPCHK(hipMemsetAsync(Mband, 0, sizeof(double) * 4 * band_cells, stream));
     xm_d = dev_alloc<double>(3 * nm);
     ut_d = dev_alloc<double>(3 * nm);
@@ -532,6 +559,7 @@ struct Level {
     HIPCHK(hipMemsetAsync(U_d, 0, sizeof(double) * 3 * nm, stream));
     HIPCHK(hipMemsetAsync(Mm_d, 0, sizeof(double) * 3 * nm, stream));
     rowptr_d = dev_alloc<int>(band_cells + 1);
+    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + 1));
     upload_markers();
     ncol = (c.dim == 3) ? 33 : 23;
   }
@@ -597,8 +625,15 @@ struct Level {
       em[e] = ents[e].m;
       ec[e] = ents[e].code;
     }
+    std::vector<int> ne;
+    for (int64_t b = 0; b < band_cells; ++b)
+      if (rowptr[b + 1] > 0) ne.push_back((int)b);
     for (int64_t b = 0; b < band_cells; ++b) rowptr[b + 1] += rowptr[b];
     nent = (int64_t)ents.size();
+    dev_free(ne_d);
+    n_ne = (int)ne.size();
+    ne_d = dev_alloc<int>(std::max(n_ne, 1));
+    if (n_ne) HIPCHK(hipMemcpyAsync(ne_d, ne.data(), sizeof(int) * n_ne, hipMemcpyHostToDevice, stream));
     if (nent > ent_cap) {
       dev_free(entm_d); dev_free(entc_d); dev_free(entw_d);
       ent_cap = std::max<int64_t>(nent, 1) * 2;
@@ -682,6 +717,10 @@ struct Level {
     a.base = base_d; a.wts = wts_d; a.scale = scale_d;
     a.Mband = Mband; a.EV = EV; a.EFI = EFI; a.EFT = EFT; a.Fband = Fband;
     a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
+    a.ne_cells = ne_d; a.n_ne = n_ne;
+    a.ev_from_macro = (c.coupling_method == 1 || std::max(c.loop_ns, 1) == 1) ? 1 : 0;
+    a.fuse_explicit = (c.coupling_method != 1) ? 1 : 0;
+    a.first_iter = 1;
     a.Ainv = Ainv_d; a.rhs = rhs_d; a.sums = sums_d;
     return a;
   }
@@ -696,37 +735,54 @@ struct Level {
     return b;
   }
 
-  // IBMarkerDF::update_forcing (IBMarkerDF.cpp:1522-1917) on the band
-  void ib_update_forcing(double time) {
+  // IBMarkerDF::update_forcing (IBMarkerDF.cpp:1522-1917) on the band, ending
+  // with the export of F fused with the Guo term of the coming collision and
+  // with the ComputeIBForce record of the step (row `row` of the history).
+  void ib_update_forcing(double time, int row) {
     if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
     }
-    const IbArgs a = ib_args();
+    IbArgs a = ib_args();
     const int tb = 256;
     const unsigned gb = (unsigned)((band_cells + tb - 1) / tb);
-    const unsigned gm = (unsigned)((nm + tb - 1) / tb);
-    hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
-    const bool use_ivc = (c.coupling_method == 1);
-    const int nloop = use_ivc ? 1 : std::max(c.loop_ns, 1);
+    const bool use_ivc = (c.coupling_method == 1) && ivc_ready;
+    const int nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
+    if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
+    if (c.coupling_method == 1 && !ivc_ready) a.fuse_explicit = 1;  // LU failed: explicit fallback
     for (int it = 0; it < nloop; ++it) {
+      a.first_iter = (it == 0) ? 1 : 0;
       hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
                          stream, a);
-      if (use_ivc && ivc_ready) {
-        hipLaunchKernelGGL(k_ib_ivc_rhs, dim3(gm), dim3(tb), 0, stream, a);
+      if (use_ivc) {
+        hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
         hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
-      } else {
-        hipLaunchKernelGGL(k_ib_force_explicit, dim3(gm), dim3(tb), 0, stream, a);
       }
-      hipLaunchKernelGGL(k_ib_spread, dim3(gb), dim3(tb), 0, stream, a);
+      if (n_ne)
+        hipLaunchKernelGGL(k_ib_spread, dim3((n_ne + kSpreadWaves - 1) / kSpreadWaves), dim3(64 * kSpreadWaves), 0,
+                           stream, a);
       hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
-      hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
+      if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
     }
-    hipLaunchKernelGGL(k_ib_export, dim3(gb), dim3(tb), 0, stream, a);
+    RecArgs r{};
+    r.partial = fpartial; r.sums = sums_d; r.state = frstate_d;
+    r.hist = force_dev; r.row = row; r.ncol = ncol; r.method = c.force_eval_method;
+    r.time = time + dt; r.step = (double)(istep + 1);
+    r.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
+    r.U0 = c.U0; r.R = c.R;
+    fixfn(dim3(gb), dim3(tb), stream, a, g, gbuf[cur], tau, r);
+    hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gb);
     HIPCHK(hipGetLastError());
   }
 
   // ---------------- lattice ----------------
+  GhostArgs ghost_args() {
+    GhostArgs ga{};
+    ga.g = g; ga.gbuf = gbuf[cur]; ga.faces = faces; ga.bc = bc_spec();
+    ga.slabs = slabs;
+    return ga;
+  }
+
   StepArgs step_args(bool want_state) {
     StepArgs a{};
     a.g = g;
@@ -749,6 +805,7 @@ struct Level {
     a.fout = want_state ? fout : nullptr;
     a.faces = faces;
     a.partial = partial;
+    a.slabs = slabs;
     return a;
   }
 
@@ -758,6 +815,7 @@ struct Level {
   }
 
   void init(const double* macro_host) {
+    grow_histories(1, 1);
     double* m0 = dev_alloc<double>(4 * ncells);
     HIPCHK(hipMemcpyAsync(m0, macro_host, sizeof(double) * 4 * ncells, hipMemcpyHostToDevice, stream));
     const bool small = (int64_t)c.q * ncells <= (int64_t)1 << 27;
@@ -809,20 +867,11 @@ struct Level {
       const double time = t_new;  // timeStepWithSubcycling, AmrCoreLBM.cpp:1730-1733
       t_old = t_new;
       t_new += dt;
-      if (ib) {
-        ib_update_forcing(time);
-        FixupArgs fa{};
-        fa.g = g; fa.gbuf = gbuf[cur]; fa.tau = tau;
-        for (int d = 0; d < 3; ++d) { fa.band_lo[d] = band_lo[d]; fa.band_n[d] = band_n[d]; }
-        fa.band_cells = band_cells; fa.Fband = Fband; fa.Mband = Mband;
-        fixfn(dim3((unsigned)((band_cells + tb - 1) / tb)), dim3(tb), stream, fa);
-      }
+      if (ib) ib_update_forcing(time, s);
       if (!wrap) {
-        GhostArgs ga{};
-        ga.g = g; ga.gbuf = gbuf[cur]; ga.faces = faces; ga.bc = bc_spec();
-        const int64_t P0 = c.n[0] + 2, P1 = c.n[1] + 2, P2 = (c.dim == 3) ? c.n[2] + 2 : 1;
-        ga.npad = P0 * P1 * P2 - ncells;
-        ghostfn(dim3((unsigned)((ga.npad + tb - 1) / tb)), dim3(tb), stream, ga);
+        GhostArgs ga = ghost_args();
+        const int64_t npad = slabs.face_off[slabs.nface];
+        if (npad) ghostfn(dim3((unsigned)((npad + tb - 1) / tb)), dim3(tb), stream, ga);
       }
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
@@ -833,15 +882,6 @@ struct Level {
                          mass_dev + s);
       cur ^= 1;
       ++istep;
-      if (ib) {
-        ForceRecArgs fr{};
-        fr.Fband = Fband; fr.band_cells = band_cells; fr.sums = sums_d; fr.state = frstate_d;
-        fr.hist = force_dev; fr.row = s; fr.ncol = ncol; fr.dim = c.dim; fr.method = c.force_eval_method;
-        fr.time = time + dt; fr.step = (double)istep;
-        fr.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
-        fr.dt = dt; fr.U0 = c.U0; fr.R = c.R;
-        hipLaunchKernelGGL(k_force_record, dim3(1), dim3(1024), 0, stream, fr);
-      }
       HIPCHK(hipGetLastError());
     }
     std::vector<double> mh(nsteps);
@@ -900,6 +940,8 @@ extern "C" {
 
 int ilbm_abi_version(void) { return ILBM_ABI_VERSION; }
 
+int ilbm_case_desc_size(void) { return (int)sizeof(ilbm_case_desc); }
+
 const char* ilbm_status_string(int status) {
   switch (status) {
     case ILBM_OK: return "ok";
@@ -924,6 +966,7 @@ int ilbm_level_create(const ilbm_case_desc* desc, const double* init_macro, ilbm
     L.c = *desc;
     L.validate();
     HIPCHK(hipStreamCreateWithFlags(&L.stream, hipStreamNonBlocking));
+
     L.setup_layout();
     L.setup_ib();
     L.init(init_macro);
@@ -939,6 +982,7 @@ int ilbm_level_create(const ilbm_case_desc* desc, const double* init_macro, ilbm
 int ilbm_level_destroy(ilbm_level* h) {
   if (!h) return ILBM_ERR_INVALID;
   if (h->L.stream) (void)hipStreamSynchronize(h->L.stream);
+
   delete h;
   return ILBM_OK;
 }
@@ -950,7 +994,9 @@ int ilbm_level_advance(ilbm_level* h, int nsteps, int want_state) {
 
 int ilbm_level_synchronize(ilbm_level* h) {
   if (!h) return ILBM_ERR_INVALID;
-  return guarded([&] { HIPCHK(hipStreamSynchronize(h->L.stream)); });
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(h->L.stream));
+  });
 }
 
 int ilbm_level_get_f(ilbm_level* h, double* out) {

@@ -138,6 +138,8 @@ const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */
 const char* ilbm_last_error(void);
 int ilbm_abi_version(void);
+/* sizeof(ilbm_case_desc) as compiled, for binding layout checks. */
+int ilbm_case_desc_size(void);
 
 #ifdef __cplusplus
 }

@@ -136,7 +136,8 @@ def _ib_checks(lev, orc, force_tol=1e-12):
     assert_bitexact(uvw, ouvw, "marker U")
     assert_bitexact(fxyz, ofxyz, "marker F")
     ms, oms = lev.last_marker_force_sum(), orc.last_marker_force_sum()
-    assert_bitexact(ms, oms, "marker force sum")
+    # reduction over markers (fixed tree on the GPU, serial in the oracle)
+    assert np.all(np.abs(ms - oms) <= 1e-12 * np.abs(oms) + 1e-13 * np.abs(oms).max()), (ms, oms)
     fh = lev.force_history()[-1]
     rec = orc.force_record()
     assert fh.shape == rec.shape
