@@ -1,0 +1,655 @@
+// Batched, continuously admitted IB-LBM cases on one device (compiled as
+// part of ilbm_level.hip).
+//
+// Every admitted case is a resident Level (its own populations, ghost slabs,
+// IB band, markers and histories).  A step of the whole batch is the single
+// Level step with each kernel launched ONCE for all cases of a lattice /
+// storage class: blocks look their case up in a BlockRef map and read the
+// case's arguments from a device table.  The step sequence of every class is
+// captured in a hipGraph covering `graph_steps` steps, replayed until the
+// next admission or retirement changes the batch.
+//
+// Cases advance in lock step but each stops at its own step count: a call
+// advances the batch by at most the smallest remaining count, so no case
+// runs past its request.  Results do not depend on the mix: every kernel
+// body is the one the single Level runs, with the same per-case block
+// decomposition and fixed-order reductions.
+#pragma once
+
+#include <map>
+#include <memory>
+
+namespace ilbm {
+
+using StepBFn = void (*)(dim3, hipStream_t, const StepArgs*, const BlockRef*);
+using GhostBFn = void (*)(dim3, hipStream_t, const GhostArgs*, const BlockRef*);
+using FixBFn = void (*)(dim3, hipStream_t, const IbCase*, const BlockRef*);
+
+template <class LAT, class ST, bool W>
+static void launch_step_b(dim3 g, hipStream_t s, const StepArgs* a, const BlockRef* m) {
+  hipLaunchKernelGGL((k_step_b<LAT, ST, W>), g, dim3(kBlock), 0, s, a, m);
+}
+template <class LAT, class ST>
+static void launch_ghost_b(dim3 g, hipStream_t s, const GhostArgs* a, const BlockRef* m) {
+  hipLaunchKernelGGL((k_fill_ghosts_b<LAT, ST>), g, dim3(kBlock), 0, s, a, m);
+}
+template <class LAT, class ST>
+static void launch_fix_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef* m) {
+  hipLaunchKernelGGL((k_ib_export_fixup_b<LAT, ST>), g, dim3(256), 0, s, a, m);
+}
+
+template <class LAT, class ST>
+static void pick_b(bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff) {
+  sf = wrap ? launch_step_b<LAT, ST, true> : launch_step_b<LAT, ST, false>;
+  gf = launch_ghost_b<LAT, ST>;
+  ff = launch_fix_b<LAT, ST>;
+}
+
+static void pick_class(int lat, int st, bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff) {
+  auto pk = [&](auto latt) {
+    using L = decltype(latt);
+    if (st == 0) pick_b<L, StF64>(wrap, sf, gf, ff);
+    else if (st == 1) pick_b<L, StF32>(wrap, sf, gf, ff);
+    else pick_b<L, StF16S>(wrap, sf, gf, ff);
+  };
+  if (lat == 0) pk(D2Q9{});
+  else if (lat == 1) pk(D3Q19{});
+  else pk(D3Q27{});
+}
+
+struct BatchCase {
+  int id = -1;
+  std::unique_ptr<Level> L;
+  int target = 0, done = 0;
+  int64_t bytes = 0;
+  int tile_bx = 64;
+  int base_cur = 0;         // L->cur when the batch tables were built
+};
+
+// Device tables of one (lattice, storage, layout) class.
+struct BatchGroup {
+  int lat = 0, st = 0;
+  bool wrap = true;
+  StepBFn step = nullptr;
+  GhostBFn ghost = nullptr;
+  FixBFn fix = nullptr;
+  std::vector<int> members;  // indices into Batch::cases
+  StepArgs* dstep[2] = {nullptr, nullptr};
+  GhostArgs* dghost[2] = {nullptr, nullptr};
+  IbCase* dib[2] = {nullptr, nullptr};
+  CaseTail* dtail = nullptr;
+  BlockRef* dmap = nullptr;
+  // block maps inside dmap: offset and count of each kernel's grid
+  enum { M_STEP, M_GHOST, M_BAND, M_INTERP, M_SPREAD, M_RHS, M_MATVEC, M_IBCASE, M_TAIL, M_N };
+  int64_t moff[M_N] = {}, mcnt[M_N] = {};
+  int max_loop = 0;
+  bool any_ivc = false, any_iter = false;
+
+  void release() {
+    for (int p = 0; p < 2; ++p) { dev_free(dstep[p]); dev_free(dghost[p]); dev_free(dib[p]); }
+    dev_free(dtail); dev_free(dmap);
+    for (int p = 0; p < 2; ++p) { dstep[p] = nullptr; dghost[p] = nullptr; dib[p] = nullptr; }
+    dtail = nullptr; dmap = nullptr;
+  }
+};
+
+// Device bytes a case holds while resident (what admission charges).
+static int64_t case_footprint(const ilbm_case_desc& c0, int nsteps) {
+  ilbm_case_desc c = c0;
+  if (c.dim == 2) c.n[2] = 1;
+  const int Q = c.q;
+  const int64_t cells = (int64_t)c.n[0] * c.n[1] * c.n[2];
+  const int64_t qs = (cells + 127) / 128 * 128;
+  const int pb = (c.storage == 0) ? 8 : (c.storage == 1) ? 4 : 2;
+  int64_t slab = 0, faces = 0;
+  for (int d = 0; d < c.dim; ++d) {
+    if (c.periodic[d]) continue;
+    int64_t f = 1;
+    for (int e = 0; e < c.dim; ++e)
+      if (e != d) f *= (int64_t)c.n[e] + 2;
+    slab += 2 * f;
+    faces += 2 * f;
+  }
+  int64_t b = 2 * (int64_t)pb * Q * (qs + slab) + (int64_t)pb * Q * faces;
+  b += 8 * (cells / 64 + 64);                          // block partials
+  if (c.use_ib) {
+    // band: stencil box of the body, markers and their lists
+    const double ext = 2.0 * c.R / c.dx[0] + 7.0;
+    int64_t band = 1, nm;
+    for (int d = 0; d < c.dim; ++d) band *= std::min<int64_t>((int64_t)std::ceil(ext), c.n[d]);
+    if (c.dim == 2) nm = (int64_t)std::ceil(2.0 * kPi * c.R / c.dx[0]) + 16;
+    else nm = (int64_t)std::ceil(4.0 * kPi * c.R * c.R / (c.dx[0] * c.dx[0])) + 32;
+    if (c.n_marker > 0) nm = std::max<int64_t>(nm, c.n_marker);
+    b += band * 8 * 16 + band * 4 + nm * 8 * 40;
+    b += nm * (c.dim == 3 ? 125 : 25) * 16;              // spreading entries
+    if (c.coupling_method == 1) b += nm * nm * 8;
+    b += (int64_t)nsteps * 8 * ((c.dim == 3) ? 33 : 23);  // force history
+  }
+  b += (int64_t)nsteps * 8;                               // mass history
+  return b;
+}
+
+// Autotuned tile width per (lattice, storage, layout, x-extent bucket),
+// shared by every batch of the process so a case gets the same tiling
+// whichever batch it joins.
+static std::map<int64_t, int>& tile_table() {
+  static std::map<int64_t, int> t;
+  return t;
+}
+
+struct Batch {
+  hipStream_t stream = nullptr;
+  int64_t budget = 0, used = 0;
+  int max_cases = 0;
+  int graph_steps = 8;
+  int next_id = 1;
+  std::vector<BatchCase> cases;
+  std::vector<BatchGroup> groups;
+  bool dirty = true;
+  int phase = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  bool graph_failed = false;
+  // statistics
+  int64_t graph_launches = 0, captures = 0, eager_steps = 0;
+  double lups = 0.0;
+
+  ~Batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    for (auto& g : groups) g.release();
+    cases.clear();
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
+
+  BatchCase* find(int id) {
+    for (auto& c : cases)
+      if (c.id == id) return &c;
+    return nullptr;
+  }
+
+  // ---------------- autotuner ----------------
+  static int bucket(int n0) {
+    int b = 0;
+    while ((64 << b) < n0 && b < 6) ++b;
+    return b;
+  }
+
+  // Tile width of the stream-collide launch for a case of this class and
+  // x extent: the candidate widths 64 / 128 / 256 are timed on a scratch
+  // level of the case's own shape (no IB) and the fastest is kept for the
+  // process, so every later case of the bucket uses the same tiling.
+  int tile_for(const Level& ref) {
+    const int64_t key = ((int64_t)ref.lat * 4 + ref.st) * 2 * 16 + (ref.wrap ? 16 : 0) + bucket(ref.c.n[0]);
+    auto& tiles = tile_table();
+    auto it = tiles.find(key);
+    if (it != tiles.end()) return it->second;
+    std::vector<int> cand;
+    for (int bx = 64; bx <= 256; bx *= 2) {
+      cand.push_back(bx);
+      if (bx >= ref.c.n[0]) break;
+    }
+    int best = cand[0];
+    if (cand.size() > 1) {
+      Level T;
+      T.c = ref.c;
+      T.c.use_ib = 0;
+      T.c.use_prescribed_force = 0;
+      T.stream = stream;
+      T.own_stream = false;
+      T.validate();
+      T.setup_layout();
+      std::vector<double> m0(4 * (size_t)T.ncells, 0.0);
+      std::fill(m0.begin(), m0.begin() + T.ncells, 1.0);
+      T.init(m0.data());
+      int64_t maxblk = 0;
+      for (int bx : cand) maxblk = std::max(maxblk, T.tile_blocks(bx));
+      T.ensure_partial(maxblk);
+      StepBFn sf; GhostBFn gf; FixBFn ff;
+      pick_class(T.lat, T.st, T.wrap, sf, gf, ff);
+      StepArgs* da = dev_alloc<StepArgs>(1);
+      BlockRef* dm = dev_alloc<BlockRef>(maxblk);
+      hipEvent_t e0, e1;
+      HIPCHK(hipEventCreate(&e0));
+      HIPCHK(hipEventCreate(&e1));
+      double best_ms = 1e30;
+      for (int bx : cand) {
+        StepArgs a = T.step_args(false);
+        a.tile_bx = bx;
+        const int64_t nb = T.tile_blocks(bx);
+        std::vector<BlockRef> m(nb);
+        for (int64_t b = 0; b < nb; ++b) m[b] = BlockRef{0, (int)b};
+        HIPCHK(hipMemcpyAsync(da, &a, sizeof(a), hipMemcpyHostToDevice, stream));
+        HIPCHK(hipMemcpyAsync(dm, m.data(), sizeof(BlockRef) * nb, hipMemcpyHostToDevice, stream));
+        for (int r = 0; r < 3; ++r) sf(dim3((unsigned)nb), stream, da, dm);
+        HIPCHK(hipEventRecord(e0, stream));
+        const int reps = 10;
+        for (int r = 0; r < reps; ++r) sf(dim3((unsigned)nb), stream, da, dm);
+        HIPCHK(hipEventRecord(e1, stream));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        if (ms < best_ms) { best_ms = ms; best = bx; }
+      }
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      dev_free(da);
+      dev_free(dm);
+    }
+    tiles[key] = best;
+    return best;
+  }
+
+  // ---------------- admission ----------------
+  int add(const ilbm_case_desc& desc, const double* init, int nsteps) {
+    if (nsteps <= 0) throw StatusError{ILBM_ERR_INVALID, "nsteps must be positive"};
+    if (desc.use_ib && desc.geometry_type == 2)
+      throw StatusError{ILBM_ERR_UNSUPPORTED, "moving bodies are not batched"};
+    const int64_t bytes = case_footprint(desc, nsteps);
+    if ((int)cases.size() >= max_cases || used + bytes > budget)
+      throw StatusError{ILBM_ERR_FULL, "batch full (case slots or HBM budget)"};
+    BatchCase bc;
+    bc.L.reset(new Level());
+    Level& L = *bc.L;
+    L.c = desc;
+    L.stream = stream;
+    L.own_stream = false;
+    L.validate();
+    L.setup_layout();
+    L.setup_ib();
+    L.init(init);
+    bc.tile_bx = tile_for(L);
+    L.ensure_partial(L.tile_blocks(bc.tile_bx));
+    L.grow_histories(nsteps, nsteps);
+    const double clk[4] = {L.t_new, (double)L.istep, (double)L.istep, 0.0};
+    HIPCHK(hipMemcpyAsync(L.clock_d, clk, sizeof(clk), hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    bc.id = next_id++;
+    bc.target = nsteps;
+    bc.bytes = bytes;
+    used += bytes;
+    cases.push_back(std::move(bc));
+    dirty = true;
+    return cases.back().id;
+  }
+
+  void remove(int id) {
+    for (size_t i = 0; i < cases.size(); ++i) {
+      if (cases[i].id != id) continue;
+      HIPCHK(hipStreamSynchronize(stream));
+      used -= cases[i].bytes;
+      cases.erase(cases.begin() + i);
+      dirty = true;
+      return;
+    }
+    throw StatusError{ILBM_ERR_INVALID, "unknown case id"};
+  }
+
+  // ---------------- tables ----------------
+  void rebuild() {
+    HIPCHK(hipStreamSynchronize(stream));
+    drop_graph();
+    graph_failed = false;
+    for (auto& g : groups) g.release();
+    groups.clear();
+    phase = 0;
+    for (size_t i = 0; i < cases.size(); ++i) {
+      BatchCase& bc = cases[i];
+      if (bc.done >= bc.target) continue;  // finished cases stay resident until removed
+      Level& L = *bc.L;
+      bc.base_cur = L.cur;
+      int gi = -1;
+      for (size_t k = 0; k < groups.size(); ++k)
+        if (groups[k].lat == L.lat && groups[k].st == L.st && groups[k].wrap == L.wrap) gi = (int)k;
+      if (gi < 0) {
+        BatchGroup g;
+        g.lat = L.lat; g.st = L.st; g.wrap = L.wrap;
+        pick_class(g.lat, g.st, g.wrap, g.step, g.ghost, g.fix);
+        groups.push_back(g);
+        gi = (int)groups.size() - 1;
+      }
+      groups[gi].members.push_back((int)i);
+    }
+    for (auto& g : groups) build_group(g);
+    dirty = false;
+  }
+
+  void build_group(BatchGroup& G) {
+    const size_t n = G.members.size();
+    std::vector<StepArgs> sa[2];
+    std::vector<GhostArgs> ga[2];
+    std::vector<IbCase> ia[2];
+    std::vector<CaseTail> tails(n);
+    std::vector<BlockRef> maps[BatchGroup::M_N];
+    for (int p = 0; p < 2; ++p) { sa[p].resize(n); ga[p].resize(n); ia[p].resize(n); }
+    G.max_loop = 0;
+    for (size_t ci = 0; ci < n; ++ci) {
+      BatchCase& bc = cases[G.members[ci]];
+      Level& L = *bc.L;
+      const int c = (int)ci;
+      for (int p = 0; p < 2; ++p) {
+        L.cur = bc.base_cur ^ p;
+        sa[p][ci] = L.step_args(false);
+        sa[p][ci].tile_bx = bc.tile_bx;
+        ga[p][ci] = L.ghost_args();
+        IbCase ic{};
+        if (L.ib) {
+          ic.ib = L.ib_args();
+          ic.g = L.g;
+          ic.gbuf = L.gbuf[L.cur];
+          ic.tau = L.tau;
+          RecArgs& r = ic.rec;
+          r.partial = L.fpartial; r.sums = L.sums_d; r.state = L.frstate_d;
+          r.hist = L.force_dev; r.row = 0; r.ncol = L.ncol; r.method = L.c.force_eval_method;
+          r.dV = (L.c.dim == 3) ? L.c.dx[0] * L.c.dx[1] * L.c.dx[2] : L.c.dx[0] * L.c.dx[1];
+          r.U0 = L.c.U0; r.R = L.c.R;
+          r.clock = L.clock_d;
+          ic.nblk_fix = (int)((L.band_cells + 255) / 256);
+        }
+        ia[p][ci] = ic;
+      }
+      L.cur = bc.base_cur;
+      const int64_t nb = L.tile_blocks(bc.tile_bx);
+      tails[ci] = CaseTail{L.partial, nb, L.mass_dev, L.clock_d, L.dt};
+      for (int64_t b = 0; b < nb; ++b) maps[BatchGroup::M_STEP].push_back({c, (int)b});
+      if (!L.wrap) {
+        const int64_t npad = L.slabs.face_off[L.slabs.nface];
+        for (int64_t b = 0; b < (npad + kBlock - 1) / kBlock; ++b) maps[BatchGroup::M_GHOST].push_back({c, (int)b});
+      }
+      maps[BatchGroup::M_TAIL].push_back({c, 0});
+      if (L.ib) {
+        const IbArgs& a = ia[0][ci].ib;
+        for (int64_t b = 0; b < (L.band_cells + 255) / 256; ++b) maps[BatchGroup::M_BAND].push_back({c, (int)b});
+        for (int b = 0; b < (L.nm + kInterpWaves - 1) / kInterpWaves; ++b)
+          maps[BatchGroup::M_INTERP].push_back({c, b});
+        for (int b = 0; b < (L.n_ne + kSpreadWaves - 1) / kSpreadWaves; ++b)
+          maps[BatchGroup::M_SPREAD].push_back({c, b});
+        if (a.use_ivc) {
+          for (int b = 0; b < (L.nm + 255) / 256; ++b) maps[BatchGroup::M_RHS].push_back({c, b});
+          for (int b = 0; b < (L.nm + 3) / 4; ++b) maps[BatchGroup::M_MATVEC].push_back({c, b});
+          G.any_ivc = true;
+        }
+        if (!a.ev_from_macro) G.any_iter = true;
+        maps[BatchGroup::M_IBCASE].push_back({c, 0});
+        G.max_loop = std::max(G.max_loop, a.nloop);
+      }
+    }
+    int64_t tot = 0;
+    for (int m = 0; m < BatchGroup::M_N; ++m) {
+      G.moff[m] = tot;
+      G.mcnt[m] = (int64_t)maps[m].size();
+      tot += G.mcnt[m];
+    }
+    std::vector<BlockRef> all;
+    all.reserve(tot);
+    for (int m = 0; m < BatchGroup::M_N; ++m) all.insert(all.end(), maps[m].begin(), maps[m].end());
+    for (int64_t m = 0; m < BatchGroup::M_N; ++m)
+      if (G.mcnt[m] > 0x7fffffff) throw StatusError{ILBM_ERR_UNSUPPORTED, "batched grid too large"};
+    G.dmap = dev_alloc<BlockRef>(std::max<int64_t>(tot, 1));
+    HIPCHK(hipMemcpyAsync(G.dmap, all.data(), sizeof(BlockRef) * tot, hipMemcpyHostToDevice, stream));
+    for (int p = 0; p < 2; ++p) {
+      G.dstep[p] = dev_alloc<StepArgs>(n);
+      G.dghost[p] = dev_alloc<GhostArgs>(n);
+      G.dib[p] = dev_alloc<IbCase>(n);
+      HIPCHK(hipMemcpyAsync(G.dstep[p], sa[p].data(), sizeof(StepArgs) * n, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(G.dghost[p], ga[p].data(), sizeof(GhostArgs) * n, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(G.dib[p], ia[p].data(), sizeof(IbCase) * n, hipMemcpyHostToDevice, stream));
+    }
+    G.dtail = dev_alloc<CaseTail>(n);
+    HIPCHK(hipMemcpyAsync(G.dtail, tails.data(), sizeof(CaseTail) * n, hipMemcpyHostToDevice, stream));
+    HIPCHK(hipStreamSynchronize(stream));  // host tables go out of scope
+  }
+
+  // ---------------- one batched step ----------------
+  // The single Level's order: IB forcing with the export / Guo fix-up and
+  // the force record, ghost slabs, stream-collide, Sum(Rho) and the clock.
+  void enqueue_step(int p) {
+    for (auto& G : groups) {
+      const BlockRef* m = G.dmap;
+      auto grid = [&](int k) { return dim3((unsigned)G.mcnt[k]); };
+      if (G.mcnt[BatchGroup::M_IBCASE] > 0) {
+        const IbCase* A = G.dib[p];
+        if (G.any_iter)
+          hipLaunchKernelGGL(k_ib_begin_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
+                             m + G.moff[BatchGroup::M_BAND]);
+        for (int it = 0; it < G.max_loop; ++it) {
+          hipLaunchKernelGGL(k_ib_interp_b, grid(BatchGroup::M_INTERP), dim3(64 * kInterpWaves), 0, stream, A,
+                             m + G.moff[BatchGroup::M_INTERP], it);
+          if (G.any_ivc) {
+            hipLaunchKernelGGL(k_ib_ivc_rhs_b, grid(BatchGroup::M_RHS), dim3(256), 0, stream, A,
+                               m + G.moff[BatchGroup::M_RHS], it);
+            hipLaunchKernelGGL(k_ib_ivc_matvec_b, grid(BatchGroup::M_MATVEC), dim3(256), 0, stream, A,
+                               m + G.moff[BatchGroup::M_MATVEC], it);
+          }
+          if (G.mcnt[BatchGroup::M_SPREAD] > 0)
+            hipLaunchKernelGGL(k_ib_spread_b, grid(BatchGroup::M_SPREAD), dim3(64 * kSpreadWaves), 0, stream, A,
+                               m + G.moff[BatchGroup::M_SPREAD], it);
+          hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
+                             m + G.moff[BatchGroup::M_IBCASE], it);
+          if (G.any_iter)
+            hipLaunchKernelGGL(k_ib_correct_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
+                               m + G.moff[BatchGroup::M_BAND], it);
+        }
+        G.fix(grid(BatchGroup::M_BAND), stream, A, m + G.moff[BatchGroup::M_BAND]);
+        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(256), 0, stream, A,
+                           m + G.moff[BatchGroup::M_IBCASE]);
+      }
+      if (G.mcnt[BatchGroup::M_GHOST] > 0)
+        G.ghost(grid(BatchGroup::M_GHOST), stream, G.dghost[p], m + G.moff[BatchGroup::M_GHOST]);
+      G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
+      hipLaunchKernelGGL(k_case_tail_b, grid(BatchGroup::M_TAIL), dim3(1024), 0, stream, G.dtail);
+    }
+  }
+
+  void capture() {
+    drop_graph();
+    HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < graph_steps; ++s) enqueue_step(s & 1);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(stream, &gr);
+    if (e != hipSuccess || !gr) {
+      graph_failed = true;
+      (void)hipGetLastError();
+      return;
+    }
+    graph = gr;
+    if (hipGraphInstantiate(&gexec, graph, nullptr, nullptr, 0) != hipSuccess) {
+      graph_failed = true;
+      (void)hipGetLastError();
+      drop_graph();
+      return;
+    }
+    ++captures;
+  }
+
+  int remaining_min() const {
+    int r = 1 << 30;
+    for (const auto& c : cases)
+      if (c.done < c.target) r = std::min(r, c.target - c.done);
+    return r == (1 << 30) ? 0 : r;
+  }
+
+  // Advances every unfinished case by k = min(max_steps, smallest remaining)
+  // steps; returns k.
+  int advance(int max_steps, bool use_graph) {
+    if (max_steps <= 0) return 0;
+    if (dirty) rebuild();
+    const int k = std::min(max_steps, remaining_min());
+    if (k <= 0) return 0;
+    int left = k;
+    while (left > 0) {
+      if (use_graph && phase == 0 && left >= graph_steps && graph_steps >= 2 && !graph_failed) {
+        if (!gexec) capture();
+        if (gexec) {
+          HIPCHK(hipGraphLaunch(gexec, stream));
+          ++graph_launches;
+          left -= graph_steps;
+          continue;
+        }
+      }
+      enqueue_step(phase);
+      HIPCHK(hipGetLastError());
+      phase ^= 1;
+      ++eager_steps;
+      --left;
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    // host bookkeeping and the streamed-back histories of the k steps
+    for (auto& bc : cases) {
+      if (bc.done >= bc.target) continue;
+      Level& L = *bc.L;
+      const int row0 = bc.done;
+      std::vector<double> mh(k);
+      HIPCHK(hipMemcpyAsync(mh.data(), L.mass_dev + row0, sizeof(double) * k, hipMemcpyDeviceToHost, stream));
+      std::vector<double> fh;
+      if (L.ib) {
+        fh.resize((size_t)k * L.ncol);
+        HIPCHK(hipMemcpyAsync(fh.data(), L.force_dev + (size_t)row0 * L.ncol, sizeof(double) * k * L.ncol,
+                              hipMemcpyDeviceToHost, stream));
+      }
+      HIPCHK(hipStreamSynchronize(stream));
+      L.mass_host.insert(L.mass_host.end(), mh.begin(), mh.end());
+      L.force_host.insert(L.force_host.end(), fh.begin(), fh.end());
+      for (int s = 0; s < k; ++s) {
+        L.t_old = L.t_new;
+        L.t_new += L.dt;
+      }
+      L.istep += k;
+      L.cur ^= (k & 1);
+      L.state_valid = false;
+      bc.done += k;
+      lups += (double)L.ncells * k;
+    }
+    // a case that finished leaves the lock-step set: tables change
+    for (auto& bc : cases)
+      if (bc.done >= bc.target && bc.done - k < bc.target) dirty = true;
+    return k;
+  }
+};
+
+}  // namespace ilbm
+
+// ---------------------------------------------------------------------------
+// C ABI of the batch (include/ilbm.h)
+// ---------------------------------------------------------------------------
+struct ilbm_batch {
+  ilbm::Batch B;
+};
+
+extern "C" {
+
+int ilbm_batch_create(int64_t hbm_budget_bytes, int max_cases, int graph_steps, ilbm_batch** out) {
+  if (!out || hbm_budget_bytes <= 0 || max_cases <= 0 || graph_steps < 0) return ILBM_ERR_INVALID;
+  *out = nullptr;
+  ilbm_batch* h = new (std::nothrow) ilbm_batch();
+  if (!h) return ILBM_ERR_ALLOC;
+  const int st = guarded([&] {
+    h->B.budget = hbm_budget_bytes;
+    h->B.max_cases = max_cases;
+    h->B.graph_steps = (graph_steps / 2) * 2;
+    HIPCHK(hipStreamCreateWithFlags(&h->B.stream, hipStreamNonBlocking));
+  });
+  if (st != ILBM_OK) {
+    delete h;
+    return st;
+  }
+  *out = h;
+  return ILBM_OK;
+}
+
+int ilbm_batch_destroy(ilbm_batch* h) {
+  if (!h) return ILBM_ERR_INVALID;
+  delete h;
+  return ILBM_OK;
+}
+
+int ilbm_batch_footprint(const ilbm_case_desc* desc, int nsteps, int64_t* bytes) {
+  if (!desc || !bytes || nsteps < 0) return ILBM_ERR_INVALID;
+  *bytes = ilbm::case_footprint(*desc, nsteps);
+  return ILBM_OK;
+}
+
+int ilbm_batch_add(ilbm_batch* h, const ilbm_case_desc* desc, const double* init_macro, int nsteps, int* case_id) {
+  if (!h || !desc || !init_macro || !case_id) return ILBM_ERR_INVALID;
+  *case_id = -1;
+  return guarded([&] { *case_id = h->B.add(*desc, init_macro, nsteps); });
+}
+
+int ilbm_batch_advance(ilbm_batch* h, int max_steps, int use_graph, int* steps_done) {
+  if (!h || !steps_done) return ILBM_ERR_INVALID;
+  *steps_done = 0;
+  return guarded([&] { *steps_done = h->B.advance(max_steps, use_graph != 0); });
+}
+
+int ilbm_batch_finished(ilbm_batch* h, int* ids, int max_ids, int* n) {
+  if (!h || !n) return ILBM_ERR_INVALID;
+  int k = 0;
+  for (const auto& c : h->B.cases)
+    if (c.done >= c.target) {
+      if (ids && k < max_ids) ids[k] = c.id;
+      ++k;
+    }
+  *n = k;
+  return ILBM_OK;
+}
+
+int ilbm_batch_progress(ilbm_batch* h, int case_id, int* done, int* target) {
+  if (!h || !done || !target) return ILBM_ERR_INVALID;
+  const ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  *done = c->done;
+  *target = c->target;
+  return ILBM_OK;
+}
+
+int ilbm_batch_force_history(ilbm_batch* h, int case_id, double* out, int max_rows, int* n_rows, int* n_cols) {
+  if (!h) return ILBM_ERR_INVALID;
+  ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  return force_history_of(*c->L, out, max_rows, n_rows, n_cols);
+}
+
+int ilbm_batch_mass_history(ilbm_batch* h, int case_id, double* out, int max_rows, int* n_out) {
+  if (!h) return ILBM_ERR_INVALID;
+  ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  return mass_history_of(*c->L, out, max_rows, n_out);
+}
+
+int ilbm_batch_remove(ilbm_batch* h, int case_id) {
+  if (!h) return ILBM_ERR_INVALID;
+  return guarded([&] { h->B.remove(case_id); });
+}
+
+int ilbm_batch_case_tile(ilbm_batch* h, int case_id, int* tile_bx) {
+  if (!h || !tile_bx) return ILBM_ERR_INVALID;
+  const ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  *tile_bx = c->tile_bx;
+  return ILBM_OK;
+}
+
+int ilbm_batch_info(ilbm_batch* h, double out[8]) {
+  if (!h || !out) return ILBM_ERR_INVALID;
+  const ilbm::Batch& B = h->B;
+  int active = 0;
+  for (const auto& c : B.cases) active += (c.done < c.target) ? 1 : 0;
+  out[0] = (double)B.cases.size();
+  out[1] = (double)active;
+  out[2] = (double)B.used;
+  out[3] = (double)B.budget;
+  out[4] = (double)B.graph_launches;
+  out[5] = (double)B.captures;
+  out[6] = (double)B.eager_steps;
+  out[7] = B.lups;
+  return ILBM_OK;
+}
+
+}  // extern "C"

@@ -77,6 +77,8 @@ struct IbArgs {
   int first_iter;
   const int* ne_cells;  // band cells that receive spreading entries
   int n_ne;
+  int nloop;            // direct-forcing iterations per step
+  int use_ivc;          // IVC solve (operator factorized) instead of explicit forcing
 };
 
 __device__ inline double delta_fn(double xf, double xp, double h, int type) {
@@ -178,14 +180,15 @@ __device__ inline double ev_value(const IbArgs& a, const int p[3], int comp) {
   return a.EV[comp * a.band_cells + b];
 }
 
-__global__ void k_ib_begin(IbArgs a) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ib_begin_body(const IbArgs& a, int64_t b) {
   if (b >= a.band_cells) return;
   a.EV[b] = a.Mband[a.band_cells + b];
   a.EV[a.band_cells + b] = a.Mband[2 * a.band_cells + b];
   a.EV[2 * a.band_cells + b] = (a.dim == 3) ? a.Mband[3 * a.band_cells + b] : 0.0;
   for (int c = 0; c < 3; ++c) a.EFT[c * a.band_cells + b] = 0.0;
 }
+
+__global__ void k_ib_begin(IbArgs a) { ib_begin_body(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
 
 // cell index and 5-point weights of every marker, and the partition sums
 // that the optional renormalisation uses (:640-690, :1358-1403)
@@ -294,11 +297,11 @@ __device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, doubl
 // parallel, then four lanes accumulate them in the reference's (ii, jj, kk)
 // order, so the sums are bit-identical to the serial loop (:642-681).
 constexpr int kInterpWaves = 4;
-__global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) {
+__device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
   __shared__ double prod[kInterpWaves][4][128];
   __shared__ unsigned char valid[kInterpWaves][128];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m = blockIdx.x * kInterpWaves + wv;
+  const int m = blk * kInterpWaves + wv;
   const bool active = m < a.nm;
   const int nm = a.nm;
   const double d = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
@@ -353,6 +356,8 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) { interp_body(a, blockIdx.x); }
+
 __global__ void k_ib_force_explicit(IbArgs a) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.nm) return;
@@ -363,8 +368,7 @@ __global__ void k_ib_force_explicit(IbArgs a) {
 }
 
 // IVC right-hand side b = (U_target - U) / dt (:1276-1296)
-__global__ void k_ib_ivc_rhs(IbArgs a) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void ivc_rhs_body(const IbArgs& a, int m) {
   if (m >= a.nm) return;
   const int nm = a.nm;
   double Ux, Uy, Uz;
@@ -374,10 +378,12 @@ __global__ void k_ib_ivc_rhs(IbArgs a) {
   a.rhs[2 * nm + m] = (Uz - a.U[2 * nm + m]) / a.dt;
 }
 
+__global__ void k_ib_ivc_rhs(IbArgs a) { ivc_rhs_body(a, blockIdx.x * blockDim.x + threadIdx.x); }
+
 // F += A^{-1} b, one wavefront per marker row (fixed reduction tree).
-__global__ void __launch_bounds__(256) k_ib_ivc_matvec(IbArgs a) {
+__device__ __forceinline__ void ivc_matvec_body(const IbArgs& a, int blk) {
   const int lane = threadIdx.x & 63;
-  const int row = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int row = blk * 4 + (threadIdx.x >> 6);
   if (row >= a.nm) return;
   const int nm = a.nm;
   const double* Ar = a.Ainv + (int64_t)row * nm;
@@ -393,6 +399,8 @@ __global__ void __launch_bounds__(256) k_ib_ivc_matvec(IbArgs a) {
     a.F[row] += sx; a.F[nm + row] += sy; a.F[2 * nm + row] += sz;
   }
 }
+
+__global__ void __launch_bounds__(256) k_ib_ivc_matvec(IbArgs a) { ivc_matvec_body(a, blockIdx.x); }
 
 // weight of every spreading entry: ((tU*tV)*tW)*scale as in :1426 / :1435
 __global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
@@ -414,11 +422,11 @@ __global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
 // serial scatter followed by SumBoundary.
 constexpr int kSpreadWaves = 4;
 constexpr int kSpreadChunk = 128;
-__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread(IbArgs a) {
+__device__ __forceinline__ void spread_body(const IbArgs& a, int blk) {
   __shared__ double pr[kSpreadWaves][3][kSpreadChunk];
   __shared__ int gp[kSpreadWaves][kSpreadChunk];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int ci = blockIdx.x * kSpreadWaves + wv;
+  const int ci = blk * kSpreadWaves + wv;
   const bool active = ci < a.n_ne;
   const int nm = a.nm;
   const int b = active ? a.ne_cells[ci] : 0;
@@ -471,10 +479,12 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread(IbArgs a) {
   }
 }
 
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread(IbArgs a) { spread_body(a, blockIdx.x); }
+
 // Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
 // iterations) and the moments r x (F dv) (:1500-1503), with their sum as
 // the body torque.  One block, fixed reduction tree.
-__global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) {
+__device__ __forceinline__ void sums_body(const IbArgs& a, int first_iter) {
   __shared__ double lds[6][16];
   const int nm = a.nm;
   const int t = threadIdx.x;
@@ -497,13 +507,14 @@ __global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) {
   if (t < 6) {
     double s = 0.0;
     for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) s += lds[t][wv];
-    if (t < 3) a.sums[t] = a.first_iter ? s : a.sums[t] + s;
+    if (t < 3) a.sums[t] = first_iter ? s : a.sums[t] + s;
     else a.sums[t] = s;
   }
 }
 
-__global__ void k_ib_correct(IbArgs a) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) { sums_body(a, a.first_iter); }
+
+__device__ __forceinline__ void correct_body(const IbArgs& a, int64_t b) {
   if (b >= a.band_cells) return;
   for (int c = 0; c < 3; ++c) {
     const int64_t o = c * a.band_cells + b;
@@ -511,6 +522,8 @@ __global__ void k_ib_correct(IbArgs a) {
     a.EFT[o] += a.EFI[o];
   }
 }
+
+__global__ void k_ib_correct(IbArgs a) { correct_body(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x); }
 
 __global__ void k_ib_export(IbArgs a) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -527,6 +540,7 @@ struct RecArgs {
   double* hist;         // history rows of ncol doubles
   int row, ncol, method;
   double time, step, dV, U0, R;
+  const double* clock;  // batched runs: [0] t_new, [1] steps taken, [2] history row of step 0
 };
 
 __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const double sum[3]);
@@ -536,11 +550,12 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const doub
 // which the stream-collide kernel left out because F was not yet known.
 // Each block also sums its exported force for the step's force record.
 template <class LAT, class ST>
-__global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, void* gbuf, double tau, RecArgs rec) {
+__device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
+                                                  const RecArgs& rec, int blk, int nblk) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   __shared__ double lds[3][4];
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = (int64_t)blk * 256 + threadIdx.x;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   if (b < a.band_cells) {
     const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
@@ -580,14 +595,19 @@ __global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, v
     for (int c = 0; c < 3; ++c) {
       double s = 0.0;
       for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) s += lds[c][wv];
-      rec.partial[(int64_t)c * gridDim.x + blockIdx.x] = s;
+      rec.partial[(int64_t)c * nblk + blk] = s;
     }
   }
 }
 
+template <class LAT, class ST>
+__global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, void* gbuf, double tau, RecArgs rec) {
+  export_fixup_body<LAT, ST>(a, g, gbuf, tau, rec, blockIdx.x, gridDim.x);
+}
+
 // Adds the export kernel's block sums in block order and writes the
 // force.dat row of the step.
-__global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) {
+__device__ __forceinline__ void force_record_body(const IbArgs& a, const RecArgs& rec, int nblk) {
   __shared__ double lds[3][4];
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = threadIdx.x; t < nblk; t += blockDim.x)
@@ -604,6 +624,8 @@ __global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int
     write_force_record(a, rec, sum);
   }
 }
+
+__global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) { force_record_body(a, rec, nblk); }
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of
 // the exported force and the marker sums, appended to the history.
@@ -632,10 +654,17 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& rc, const dou
     C[8] = Fxx / denom; C[9] = Fyx / denom;
     C[10] = Fzt / denom; C[11] = Fzb / denom; C[12] = Fze / denom; C[13] = Fzm / denom; C[14] = Fzx / denom;
   }
-  const double tconv = (D > 0.0) ? (rc.time * rc.U0 / D) : 0.0;
-  double* r = rc.hist + (int64_t)rc.row * rc.ncol;
+  double time = rc.time, step = rc.step;
+  int row = rc.row;
+  if (rc.clock) {  // the step's time and row from the case's device clock
+    time = rc.clock[0] + a.dt;
+    step = rc.clock[1] + 1.0;
+    row = (int)(rc.clock[1] - rc.clock[2]);
+  }
+  const double tconv = (D > 0.0) ? (time * rc.U0 / D) : 0.0;
+  double* r = rc.hist + (int64_t)row * rc.ncol;
   int n = 0;
-  r[n++] = rc.step; r[n++] = rc.time;
+  r[n++] = step; r[n++] = time;
   r[n++] = Fxt; r[n++] = Fyt; r[n++] = C[0]; r[n++] = C[1];
   r[n++] = Fxb; r[n++] = Fyb; r[n++] = C[2]; r[n++] = C[3];
   r[n++] = Fxe; r[n++] = Fye; r[n++] = C[4]; r[n++] = C[5];
@@ -645,6 +674,85 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& rc, const dou
     r[n++] = Fzt; r[n++] = C[10]; r[n++] = Fzb; r[n++] = C[11]; r[n++] = Fze; r[n++] = C[12];
     r[n++] = Fzm; r[n++] = C[13]; r[n++] = Fzx; r[n++] = C[14];
   }
+}
+
+// ---------------------------------------------------------------------------
+// Batched launches: every block reads its case from a BlockRef map; one
+// argument table entry per case.  A case whose iteration count is below
+// `it` skips the iteration (uniformly for the whole block).
+// ---------------------------------------------------------------------------
+struct IbCase {
+  IbArgs ib;
+  GridGeom g;
+  void* gbuf;          // populations the export's Guo term goes into
+  double tau;
+  RecArgs rec;
+  int nblk_fix;        // export blocks (256 band cells each)
+};
+
+__global__ void k_ib_begin_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (a.ev_from_macro) return;
+  ib_begin_body(a, (int64_t)r.b * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp_b(const IbCase* __restrict__ A,
+                                                                   const BlockRef* __restrict__ map, int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop) return;
+  interp_body(a, r.b);
+}
+
+__global__ void k_ib_ivc_rhs_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map, int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop || !a.use_ivc) return;
+  ivc_rhs_body(a, r.b * 256 + threadIdx.x);
+}
+
+__global__ void __launch_bounds__(256) k_ib_ivc_matvec_b(const IbCase* __restrict__ A,
+                                                         const BlockRef* __restrict__ map, int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop || !a.use_ivc) return;
+  ivc_matvec_body(a, r.b);
+}
+
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_b(const IbCase* __restrict__ A,
+                                                                   const BlockRef* __restrict__ map, int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop) return;
+  spread_body(a, r.b);
+}
+
+__global__ void __launch_bounds__(1024) k_ib_sums_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map,
+                                                    int it) {
+  const IbArgs& a = A[map[blockIdx.x].c].ib;
+  if (it >= a.nloop) return;
+  sums_body(a, it == 0);
+}
+
+__global__ void k_ib_correct_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map, int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop || a.ev_from_macro) return;
+  correct_body(a, (int64_t)r.b * 256 + threadIdx.x);
+}
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(256) k_ib_export_fixup_b(const IbCase* __restrict__ A,
+                                                           const BlockRef* __restrict__ map) {
+  const BlockRef r = map[blockIdx.x];
+  const IbCase& c = A[r.c];
+  export_fixup_body<LAT, ST>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
+}
+
+__global__ void __launch_bounds__(256) k_force_record_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map) {
+  const IbCase& c = A[map[blockIdx.x].c];
+  force_record_body(c.ib, c.rec, c.nblk_fix);
 }
 
 }  // namespace ilbm

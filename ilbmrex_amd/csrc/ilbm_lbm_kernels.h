@@ -101,6 +101,14 @@ struct StepArgs {
   FaceBuf faces;         // pre-collision boundary layer (non-periodic faces)
   int nonper[3];
   double* partial;       // per-block mass partials
+  int tile_bx;           // tile width of the batched launch (block = tile_bx x kBlock/tile_bx)
+};
+
+// Block of a batched launch: case index into the launch's argument table and
+// the block's index inside that case's own grid.
+struct BlockRef {
+  int c;
+  int b;
 };
 
 // ---------------------------------------------------------------------------
@@ -198,23 +206,17 @@ __device__ inline typename ST::compute_t bgk(typename ST::compute_t f, typename 
 // (A last-block-done reduction would need an agent-scope release per block,
 // which on a multi-XCD device writes back the XCD's L2 -- far costlier than
 // one more small kernel.)
-__device__ inline void finish_mass(const StepArgs& a, double s, double* lds) {
-  (void)lds;
-  if (threadIdx.x == 0 && threadIdx.y == 0)
-    a.partial[((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
+__device__ inline void finish_mass(const StepArgs& a, double s, int64_t blk) {
+  if (threadIdx.x == 0 && threadIdx.y == 0) a.partial[blk] = s;
 }
 
 // ---------------------------------------------------------------------------
 // k_step: pull-stream + forced macros + BGK collision, one pass.
 // ---------------------------------------------------------------------------
 template <class LAT, class ST, bool WRAP>
-__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+__device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k, int64_t blk, double* lds) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
-  __shared__ double lds[kBlock / 64];
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int j = blockIdx.y * blockDim.y + threadIdx.y;
-  const int k = blockIdx.z;
   const GridGeom& g = a.g;
   const bool active = (i < g.n[0]) && (j < g.n[1]);
   double rho_acc = 0.0;
@@ -362,7 +364,38 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     }
   }
   const double s = block_sum(rho_acc, lds);
-  finish_mass(a, s, lds);
+  finish_mass(a, s, blk);
+}
+
+template <class LAT, class ST, bool WRAP>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  __shared__ double lds[kBlock / 64];
+  const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+  step_body<LAT, ST, WRAP>(a, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
+                           blockIdx.z, blk, lds);
+}
+
+// Tile (bx, by) of block b of a case's grid, x fastest, as k_step's 2D grid.
+__device__ inline void batch_tile(const GridGeom& g, int bx, int b, int* i, int* j, int* k) {
+  const int by = kBlock / bx;
+  const int gx = (g.n[0] + bx - 1) / bx, gy = (g.n[1] + by - 1) / by;
+  const int bxi = b % gx, rest = b / gx;
+  const int byi = rest % gy, bz = rest / gy;
+  const int t = threadIdx.x;
+  *i = bxi * bx + (t & (bx - 1));
+  *j = byi * by + t / bx;
+  *k = bz;
+}
+
+// Many cases of one lattice / storage class in one launch.
+template <class LAT, class ST, bool WRAP>
+__global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map) {
+  __shared__ double lds[kBlock / 64];
+  const BlockRef r = map[blockIdx.x];
+  const StepArgs& a = A[r.c];
+  int i, j, k;
+  batch_tile(a.g, a.tile_bx, r.b, &i, &j, &k);
+  step_body<LAT, ST, WRAP>(a, i, j, k, r.b, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
     }
   }
   const double s = block_sum(rho_acc, lds);
-  finish_mass(a, s, lds);
+  finish_mass(a, s, ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -597,11 +630,10 @@ __device__ void fillpatch_value(const GhostArgs& a, const int iv[3], typename ST
 }
 
 template <class LAT, class ST>
-__global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
+__device__ __forceinline__ void ghost_body(const GhostArgs& a, int64_t t) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   const int64_t npad = a.slabs.face_off[a.slabs.nface];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npad) return;
   const GridGeom& g = a.g;
   int iv[3];
@@ -630,18 +662,56 @@ __global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
   for (int q = 0; q < LAT::Q; ++q) ST::store(out, (int64_t)q * npad + t, val[q], T(LAT::w[q]));
 }
 
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_fill_ghosts(GhostArgs a) {
+  ghost_body<LAT, ST>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_fill_ghosts_b(const GhostArgs* __restrict__ A,
+                                                          const BlockRef* __restrict__ map) {
+  const BlockRef r = map[blockIdx.x];
+  ghost_body<LAT, ST>(A[r.c], (int64_t)r.b * kBlock + threadIdx.x);
+}
+
 // Fixed-order reduction of block partials (Evolve's Sum(Rho)).
-__global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t n, double* out) {
+__device__ __forceinline__ double sum_partials_body(const double* p, int64_t n) {
   __shared__ double lds[16];
   double s = 0.0;
   for (int64_t t = threadIdx.x; t < n; t += blockDim.x) s += p[t];
   s = wave_sum(s);
   if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double r = 0.0;
+  double r = 0.0;
+  if (threadIdx.x == 0)
     for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) r += lds[wv];
-    *out = r;
+  return r;
+}
+
+__global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t n, double* out) {
+  const double r = sum_partials_body(p, n);
+  if (threadIdx.x == 0) *out = r;
+}
+
+// End of a batched step, one block per case: Sum(Rho) into the case's mass
+// history row, then the case's clock advances (t_new += dt, step += 1) as
+// timeStepWithSubcycling does (AmrCoreLBM.cpp:1730-1733).
+struct CaseTail {
+  const double* partial;
+  int64_t nblk;
+  double* mass;          // mass history rows
+  double* clock;         // [0] t_new, [1] steps taken, [2] history row of step 0
+  double dt;
+};
+
+__global__ void __launch_bounds__(1024) k_case_tail_b(const CaseTail* __restrict__ T) {
+  const CaseTail& c = T[blockIdx.x];
+  const double r = sum_partials_body(c.partial, c.nblk);
+  if (threadIdx.x == 0) {
+    const int row = (int)(c.clock[1] - c.clock[2]);
+    c.mass[row] = r;
+    c.clock[0] += c.dt;
+    c.clock[1] += 1.0;
   }
 }
 

@@ -342,6 +342,9 @@ struct Level {
   double tau = 0, dt = 0, t_new = 0, t_old = -1.0e100;
   int istep = 0;
   hipStream_t stream = nullptr;
+  bool own_stream = true;
+  double* clock_d = nullptr;     // batched runs: t_new, steps taken, history row base
+  int64_t partial_cap = 0;
   Slabs slabs{};
   double* fpartial = nullptr;    // [3][export blocks]
   // exports
@@ -394,8 +397,8 @@ struct Level {
     dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
-    dev_free(ne_d); dev_free(fpartial);
-    if (stream) (void)hipStreamDestroy(stream);
+    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d);
+    if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 
   void validate() {
@@ -479,7 +482,8 @@ struct Level {
     block = dim3(bx, by, 1);
     grid = dim3((c.n[0] + bx - 1) / bx, (c.n[1] + by - 1) / by, c.n[2]);
     nblocks = (int64_t)grid.x * grid.y * grid.z;
-    partial = dev_alloc<double>(nblocks);
+    ensure_partial(nblocks);
+    clock_d = dev_alloc<double>(4);
 
     auto pk = [&](auto latt) {
       using L = decltype(latt);
@@ -492,6 +496,19 @@ struct Level {
     else pk(D3Q27{});
     tau = 3.0 * c.nu / c.dx[0] + 0.5;  // AmrCoreLBM.cpp:225
     dt = c.dx[0];                       // AmrCoreLBM.cpp:228
+  }
+
+  // number of blocks of a tile of width bx (height kBlock / bx)
+  int64_t tile_blocks(int bx) const {
+    const int by = kBlock / bx;
+    return (int64_t)((c.n[0] + bx - 1) / bx) * ((c.n[1] + by - 1) / by) * c.n[2];
+  }
+
+  void ensure_partial(int64_t n) {
+    if (n <= partial_cap) return;
+    dev_free(partial);
+    partial = dev_alloc<double>(n);
+    partial_cap = n;
   }
 
   // ---------------- immersed boundary set-up ----------------
@@ -719,7 +736,10 @@ struct Level {
     a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
     a.ne_cells = ne_d; a.n_ne = n_ne;
     a.ev_from_macro = (c.coupling_method == 1 || std::max(c.loop_ns, 1) == 1) ? 1 : 0;
-    a.fuse_explicit = (c.coupling_method != 1) ? 1 : 0;
+    a.use_ivc = (c.coupling_method == 1 && ivc_ready) ? 1 : 0;
+    // IVC whose LU failed falls back to explicit forcing (IBMarkerDF.cpp:1253-1259)
+    a.fuse_explicit = (c.coupling_method != 1 || !ivc_ready) ? 1 : 0;
+    a.nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
     a.first_iter = 1;
     a.Ainv = Ainv_d; a.rhs = rhs_d; a.sums = sums_d;
     return a;
@@ -746,10 +766,9 @@ struct Level {
     IbArgs a = ib_args();
     const int tb = 256;
     const unsigned gb = (unsigned)((band_cells + tb - 1) / tb);
-    const bool use_ivc = (c.coupling_method == 1) && ivc_ready;
-    const int nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
+    const bool use_ivc = a.use_ivc != 0;
+    const int nloop = a.nloop;
     if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
-    if (c.coupling_method == 1 && !ivc_ready) a.fuse_explicit = 1;  // LU failed: explicit fallback
     for (int it = 0; it < nloop; ++it) {
       a.first_iter = (it == 0) ? 1 : 0;
       hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
@@ -770,6 +789,7 @@ struct Level {
     r.time = time + dt; r.step = (double)(istep + 1);
     r.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
     r.U0 = c.U0; r.R = c.R;
+    r.clock = nullptr;
     fixfn(dim3(gb), dim3(tb), stream, a, g, gbuf[cur], tau, r);
     hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gb);
     HIPCHK(hipGetLastError());
@@ -806,6 +826,7 @@ struct Level {
     a.faces = faces;
     a.partial = partial;
     a.slabs = slabs;
+    a.tile_bx = (int)block.x;
     return a;
   }
 
@@ -950,6 +971,7 @@ const char* ilbm_status_string(int status) {
     case ILBM_ERR_HIP: return "HIP runtime error";
     case ILBM_ERR_ALLOC: return "allocation failed";
     case ILBM_ERR_STATE: return "invalid state";
+    case ILBM_ERR_FULL: return "batch full";
     default: return "unknown";
   }
 }
@@ -1034,17 +1056,18 @@ int ilbm_level_get_forcing(ilbm_level* h, double* out) {
   });
 }
 
-int ilbm_level_mass_history(ilbm_level* h, double* out, int max_rows, int* n_out) {
-  if (!h || !n_out) return ILBM_ERR_INVALID;
-  const auto& v = h->L.mass_host;
+}  // extern "C"
+
+static int mass_history_of(const ilbm::Level& L, double* out, int max_rows, int* n_out) {
+  if (!n_out) return ILBM_ERR_INVALID;
+  const auto& v = L.mass_host;
   *n_out = (int)v.size();
   if (out) std::copy(v.begin(), v.begin() + std::min<size_t>(v.size(), (size_t)std::max(max_rows, 0)), out);
   return ILBM_OK;
 }
 
-int ilbm_level_force_history(ilbm_level* h, double* out, int max_rows, int* n_rows, int* n_cols) {
-  if (!h || !n_rows || !n_cols) return ILBM_ERR_INVALID;
-  const ilbm::Level& L = h->L;
+static int force_history_of(const ilbm::Level& L, double* out, int max_rows, int* n_rows, int* n_cols) {
+  if (!n_rows || !n_cols) return ILBM_ERR_INVALID;
   *n_cols = L.ncol;
   *n_rows = L.ncol ? (int)(L.force_host.size() / L.ncol) : 0;
   if (out && L.ncol) {
@@ -1052,6 +1075,18 @@ int ilbm_level_force_history(ilbm_level* h, double* out, int max_rows, int* n_ro
     std::copy(L.force_host.begin(), L.force_host.begin() + n, out);
   }
   return ILBM_OK;
+}
+
+extern "C" {
+
+int ilbm_level_mass_history(ilbm_level* h, double* out, int max_rows, int* n_out) {
+  if (!h) return ILBM_ERR_INVALID;
+  return mass_history_of(h->L, out, max_rows, n_out);
+}
+
+int ilbm_level_force_history(ilbm_level* h, double* out, int max_rows, int* n_rows, int* n_cols) {
+  if (!h) return ILBM_ERR_INVALID;
+  return force_history_of(h->L, out, max_rows, n_rows, n_cols);
 }
 
 int ilbm_level_num_markers(ilbm_level* h, int* out) {
@@ -1169,3 +1204,5 @@ int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
 }
 
 }  // extern "C"
+
+#include "ilbm_batch_impl.h"

@@ -55,8 +55,14 @@ EXPORTED = [
     "ilbm_level_marker_dv", "ilbm_level_marker_state", "ilbm_level_marker_force_sum",
     "ilbm_level_marker_torque", "ilbm_level_info", "ilbm_level_kernel_time",
     "ilbm_level_set_timing", "ilbm_copy_probe", "ilbm_status_string", "ilbm_last_error",
-    "ilbm_abi_version",
+    "ilbm_abi_version", "ilbm_case_desc_size",
+    "ilbm_batch_create", "ilbm_batch_destroy", "ilbm_batch_footprint", "ilbm_batch_add",
+    "ilbm_batch_advance", "ilbm_batch_finished", "ilbm_batch_progress",
+    "ilbm_batch_force_history", "ilbm_batch_mass_history", "ilbm_batch_remove",
+    "ilbm_batch_case_tile", "ilbm_batch_info",
 ]
+
+ERR_FULL = 6
 
 _lib = None
 
@@ -91,6 +97,18 @@ def load():
     L.ilbm_level_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_level_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int)]
     L.ilbm_level_set_timing.argtypes = [vp, C.c_int]
+    L.ilbm_batch_create.argtypes = [C.c_int64, C.c_int, C.c_int, P(vp)]
+    L.ilbm_batch_destroy.argtypes = [vp]
+    L.ilbm_batch_footprint.argtypes = [P(CaseDesc), C.c_int, P(C.c_int64)]
+    L.ilbm_batch_add.argtypes = [vp, P(CaseDesc), P(C.c_double), C.c_int, P(C.c_int)]
+    L.ilbm_batch_advance.argtypes = [vp, C.c_int, C.c_int, P(C.c_int)]
+    L.ilbm_batch_finished.argtypes = [vp, P(C.c_int), C.c_int, P(C.c_int)]
+    L.ilbm_batch_progress.argtypes = [vp, C.c_int, P(C.c_int), P(C.c_int)]
+    L.ilbm_batch_force_history.argtypes = [vp, C.c_int, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
+    L.ilbm_batch_mass_history.argtypes = [vp, C.c_int, P(C.c_double), C.c_int, P(C.c_int)]
+    L.ilbm_batch_remove.argtypes = [vp, C.c_int]
+    L.ilbm_batch_case_tile.argtypes = [vp, C.c_int, P(C.c_int)]
+    L.ilbm_batch_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_status_string.restype = C.c_char_p
     L.ilbm_status_string.argtypes = [C.c_int]
     L.ilbm_last_error.restype = C.c_char_p
@@ -261,6 +279,115 @@ class Level:
         ms, n = C.c_double(0), C.c_int(0)
         _check(load().ilbm_level_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
         return ms.value, n.value
+
+
+class BatchFull(IlbmError):
+    """Admission refused: the HBM budget or the case slots are exhausted."""
+
+
+class Batch:
+    """Many independent cases resident on the GPU, advanced together
+    (ilbm_batch_* in include/ilbm.h).  A case is admitted while others run,
+    advances in lock step with them and stops at its own step count."""
+
+    def __init__(self, hbm_budget_bytes: int, max_cases: int = 1024, graph_steps: int = 8):
+        L = load()
+        h = C.c_void_p()
+        _check(L.ilbm_batch_create(int(hbm_budget_bytes), int(max_cases), int(graph_steps), C.byref(h)),
+               "ilbm_batch_create")
+        self._h = h
+        self._cases = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ilbm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    @staticmethod
+    def footprint(case: "cf.Case", nsteps: int) -> int:
+        b = C.c_int64(0)
+        d = to_desc(case)
+        _check(load().ilbm_batch_footprint(C.byref(d), int(nsteps), C.byref(b)), "footprint")
+        return b.value
+
+    def add(self, case: "cf.Case", init_macro: Optional[np.ndarray], nsteps: int) -> int:
+        """Admits a case; raises BatchFull when it does not fit."""
+        case.validate()
+        if init_macro is None:
+            init_macro = cf.uniform_init(case)
+        m = np.ascontiguousarray(init_macro, dtype=np.float64).reshape(4, -1)
+        if m.shape[1] != case.ncells:
+            raise ValueError("init_macro must be [4][cells]")
+        d = to_desc(case)
+        cid = C.c_int(-1)
+        rc = load().ilbm_batch_add(self._h, C.byref(d), _dp(m), int(nsteps), C.byref(cid))
+        if rc == ERR_FULL:
+            raise BatchFull("batch full")
+        _check(rc, "ilbm_batch_add")
+        self._cases[cid.value] = case
+        return cid.value
+
+    def advance(self, max_steps: int, use_graph: bool = True) -> int:
+        n = C.c_int(0)
+        _check(load().ilbm_batch_advance(self._h, int(max_steps), int(use_graph), C.byref(n)), "advance")
+        return n.value
+
+    def finished(self) -> list:
+        n = C.c_int(0)
+        _check(load().ilbm_batch_finished(self._h, None, 0, C.byref(n)), "finished")
+        ids = (C.c_int * max(n.value, 1))()
+        _check(load().ilbm_batch_finished(self._h, ids, n.value, C.byref(n)), "finished")
+        return [ids[i] for i in range(n.value)]
+
+    def progress(self, cid: int):
+        d, t = C.c_int(0), C.c_int(0)
+        _check(load().ilbm_batch_progress(self._h, int(cid), C.byref(d), C.byref(t)), "progress")
+        return d.value, t.value
+
+    def force_history(self, cid: int) -> np.ndarray:
+        L = load()
+        nr, nc = C.c_int(0), C.c_int(0)
+        _check(L.ilbm_batch_force_history(self._h, int(cid), None, 0, C.byref(nr), C.byref(nc)), "force_history")
+        out = np.empty(max(nr.value * nc.value, 1))
+        _check(L.ilbm_batch_force_history(self._h, int(cid), _dp(out), nr.value, C.byref(nr), C.byref(nc)),
+               "force_history")
+        return out[:nr.value * nc.value].reshape(nr.value, max(nc.value, 1))
+
+    def mass_history(self, cid: int) -> np.ndarray:
+        L = load()
+        n = C.c_int(0)
+        _check(L.ilbm_batch_mass_history(self._h, int(cid), None, 0, C.byref(n)), "mass_history")
+        out = np.empty(max(n.value, 1))
+        _check(L.ilbm_batch_mass_history(self._h, int(cid), _dp(out), n.value, C.byref(n)), "mass_history")
+        return out[:n.value]
+
+    def remove(self, cid: int):
+        _check(load().ilbm_batch_remove(self._h, int(cid)), "remove")
+        self._cases.pop(cid, None)
+
+    def tile(self, cid: int) -> int:
+        t = C.c_int(0)
+        _check(load().ilbm_batch_case_tile(self._h, int(cid), C.byref(t)), "case_tile")
+        return t.value
+
+    def info(self) -> dict:
+        out = np.zeros(8)
+        _check(load().ilbm_batch_info(self._h, _dp(out)), "info")
+        return {"resident": int(out[0]), "active": int(out[1]), "bytes": int(out[2]), "budget": int(out[3]),
+                "graph_launches": int(out[4]), "captures": int(out[5]), "eager_steps": int(out[6]),
+                "lups": float(out[7])}
 
 
 def load_probe(nbytes: int, iters: int) -> float:

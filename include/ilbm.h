@@ -44,7 +44,8 @@ typedef enum {
   ILBM_ERR_UNSUPPORTED = 2, /* lattice table / BC type / option not built */
   ILBM_ERR_HIP = 3,         /* HIP runtime error */
   ILBM_ERR_ALLOC = 4,       /* device allocation failed */
-  ILBM_ERR_STATE = 5        /* call not valid in the current state */
+  ILBM_ERR_STATE = 5,       /* call not valid in the current state */
+  ILBM_ERR_FULL = 6         /* batch admission refused: HBM budget or case slots */
 } ilbm_status;
 
 typedef enum { ILBM_STORE_F64 = 0, ILBM_STORE_F32 = 1, ILBM_STORE_F16S = 2 } ilbm_storage;
@@ -133,6 +134,44 @@ int ilbm_level_set_timing(ilbm_level* L, int enable);
  * plain device copy of `bytes` bytes over `iters` launches, used to state
  * achievable HBM bandwidth and to calibrate profiler byte counters. */
 int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms);
+
+/* ------------------------------------------------------------------------
+ * Batched cases (no counterpart in the reference, which runs one case per
+ * process).  Many independent cases stay resident on the device and advance
+ * together: each kernel of a step is launched once for all cases of a
+ * lattice / storage class, and the step sequence is replayed from a
+ * hipGraph of `graph_steps` steps.  Cases are admitted while others run
+ * (continuous batching) as long as their device footprint fits the budget;
+ * each stops at its own step count and keeps its histories until removed.
+ * Results of a case do not depend on which other cases share the batch.
+ * Moving bodies (geometry_type 2) are not batched.
+ * ------------------------------------------------------------------------ */
+typedef struct ilbm_batch ilbm_batch;
+
+int ilbm_batch_create(int64_t hbm_budget_bytes, int max_cases, int graph_steps,
+                      ilbm_batch** out);
+int ilbm_batch_destroy(ilbm_batch* B);
+/* Device bytes a case of nsteps steps is charged at admission. */
+int ilbm_batch_footprint(const ilbm_case_desc* desc, int nsteps, int64_t* bytes);
+/* ILBM_ERR_FULL when the case does not fit; nothing is then allocated. */
+int ilbm_batch_add(ilbm_batch* B, const ilbm_case_desc* desc, const double* init_macro,
+                   int nsteps, int* case_id);
+/* Advances every unfinished case by min(max_steps, smallest remaining
+ * count) steps (*steps_done); histories of those steps are copied back. */
+int ilbm_batch_advance(ilbm_batch* B, int max_steps, int use_graph, int* steps_done);
+int ilbm_batch_finished(ilbm_batch* B, int* ids, int max_ids, int* n);
+int ilbm_batch_progress(ilbm_batch* B, int case_id, int* done, int* target);
+int ilbm_batch_force_history(ilbm_batch* B, int case_id, double* host_out, int max_rows,
+                             int* n_rows, int* n_cols);
+int ilbm_batch_mass_history(ilbm_batch* B, int case_id, double* host_out, int max_rows,
+                            int* n_out);
+int ilbm_batch_remove(ilbm_batch* B, int case_id);
+/* Tile width (cells along x per workgroup) the autotuner chose for a case. */
+int ilbm_batch_case_tile(ilbm_batch* B, int case_id, int* tile_bx);
+/* [0] resident cases, [1] unfinished cases, [2] bytes charged, [3] budget,
+ * [4] graph launches, [5] graph captures, [6] steps launched without a
+ * graph, [7] lattice updates done. */
+int ilbm_batch_info(ilbm_batch* B, double out[8]);
 
 const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */
