@@ -17,6 +17,13 @@ Also reported: the same case with FP64 and FP16-shifted storage, the
 periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6), a plain copy
 kernel's bandwidth, the Cd error of reduced storage against FP64, and the
 CPU oracle (FP64, OpenMP) timed on a bounded sample.
+
+``batched``: the 256-case D2Q9 cylinder ensemble (FP32, 5000 steps, one
+launch per kernel for all cases, hipGraph replay) against the plain per-case
+FP64 path, with the Cd error of FP32 against FP64 on a sample; and the
+continuous-batching server on Poisson arrivals of mixed cylinder / sphere
+requests (p50 / p99 latency, aggregate MLUPS), plus the same short request
+stream served by the plain per-case FP64 path.
 """
 import argparse
 import json
@@ -93,6 +100,54 @@ def cpu_baseline(steps):
                       f"FP64, oracle/refcpu.c with OpenMP ({threads} threads)"}
 
 
+def batched_section(args):
+    from ilbmrex_amd import serving
+    from ilbmrex_amd.native import device_memory
+    free, total = device_memory()
+    budget = int(min(free - (16 << 30), 288e9))
+    out = {"hbm_budget_bytes": budget}
+    cases = serving.ensemble_cases(256, seed=20261202)
+    ns = args.ensemble_steps
+    log(f"[bench] ensemble: 256 cylinder cases FP32, {ns} steps")
+    wall, lups, _, info = serving.run_ensemble(cases, ns, budget)
+    ens = {"cases": 256, "steps": ns, "storage": "f32", "cells": sum(c.ncells for c in cases),
+           "wall_s": round(wall, 4), "mlups": round(lups / wall / 1e6, 1),
+           "graph_launches": info["graph_launches"], "eager_steps": info["eager_steps"]}
+    sample = cases[::32]
+    log("[bench] ensemble: plain per-case FP64 path on 8 of the cases")
+    ens["plain_fp64_mlups"] = round(serving.plain_rate(sample, 500, storage=cf.STORE_F64), 1)
+    ens["plain_fp64_sample"] = "8 of the 256 cases (every 32nd), 500 steps each, one Level at a time"
+    ens["speedup_vs_plain_fp64"] = round(ens["mlups"] / ens["plain_fp64_mlups"], 2)
+    log("[bench] ensemble: Cd of FP32 against FP64 on 8 cases")
+    _, _, h32, _ = serving.run_ensemble(sample, ns, budget, keep=True)
+    _, _, h64, _ = serving.run_ensemble([c.replace(storage=cf.STORE_F64) for c in sample], ns, budget, keep=True)
+    tail = max(1, ns // 5)
+    errs = [abs(a[-tail:, 4].mean() - b[-tail:, 4].mean()) / abs(b[-tail:, 4].mean()) for a, b in zip(h32, h64)]
+    ens["cd_err_f32_vs_f64_max"] = float(max(errs))
+    ens["cd_err_note"] = f"relative error of mean Cd over the last {tail} steps, 8 cases"
+    out["ensemble"] = ens
+    sweep = []
+    for rate in args.rates:
+        reqs = serving.serving_requests(args.requests, rate=rate, seed=20261202)
+        log(f"[bench] serving: {args.requests} requests at {rate}/s (FP32, continuous batching)")
+        st = serving.serve(reqs, budget, chunk=args.chunk)
+        st["rate_per_s"] = rate
+        sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
+    out["serving_f32"] = sweep
+    # the same short stream through both paths
+    rate = args.rates[0]
+    reqs = serving.serving_requests(16, rate=rate, seed=7, steps=(200, 2000))
+    log(f"[bench] serving: 16 short requests at {rate}/s, batched FP32 vs plain FP64")
+    b = serving.serve(reqs, budget, chunk=args.chunk)
+    reqs = serving.serving_requests(16, rate=rate, seed=7, steps=(200, 2000))
+    p = serving.serve_plain(reqs, storage=cf.STORE_F64)
+    out["serving_short_stream"] = {
+        "requests": 16, "rate_per_s": rate, "steps": [200, 2000],
+        "batched_f32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()},
+        "plain_f64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}}
+    return out
+
+
 def read_traffic():
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
@@ -113,6 +168,11 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=5000)
+    ap.add_argument("--requests", type=int, default=32)
+    ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
+    ap.add_argument("--chunk", type=int, default=200)
     ap.add_argument("--probe", action="store_true",
                     help="also run the 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
@@ -172,6 +232,10 @@ def main():
         ms = native.load_probe(1 << 30, 3)
         log(f"[bench] copy probe 1 GiB: {ms:.3f} ms ({2 * (1 << 30) / (ms * 1e-3) / 1e9:.0f} GB/s)")
 
+    batched = None
+    if not args.no_batch:
+        batched = batched_section(args)
+
     cpu = None
     if not args.no_cpu:
         log(f"[bench] CPU oracle: {args.cpu_steps} steps")
@@ -212,6 +276,7 @@ def main():
         },
         "cpu_baseline": cpu,
         "variants": variants,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

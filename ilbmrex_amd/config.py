@@ -379,3 +379,21 @@ def random_init(case: Case, seed: int = 20261202, drho: float = 0.01, du: float 
     if case.dim == 2:
         m[3] = 0.0
     return m
+
+
+def taylor_green_init(case: Case, u_max: float = 0.02, rho0: float = 1.0):
+    """Examples/Taylor_Green_Vortex/Prob.H:9-54 at t = 0 (2D)."""
+    import numpy as np
+    nx, ny = case.n[0], case.n[1]
+    Lx, Ly = nx * case.dx[0], ny * case.dx[1]
+    kx, ky = 2.0 * np.pi / Lx, 2.0 * np.pi / Ly
+    x = case.plo[0] + (0.5 + np.arange(nx)) * case.dx[0]
+    y = case.plo[1] + (0.5 + np.arange(ny)) * case.dx[1]
+    X, Y = np.meshgrid(x, y)          # [j][i]
+    m = np.zeros((4, ny, nx))
+    m[1] = -u_max * np.sqrt(ky / kx) * np.cos(kx * X) * np.sin(ky * Y)
+    m[2] = u_max * np.sqrt(kx / ky) * np.sin(kx * X) * np.cos(ky * Y)
+    p = -0.25 * rho0 * u_max * u_max * ((ky / kx) * np.cos(2 * kx * X) + (kx / ky) * np.cos(2 * ky * Y))
+    m[0] = rho0 + 3.0 * p
+    return m.reshape(4, -1)
+

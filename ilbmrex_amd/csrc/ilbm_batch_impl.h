@@ -636,6 +636,16 @@ int ilbm_batch_case_tile(ilbm_batch* h, int case_id, int* tile_bx) {
   return ILBM_OK;
 }
 
+int ilbm_device_memory(int64_t* free_bytes, int64_t* total_bytes) {
+  if (!free_bytes || !total_bytes) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    *free_bytes = (int64_t)f;
+    *total_bytes = (int64_t)t;
+  });
+}
+
 int ilbm_batch_info(ilbm_batch* h, double out[8]) {
   if (!h || !out) return ILBM_ERR_INVALID;
   const ilbm::Batch& B = h->B;

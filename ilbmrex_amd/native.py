@@ -59,7 +59,7 @@ EXPORTED = [
     "ilbm_batch_create", "ilbm_batch_destroy", "ilbm_batch_footprint", "ilbm_batch_add",
     "ilbm_batch_advance", "ilbm_batch_finished", "ilbm_batch_progress",
     "ilbm_batch_force_history", "ilbm_batch_mass_history", "ilbm_batch_remove",
-    "ilbm_batch_case_tile", "ilbm_batch_info",
+    "ilbm_batch_case_tile", "ilbm_batch_info", "ilbm_device_memory",
 ]
 
 ERR_FULL = 6
@@ -109,6 +109,7 @@ def load():
     L.ilbm_batch_remove.argtypes = [vp, C.c_int]
     L.ilbm_batch_case_tile.argtypes = [vp, C.c_int, P(C.c_int)]
     L.ilbm_batch_info.argtypes = [vp, P(C.c_double)]
+    L.ilbm_device_memory.argtypes = [P(C.c_int64), P(C.c_int64)]
     L.ilbm_status_string.restype = C.c_char_p
     L.ilbm_status_string.argtypes = [C.c_int]
     L.ilbm_last_error.restype = C.c_char_p
@@ -388,6 +389,13 @@ class Batch:
         return {"resident": int(out[0]), "active": int(out[1]), "bytes": int(out[2]), "budget": int(out[3]),
                 "graph_launches": int(out[4]), "captures": int(out[5]), "eager_steps": int(out[6]),
                 "lups": float(out[7])}
+
+
+def device_memory():
+    """(free, total) device bytes."""
+    f, t = C.c_int64(0), C.c_int64(0)
+    _check(load().ilbm_device_memory(C.byref(f), C.byref(t)), "device_memory")
+    return f.value, t.value
 
 
 def load_probe(nbytes: int, iters: int) -> float:

@@ -163,6 +163,33 @@ def serve(requests: List[Request], budget: int, chunk: int = 200, graph_steps: i
             "captures": info["captures"]}
 
 
+def serve_plain(requests: List[Request], storage: int = cf.STORE_F64, clock=time.perf_counter):
+    """The plain per-case path as a server: requests run one at a time, in
+    arrival order, each on its own Level with the given storage."""
+    from .native import Level
+    reqs = sorted(requests, key=lambda r: r.arrival)
+    lups = 0.0
+    t0 = clock()
+    for r in reqs:
+        wait = r.arrival - (clock() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        c = r.case.replace(storage=storage)
+        r.admit = clock() - t0
+        lev = Level(c, initial_state(c))
+        lev.advance(r.nsteps)
+        lev.synchronize()
+        r.history = None
+        lev.close()
+        r.finish = clock() - t0
+        lups += c.ncells * r.nsteps
+    makespan = clock() - t0
+    lat = np.array([r.latency for r in reqs])
+    return {"p50_s": float(np.percentile(lat, 50)), "p99_s": float(np.percentile(lat, 99)),
+            "mean_s": float(lat.mean()), "makespan_s": makespan, "mlups": lups / makespan / 1e6,
+            "requests": len(reqs)}
+
+
 def plain_rate(cases: List[cf.Case], nsteps: int, storage: int = cf.STORE_F64) -> float:
     """MLUPS of the plain per-case path: each case alone on its own Level,
     one after another."""

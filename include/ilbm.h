@@ -172,6 +172,8 @@ int ilbm_batch_case_tile(ilbm_batch* B, int case_id, int* tile_bx);
  * [4] graph launches, [5] graph captures, [6] steps launched without a
  * graph, [7] lattice updates done. */
 int ilbm_batch_info(ilbm_batch* B, double out[8]);
+/* Free and total device memory (hipMemGetInfo), for admission budgets. */
+int ilbm_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
 const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */

@@ -80,10 +80,15 @@ typedef struct {
   int have_cache;
 } rc_ib;
 
+struct rc_amr;
+
 struct rc_level {
   rc_case c;
   int ng;
-  int dom_lo[3], dom_hi[3];
+  int dom_lo[3], dom_hi[3];   /* the level's domain (geom[lev].Domain()) */
+  int vlo[3], vhi[3];         /* the valid box (one box per level) */
+  struct rc_amr *amr;         /* hierarchy this level belongs to, or NULL */
+  int lev;
   fab f_new, f_old, m_new, m_old, forcing;
   double tau, dt;
   double t_new, t_old;
@@ -864,7 +869,7 @@ static void ib_update_forcing(rc_level *L, const fab *smS, fab *F, double dt, do
   rc_ib *ib = &L->ib;
   const rc_case *c = &L->c;
   int lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) { lo[d] = L->dom_lo[d]; hi[d] = L->dom_hi[d]; }
+  for (int d = 0; d < 3; ++d) { lo[d] = L->vlo[d]; hi[d] = L->vhi[d]; }
   for (int d = 0; d < c->dim; ++d) { lo[d] -= 2; hi[d] += 2; }
   fab EV = fab_alloc(lo, hi, 3), EFI = fab_alloc(lo, hi, 3), EFT = fab_alloc(lo, hi, 3);
   for (int k = lo[2]; k <= hi[2]; ++k)
@@ -892,17 +897,17 @@ static void ib_update_forcing(rc_level *L, const fab *smS, fab *F, double dt, do
       sz += ib->fz[m] * ib->dv;
     }
     ib->last_sum[0] += sx; ib->last_sum[1] += sy; ib->last_sum[2] += sz;
-    for (int k = L->dom_lo[2]; k <= L->dom_hi[2]; ++k)
-      for (int j = L->dom_lo[1]; j <= L->dom_hi[1]; ++j)
-        for (int i = L->dom_lo[0]; i <= L->dom_hi[0]; ++i)
+    for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+      for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+        for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
           for (int n = 0; n < 3; ++n) {
             AT(EV, i, j, k, n) += dt * AT(EFI, i, j, k, n);
             AT(EFT, i, j, k, n) += AT(EFI, i, j, k, n);
           }
   }
-  for (int k = L->dom_lo[2]; k <= L->dom_hi[2]; ++k)
-    for (int j = L->dom_lo[1]; j <= L->dom_hi[1]; ++j)
-      for (int i = L->dom_lo[0]; i <= L->dom_hi[0]; ++i)
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
         for (int n = 0; n < 3; ++n)
           AT(*F, i, j, k, n) = dt * AT(*smS, i, j, k, 0) * AT(EFT, i, j, k, n);
   fill_boundary(L, F, 3);
@@ -918,7 +923,10 @@ rc_level *rc_create(const rc_case *c, const double *init_macro) {
   if (L->c.dim == 2) { L->c.n[2] = 1; L->c.periodic[2] = 1; }
   L->ng = 3;
   int lo[3], hi[3];
-  for (int d = 0; d < 3; ++d) { L->dom_lo[d] = 0; L->dom_hi[d] = L->c.n[d] - 1; lo[d] = 0; hi[d] = L->c.n[d] - 1; }
+  for (int d = 0; d < 3; ++d) {
+    L->dom_lo[d] = 0; L->dom_hi[d] = L->c.n[d] - 1; lo[d] = 0; hi[d] = L->c.n[d] - 1;
+    L->vlo[d] = L->dom_lo[d]; L->vhi[d] = L->dom_hi[d];
+  }
   for (int d = 0; d < L->c.dim; ++d) { lo[d] -= L->ng; hi[d] += L->ng; }
   L->f_new = fab_alloc(lo, hi, L->c.q);
   L->f_old = fab_alloc(lo, hi, L->c.q);
@@ -976,6 +984,9 @@ void rc_destroy(rc_level *L) {
   free(L);
 }
 
+static void amr_fill_macro(struct rc_amr *A, int l, double time, fab *dst, int ncomp);
+static void amr_fill_meso(const rc_level *L, fab *sfS, const fab *smS);
+
 /* AdvancePhiAtLevel, Source/Core/AdvanceLBMAtLevel.cpp:20-387 */
 static void advance_level(rc_level *L, double time, double dt_lev) {
   rc_case *c = &L->c;
@@ -992,24 +1003,29 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
     }
   fab sfS = fab_alloc(L->f_new.lo, L->f_new.hi, Q);
   fab smS = fab_alloc(L->m_new.lo, L->m_new.hi, c->nmac);
-  fill_patch(L, &sfS, &L->f_old, Q, 1);
-  fill_patch(L, &smS, &L->m_old, c->nmac, 0);
+  if (L->amr && L->lev > 0) {
+    /* FillPatchMacro / FillPatchMesoscopic for lev > 0 (AmrCoreLBM.cpp:1545-1627) */
+    amr_fill_macro(L->amr, L->lev, time, &smS, c->nmac);
+    amr_fill_meso(L, &sfS, &smS);
+  } else {
+    fill_patch(L, &sfS, &L->f_old, Q, 1);
+    fill_patch(L, &smS, &L->m_old, c->nmac, 0);
+  }
 
   if (c->use_ib) {
     ib_update_forcing(L, &smS, &fx_cc, dt_lev, time);
-    for (int k = L->dom_lo[2]; k <= L->dom_hi[2]; ++k)
-      for (int j = L->dom_lo[1]; j <= L->dom_hi[1]; ++j)
-        for (int i = L->dom_lo[0]; i <= L->dom_hi[0]; ++i)
+    for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+      for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+        for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
           for (int n = 0; n < 3; ++n) AT(L->forcing, i, j, k, n) = AT(fx_cc, i, j, k, n);
   }
   fill_boundary(L, &fx_cc, 3);
 
   const double tau = L->tau;
-  const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int k = 0; k < n2; ++k)
-    for (int j = 0; j < n1; ++j)
-      for (int i = 0; i < n0; ++i) {
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
         const double rho = AT(smS, i, j, k, 0), u = AT(smS, i, j, k, 1), v = AT(smS, i, j, k, 2), w = AT(smS, i, j, k, 3);
         double feq[RC_MAXQ], f[RC_MAXQ];
         for (int q = 0; q < Q; ++q) {
@@ -1028,7 +1044,7 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
       for (int i = sfS.lo[0]; i <= sfS.hi[0]; ++i) {
         const int iv[3] = {i, j, k};
         int valid = 1;
-        for (int d = 0; d < 3; ++d) if (iv[d] < L->dom_lo[d] || iv[d] > L->dom_hi[d]) valid = 0;
+        for (int d = 0; d < 3; ++d) if (iv[d] < L->vlo[d] || iv[d] > L->vhi[d]) valid = 0;
         if (valid) continue;
         int has_bb = 0, unsup = 0;
         for (int d = 0; d < dim; ++d) {
@@ -1039,8 +1055,8 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
         for (int q = 0; q < Q; ++q) {
           const int it = i + (int)c->cx[q], jt = j + (int)c->cy[q];
           const int kt = (dim == 3) ? k + (int)c->cz[q] : k;
-          if (it < L->dom_lo[0] || it > L->dom_hi[0] || jt < L->dom_lo[1] || jt > L->dom_hi[1] ||
-              kt < L->dom_lo[2] || kt > L->dom_hi[2]) continue;
+          if (it < L->vlo[0] || it > L->vhi[0] || jt < L->vlo[1] || jt > L->vhi[1] ||
+              kt < L->vlo[2] || kt > L->vhi[2]) continue;
           const int rx = -(int)c->cx[q], ry = -(int)c->cy[q], rz = (dim == 3) ? -(int)c->cz[q] : 0;
           int qr = q;
           for (int p = 0; p < Q; ++p)
@@ -1052,9 +1068,9 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
   /* stream (Source/LBM/stream.H:9-22) and forced macros
    * (Source/LBM/calculateMacroForcing.H:9-32) */
 #pragma omp parallel for collapse(2) schedule(static)
-  for (int k = 0; k < n2; ++k)
-    for (int j = 0; j < n1; ++j)
-      for (int i = 0; i < n0; ++i) {
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
         for (int q = 0; q < Q; ++q)
           AT(L->f_new, i, j, k, q) = AT(sfS, i - (int)c->cx[q], j - (int)c->cy[q], k - (int)c->cz[q], q);
         double r = 0, m1 = 0, m2 = 0, m3 = 0;
@@ -1079,9 +1095,9 @@ static void compute_ib_force(rc_level *L, double time, int step) {
   double Fx_eul, Fy_eul, Fz_eul, Fx_m = 0, Fy_m = 0, Fz_m = 0;
   {
     double sx = 0, sy = 0, sz = 0;
-    for (int k = L->dom_lo[2]; k <= L->dom_hi[2]; ++k)
-      for (int j = L->dom_lo[1]; j <= L->dom_hi[1]; ++j)
-        for (int i = L->dom_lo[0]; i <= L->dom_hi[0]; ++i) {
+    for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+      for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+        for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
           sx += AT(L->forcing, i, j, k, 0);
           sy += AT(L->forcing, i, j, k, 1);
           sz += AT(L->forcing, i, j, k, 2);
@@ -1149,20 +1165,20 @@ void rc_advance(rc_level *L, int nsteps) {
 
 double rc_mass_sum(const rc_level *L) {
   double s = 0.0;
-  for (int k = L->dom_lo[2]; k <= L->dom_hi[2]; ++k)
-    for (int j = L->dom_lo[1]; j <= L->dom_hi[1]; ++j)
-      for (int i = L->dom_lo[0]; i <= L->dom_hi[0]; ++i) s += AT(L->m_new, i, j, k, 0);
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) s += AT(L->m_new, i, j, k, 0);
   return s;
 }
 
 static int get_valid(const rc_level *L, const fab *F, int ncomp, double *out) {
-  const int n0 = L->c.n[0], n1 = L->c.n[1], n2 = L->c.n[2];
+  const int n0 = L->vhi[0] - L->vlo[0] + 1, n1 = L->vhi[1] - L->vlo[1] + 1, n2 = L->vhi[2] - L->vlo[2] + 1;
   const int64_t nv = (int64_t)n0 * n1 * n2;
   for (int n = 0; n < ncomp; ++n)
     for (int k = 0; k < n2; ++k)
       for (int j = 0; j < n1; ++j)
         for (int i = 0; i < n0; ++i)
-          out[n * nv + ((int64_t)k * n1 + j) * n0 + i] = AT(*F, i, j, k, n);
+          out[n * nv + ((int64_t)k * n1 + j) * n0 + i] = AT(*F, L->vlo[0] + i, L->vlo[1] + j, L->vlo[2] + k, n);
   return 0;
 }
 int rc_get_f(const rc_level *L, double *out) { return get_valid(L, &L->f_new, L->c.q, out); }
@@ -1194,7 +1210,10 @@ int rc_build_markers(const rc_case *c, double time, double *xyz, int maxn, doubl
   memset(&tmp, 0, sizeof(tmp));
   tmp.c = *c;
   if (tmp.c.dim == 2) tmp.c.n[2] = 1;
-  for (int d = 0; d < 3; ++d) { tmp.dom_lo[d] = 0; tmp.dom_hi[d] = tmp.c.n[d] - 1; }
+  for (int d = 0; d < 3; ++d) {
+    tmp.dom_lo[d] = 0; tmp.dom_hi[d] = tmp.c.n[d] - 1;
+    tmp.vlo[d] = tmp.dom_lo[d]; tmp.vhi[d] = tmp.dom_hi[d];
+  }
   ib_build(&tmp);
   ib_update_markers(&tmp, time);
   const int nm = tmp.ib.nm;
@@ -1205,3 +1224,354 @@ int rc_build_markers(const rc_case *c, double time, double *xyz, int maxn, doubl
   rc_ib_free(&tmp.ib);
   return nm;
 }
+
+/* ------------------------------------------------------------------ */
+/* AMR hierarchy: one box per level, refinement ratio 2, subcycling.     */
+/* Source/Core/AmrCoreLBM.cpp: timeStepWithSubcycling :1688-1754,        */
+/* FillPatchMesoscopic lev > 0 :1545-1627, FillPatchMacro :2166-2198,    */
+/* GetDataMacro :2239-2269, AverageDownTo :1520-1526,                    */
+/* MakeNewLevelFromCoarse :751-830, UpdateDerivedMacroFields :150-168.   */
+/* ------------------------------------------------------------------ */
+#define RC_MAXLEV 4
+#define RC_MAXC 8
+
+struct rc_amr {
+  int nlev;
+  rc_level *L[RC_MAXLEV];
+  int regrid_int;
+  int istep;
+};
+
+static int floordiv2(int i) { return (i >= 0) ? i / 2 : -((1 - i) / 2); }
+
+static int in_box(const rc_level *L, const int iv[3]) {
+  for (int d = 0; d < 3; ++d)
+    if (iv[d] < L->vlo[d] || iv[d] > L->vhi[d]) return 0;
+  return 1;
+}
+
+/* GetDataMacro at one cell: the level's macro state at `time` (new, old,
+ * or FillPatchSingleLevel's two-time LinComb a*old + b*new with
+ * a = (t1 - t)/(t1 - t0), b = (t - t0)/(t1 - t0)).  The form of the
+ * LinComb is AMReX's (version unknown): parity unpinned. */
+static double macro_at_time(const rc_level *L, double time, int i, int j, int k, int n) {
+  const double SENT = 1.0e50;
+  if (fabs(L->t_old) > SENT || fabs(L->t_new) > SENT) return AT(L->m_new, i, j, k, n);
+  const double dtloc = fabs(L->t_new - L->t_old);
+  const double teps = fmax(1.0e-12, dtloc * 1.0e-3);
+  if (time >= L->t_new - teps && time <= L->t_new + teps) return AT(L->m_new, i, j, k, n);
+  if (time >= L->t_old - teps && time <= L->t_old + teps) return AT(L->m_old, i, j, k, n);
+  const double a = (L->t_new - time) / (L->t_new - L->t_old);
+  const double b = (time - L->t_old) / (L->t_new - L->t_old);
+  return a * AT(L->m_old, i, j, k, n) + b * AT(L->m_new, i, j, k, n);
+}
+
+/* AMReX cell_cons_interp (CellConservativeLinear with linear limiting) at
+ * fine cell fiv from the coarse level's macro state at `time`: centred
+ * slopes dc = (u[+1] - u[-1])/2 per direction, limited slope
+ * sign(dc) min(|dc|, min(|df|, |db|)) (0 at an extremum) with
+ * df = 2 (u[+1] - u0), db = 2 (u0 - u[-1]); one factor per direction,
+ * min over the ncomp components of limited/centred, scales every
+ * component; fine = u0 + xoff sx + yoff sy (+ zoff sz), offsets +-1/4.
+ * Follows mf_cell_cons_lin_interp_llslope / mf_cell_cons_lin_interp of
+ * AMReX; the version the reference builds against is not recoverable, so
+ * the limiter details are parity unpinned.  Pins: linear fields are
+ * reproduced exactly, the children of a coarse cell average to it. */
+static void cons_interp_cell(const rc_level *Lc, double time, int dim, int ncomp, const int fiv[3], double *out) {
+  const int ic[3] = {floordiv2(fiv[0]), floordiv2(fiv[1]), dim == 3 ? floordiv2(fiv[2]) : fiv[2]};
+  double u0[RC_MAXC], slope[3][RC_MAXC];
+  double sf[3] = {1.0, 1.0, 1.0};
+  for (int d = 0; d < dim; ++d) {
+    int p[3] = {ic[0], ic[1], ic[2]}, m[3] = {ic[0], ic[1], ic[2]};
+    p[d] += 1; m[d] -= 1;
+    if (!in_box(Lc, p) || !in_box(Lc, m)) abort(); /* nesting margin violated */
+  }
+  for (int n = 0; n < ncomp; ++n) u0[n] = macro_at_time(Lc, time, ic[0], ic[1], ic[2], n);
+  for (int d = 0; d < dim; ++d) {
+    int p[3] = {ic[0], ic[1], ic[2]}, m[3] = {ic[0], ic[1], ic[2]};
+    p[d] += 1; m[d] -= 1;
+    for (int n = 0; n < ncomp; ++n) {
+      const double up = macro_at_time(Lc, time, p[0], p[1], p[2], n);
+      const double um = macro_at_time(Lc, time, m[0], m[1], m[2], n);
+      const double dc = 0.5 * (up - um);
+      const double df = 2.0 * (up - u0[n]);
+      const double db = 2.0 * (u0[n] - um);
+      double sx = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+      sx = copysign(1.0, dc) * fmin(sx, fabs(dc));
+      if (dc != 0.0) sf[d] = fmin(sf[d], sx / dc);
+      slope[d][n] = dc;
+    }
+  }
+  for (int d = 0; d < dim; ++d)
+    for (int n = 0; n < ncomp; ++n) slope[d][n] *= sf[d];
+  double off[3];
+  for (int d = 0; d < dim; ++d) off[d] = ((double)(fiv[d] - ic[d] * 2) + 0.5) / 2.0 - 0.5;
+  for (int n = 0; n < ncomp; ++n) {
+    double v = u0[n] + off[0] * slope[0][n];
+    v = v + off[1] * slope[1][n];
+    if (dim == 3) v = v + off[2] * slope[2][n];
+    out[n] = v;
+  }
+}
+
+/* FillPatchMacro(l, time) for l > 0 (FillPatchTwoLevels with
+ * cell_cons_interp): valid cells from the level's own data, every other
+ * cell of dst by interpolation of the coarse level's data at `time`.
+ * Fine boxes sit inside the domain, so no physical ghost occurs. */
+static void amr_fill_macro(struct rc_amr *A, int l, double time, fab *dst, int ncomp) {
+  const rc_level *L = A->L[l], *Lc = A->L[l - 1];
+  const int dim = L->c.dim;
+  for (int k = dst->lo[2]; k <= dst->hi[2]; ++k)
+    for (int j = dst->lo[1]; j <= dst->hi[1]; ++j)
+      for (int i = dst->lo[0]; i <= dst->hi[0]; ++i) {
+        const int iv[3] = {i, j, k};
+        if (in_box(L, iv)) {
+          for (int n = 0; n < ncomp; ++n) AT(*dst, i, j, k, n) = macro_at_time(L, time, i, j, k, n);
+          continue;
+        }
+        double v[RC_MAXC];
+        cons_interp_cell(Lc, time, dim, ncomp, iv, v);
+        for (int n = 0; n < ncomp; ++n) AT(*dst, i, j, k, n) = v[n];
+      }
+}
+
+/* FillPatchMesoscopic for l > 0: valid cells from f_old, ghost cells
+ * replaced by the equilibrium of the FillPatchMacro values
+ * (AmrCoreLBM.cpp:1562-1624; the interpolated populations are discarded). */
+static void amr_fill_meso(const rc_level *L, fab *sfS, const fab *smS) {
+  const rc_case *c = &L->c;
+  for (int k = sfS->lo[2]; k <= sfS->hi[2]; ++k)
+    for (int j = sfS->lo[1]; j <= sfS->hi[1]; ++j)
+      for (int i = sfS->lo[0]; i <= sfS->hi[0]; ++i) {
+        const int iv[3] = {i, j, k};
+        if (in_box(L, iv)) {
+          for (int q = 0; q < c->q; ++q) AT(*sfS, i, j, k, q) = AT(L->f_old, i, j, k, q);
+          continue;
+        }
+        const double r = fmax(AT(*smS, i, j, k, 0), 1.0e-12);
+        const double u0 = AT(*smS, i, j, k, 1), v0 = AT(*smS, i, j, k, 2), w0 = AT(*smS, i, j, k, 3);
+        const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+        for (int q = 0; q < c->q; ++q) {
+          const double cu = c->cx[q] * u0 + c->cy[q] * v0 + c->cz[q] * w0;
+          AT(*sfS, i, j, k, q) = c->w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+        }
+      }
+}
+
+/* vorticityDiagnostic / visPara (Source/LBM/visPara.H:10-54) */
+static void vis_para(const rc_case *c, const fab *m, int i, int j, int k, double *vort, double *P) {
+  const double dx = c->dx[0], dy = c->dx[1], dz = c->dx[2];
+  *P = (AT(*m, i, j, k, 0) - 1.0) / 3.0;
+  const double dv_dx = (AT(*m, i + 1, j, k, 2) - AT(*m, i - 1, j, k, 2)) / (2.0 * dx);
+  const double du_dy = (AT(*m, i, j + 1, k, 1) - AT(*m, i, j - 1, k, 1)) / (2.0 * dy);
+  if (c->dim == 3) {
+    const double du_dz = (AT(*m, i, j, k + 1, 1) - AT(*m, i, j, k - 1, 1)) / (2.0 * dz);
+    const double dw_dx = (AT(*m, i + 1, j, k, 3) - AT(*m, i - 1, j, k, 3)) / (2.0 * dx);
+    const double dv_dz = (AT(*m, i, j, k + 1, 2) - AT(*m, i, j, k - 1, 2)) / (2.0 * dz);
+    const double dw_dy = (AT(*m, i, j + 1, k, 3) - AT(*m, i, j - 1, k, 3)) / (2.0 * dy);
+    const double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+    *vort = sqrt(ox * ox + oy * oy + oz * oz);
+  } else {
+    *vort = dv_dx - du_dy;
+  }
+}
+
+/* UpdateDerivedMacroFields (AmrCoreLBM.cpp:150-168): FillPatchMacro of
+ * comps 0..3 with one ghost at `time`, then vorticity and pressure into
+ * macro_new comps 4 and 5. */
+static void update_derived(struct rc_amr *A, rc_level *L, double time) {
+  if (L->c.nmac < 5) return;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { lo[d] = L->vlo[d]; hi[d] = L->vhi[d]; }
+  for (int d = 0; d < L->c.dim; ++d) { lo[d] -= 1; hi[d] += 1; }
+  fab mp = fab_alloc(lo, hi, 4);
+  if (A && L->lev > 0) {
+    amr_fill_macro(A, L->lev, time, &mp, 4);
+  } else {
+    fab src = fab_alloc(L->m_new.lo, L->m_new.hi, 4);
+    for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+      for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+        for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
+          for (int n = 0; n < 4; ++n) AT(src, i, j, k, n) = macro_at_time(L, time, i, j, k, n);
+    fill_patch(L, &mp, &src, 4, 0);
+    fab_free(&src);
+  }
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
+        double vort, P;
+        vis_para(&L->c, &mp, i, j, k, &vort, &P);
+        AT(L->m_new, i, j, k, 4) = vort;
+        if (L->c.nmac > 5) AT(L->m_new, i, j, k, 5) = P;
+      }
+  fab_free(&mp);
+}
+
+/* AverageDownTo (AmrCoreLBM.cpp:1520-1526): amrex::average_down of f and
+ * of every macro component, c = sum over (kref, jref, iref) of the fine
+ * children, crse = c / ratio^dim. */
+static void average_down_to(struct rc_amr *A, int lc) {
+  rc_level *Lc = A->L[lc], *Lf = A->L[lc + 1];
+  const int dim = Lc->c.dim;
+  const int fk = (dim == 3) ? 2 : 1;
+  const double volfrac = 1.0 / (double)(dim == 3 ? 8 : 4);
+  int clo[3], chi[3];
+  for (int d = 0; d < 3; ++d) {
+    clo[d] = (d < dim) ? floordiv2(Lf->vlo[d]) : Lf->vlo[d];
+    chi[d] = (d < dim) ? floordiv2(Lf->vhi[d]) : Lf->vhi[d];
+  }
+  for (int k = clo[2]; k <= chi[2]; ++k)
+    for (int j = clo[1]; j <= chi[1]; ++j)
+      for (int i = clo[0]; i <= chi[0]; ++i) {
+        const int ii = 2 * i, jj = 2 * j, kk = (dim == 3) ? 2 * k : k;
+        for (int n = 0; n < Lc->c.q; ++n) {
+          double c = 0.0;
+          for (int kr = 0; kr < fk; ++kr)
+            for (int jr = 0; jr < 2; ++jr)
+              for (int ir = 0; ir < 2; ++ir) c += AT(Lf->f_new, ii + ir, jj + jr, kk + kr, n);
+          AT(Lc->f_new, i, j, k, n) = volfrac * c;
+        }
+        for (int n = 0; n < Lc->c.nmac; ++n) {
+          double c = 0.0;
+          for (int kr = 0; kr < fk; ++kr)
+            for (int jr = 0; jr < 2; ++jr)
+              for (int ir = 0; ir < 2; ++ir) c += AT(Lf->m_new, ii + ir, jj + jr, kk + kr, n);
+          AT(Lc->m_new, i, j, k, n) = volfrac * c;
+        }
+      }
+}
+
+/* timeStepWithSubcycling (AmrCoreLBM.cpp:1688-1754) without regrid. */
+static void amr_substep(struct rc_amr *A, int l, double time) {
+  rc_level *L = A->L[l];
+  L->t_old = L->t_new;
+  L->t_new += L->dt;
+  advance_level(L, time, L->dt);
+  const int step_lev = L->istep;
+  if (A->regrid_int > 0 && (step_lev + 1) % A->regrid_int == 0) update_derived(A, L, L->t_new);
+  ++L->istep;
+  if (l < A->nlev - 1) {
+    for (int i = 1; i <= 2; ++i) amr_substep(A, l + 1, time + (double)(i - 1) * A->L[l + 1]->dt);
+    average_down_to(A, l);
+    update_derived(A, L, L->t_new);
+  }
+}
+
+/* a fine level made from the coarse one (MakeNewLevelFromCoarse) */
+static rc_level *amr_new_level(struct rc_amr *A, int l, const rc_case *c0, const int blo[3], const int bhi[3]) {
+  rc_level *L = (rc_level *)calloc(1, sizeof(rc_level));
+  L->c = *c0;
+  if (L->c.dim == 2) { L->c.n[2] = 1; L->c.periodic[2] = 1; }
+  const int fac = 1 << l;
+  for (int d = 0; d < L->c.dim; ++d) { L->c.n[d] = c0->n[d] * fac; L->c.dx[d] = c0->dx[d] / (double)fac; }
+  L->c.use_ib = (l == A->nlev - 1) ? c0->use_ib : 0;
+  L->ng = 3;
+  L->amr = A;
+  L->lev = l;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    L->dom_lo[d] = 0; L->dom_hi[d] = L->c.n[d] - 1;
+    L->vlo[d] = (d < L->c.dim) ? blo[d] : 0;
+    L->vhi[d] = (d < L->c.dim) ? bhi[d] : 0;
+    lo[d] = L->vlo[d]; hi[d] = L->vhi[d];
+  }
+  for (int d = 0; d < L->c.dim; ++d) { lo[d] -= L->ng; hi[d] += L->ng; }
+  L->f_new = fab_alloc(lo, hi, L->c.q);
+  L->f_old = fab_alloc(lo, hi, L->c.q);
+  L->m_new = fab_alloc(lo, hi, L->c.nmac);
+  L->m_old = fab_alloc(lo, hi, L->c.nmac);
+  L->forcing = fab_alloc(lo, hi, 3);
+  L->tau = 0.5 + 3.0 * L->c.nu / L->c.dx[0];   /* AmrCoreLBM.cpp:232-238 */
+  L->dt = L->c.dx[0];
+  L->t_new = 0.0;
+  L->t_old = -1.0e200;
+  /* FillCoarsePatchMacro over the level (InterpFromCoarseLevel), then
+   * f = feq(macro) */
+  const rc_level *Lc = A->L[l - 1];
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
+        const int iv[3] = {i, j, k};
+        double v[RC_MAXC];
+        cons_interp_cell(Lc, 0.0, L->c.dim, L->c.nmac, iv, v);
+        for (int n = 0; n < L->c.nmac; ++n) AT(L->m_new, i, j, k, n) = v[n];
+        const double r = fmax(v[0], 1.0e-12);
+        const double u0 = v[1], v0 = v[2], w0 = v[3];
+        const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+        for (int q = 0; q < L->c.q; ++q) {
+          const double cu = L->c.cx[q] * u0 + L->c.cy[q] * v0 + L->c.cz[q] * w0;
+          const double feq = L->c.w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+          AT(L->f_new, i, j, k, q) = feq;
+          AT(L->f_old, i, j, k, q) = feq;
+        }
+      }
+  memcpy(L->m_old.p, L->m_new.p, sizeof(double) * (size_t)(L->m_new.ns * L->m_new.nc));
+  return L;
+}
+
+rc_amr *rc_amr_create(const rc_case *c0, int nlev, const int *box_lo, const int *box_hi, int regrid_int,
+                      const double *init_macro) {
+  if (nlev < 1 || nlev > RC_MAXLEV || regrid_int < 1 || c0->nmac > RC_MAXC) return NULL;
+  rc_amr *A = (rc_amr *)calloc(1, sizeof(rc_amr));
+  A->nlev = nlev;
+  A->regrid_int = regrid_int;
+  rc_case cc = *c0;
+  if (nlev > 1) cc.use_ib = 0;
+  A->L[0] = rc_create(&cc, init_macro);
+  A->L[0]->amr = A;
+  A->L[0]->lev = 0;
+  A->L[0]->c.use_ib = cc.use_ib;
+  update_derived(A, A->L[0], 0.0);
+  for (int l = 1; l < nlev; ++l) {
+    A->L[l] = amr_new_level(A, l, c0, box_lo + 3 * (l - 1), box_hi + 3 * (l - 1));
+    update_derived(A, A->L[l], 0.0);
+  }
+  rc_level *Lf = A->L[nlev - 1];
+  if (nlev > 1 && Lf->c.use_ib) {
+    ib_build(Lf);
+    ib_update_markers(Lf, 0.0);
+  }
+  return A;
+}
+
+void rc_amr_destroy(rc_amr *A) {
+  if (!A) return;
+  for (int l = 0; l < A->nlev; ++l) rc_destroy(A->L[l]);
+  free(A);
+}
+
+/* one coarse step of Evolve (AmrCoreLBM.cpp:366-420) */
+void rc_amr_step(rc_amr *A) {
+  double cur = A->L[0]->t_new;
+  amr_substep(A, 0, cur);
+  cur += A->L[0]->dt;
+  ++A->istep;
+  rc_level *Lf = A->L[A->nlev - 1];
+  if (Lf->c.use_ib) compute_ib_force(Lf, cur, A->istep);
+  for (int l = 0; l < A->nlev; ++l) A->L[l]->t_new = cur;
+}
+
+void rc_amr_advance(rc_amr *A, int nsteps) {
+  for (int s = 0; s < nsteps; ++s) rc_amr_step(A);
+}
+
+rc_level *rc_amr_level(rc_amr *A, int l) { return (l >= 0 && l < A->nlev) ? A->L[l] : NULL; }
+
+/* macro_new comps 0..nmac-1 on the valid box */
+int rc_get_macro_all(const rc_level *L, double *out) { return get_valid(L, &L->m_new, L->c.nmac, out); }
+
+/* public entry points of the interpolation pieces, for the pins */
+void rc_cons_interp_test(const double *crse, const int *clo, const int *chi, int dim, int ncomp,
+                         const int *fiv, double *out) {
+  rc_level tmp;
+  memset(&tmp, 0, sizeof(tmp));
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { lo[d] = clo[d]; hi[d] = chi[d]; tmp.vlo[d] = clo[d]; tmp.vhi[d] = chi[d]; }
+  tmp.m_new = fab_alloc(lo, hi, ncomp);
+  memcpy(tmp.m_new.p, crse, sizeof(double) * (size_t)(tmp.m_new.ns * ncomp));
+  tmp.t_old = -1.0e200;
+  tmp.t_new = 0.0;
+  cons_interp_cell(&tmp, 0.0, dim, ncomp, fiv, out);
+  fab_free(&tmp.m_new);
+}
+

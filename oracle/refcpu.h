@@ -132,6 +132,21 @@ int rc_lu_factorize(double *a, int *pivot, int n, double tiny);
 void rc_lu_solve(const double *a, const int *pivot, const double *rhs,
                  double *x, int n);
 /* Marker set as build_markers + update_lagrangian_marker(time) produce it. */
+/* AMR hierarchy: nlev levels, one box per level at ratio 2 (level l box
+ * given in level-l cell indices for l >= 1: box_lo/box_hi[3*(l-1)+d]),
+ * subcycling, equilibrium coarse-fine ghosts, average-down, derived
+ * fields every regrid_int steps.  init_macro: level-0 [4][cells]. */
+typedef struct rc_amr rc_amr;
+rc_amr *rc_amr_create(const rc_case *c, int nlev, const int *box_lo, const int *box_hi,
+                      int regrid_int, const double *init_macro);
+void rc_amr_destroy(rc_amr *A);
+void rc_amr_step(rc_amr *A);
+void rc_amr_advance(rc_amr *A, int nsteps);
+rc_level *rc_amr_level(rc_amr *A, int l);
+int rc_get_macro_all(const rc_level *L, double *out);
+void rc_cons_interp_test(const double *crse, const int *clo, const int *chi, int dim, int ncomp,
+                         const int *fiv, double *out);
+
 int rc_build_markers(const rc_case *c, double time, double *xyz, int maxn,
                      double *dv_out);
 

@@ -91,6 +91,16 @@ def lib():
         L.rc_collide_forced.argtypes = [P(C.c_double), P(C.c_double), C.c_int, C.c_double,
                                         P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)] + \
             [C.c_double] * 7 + [C.c_int]
+        L.rc_amr_create.restype = C.c_void_p
+        L.rc_amr_create.argtypes = [P(RcCase), C.c_int, P(C.c_int), P(C.c_int), C.c_int, P(C.c_double)]
+        L.rc_amr_destroy.argtypes = [C.c_void_p]
+        L.rc_amr_step.argtypes = [C.c_void_p]
+        L.rc_amr_advance.argtypes = [C.c_void_p, C.c_int]
+        L.rc_amr_level.restype = C.c_void_p
+        L.rc_amr_level.argtypes = [C.c_void_p, C.c_int]
+        L.rc_get_macro_all.argtypes = [C.c_void_p, P(C.c_double)]
+        L.rc_cons_interp_test.argtypes = [P(C.c_double), P(C.c_int), P(C.c_int), C.c_int, C.c_int,
+                                          P(C.c_int), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -203,6 +213,91 @@ class Oracle:
 
     def tau(self) -> float:
         return lib().rc_tau(self._h)
+
+
+class AmrOracle:
+    """FP64 CPU AMR hierarchy: level 0 = the case's domain, level l >= 1 one
+    box (level-l cell indices, inclusive) at refinement ratio 2."""
+
+    def __init__(self, case, boxes, regrid_int: int, init_macro: np.ndarray):
+        self.case = case
+        self.boxes = [tuple(map(tuple, b)) for b in boxes]
+        self.nlev = 1 + len(boxes)
+        self._c = to_rc_case(case)
+        lo = (C.c_int * max(3 * len(boxes), 1))()
+        hi = (C.c_int * max(3 * len(boxes), 1))()
+        for l, (blo, bhi) in enumerate(self.boxes):
+            for d in range(3):
+                lo[3 * l + d], hi[3 * l + d] = blo[d], bhi[d]
+        m = np.ascontiguousarray(init_macro, dtype=np.float64).reshape(4, -1)
+        self._h = lib().rc_amr_create(C.byref(self._c), self.nlev, lo, hi, int(regrid_int), dptr(m))
+        if not self._h:
+            raise ValueError("rc_amr_create rejected the hierarchy")
+
+    def close(self):
+        if self._h:
+            lib().rc_amr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def advance(self, n: int):
+        lib().rc_amr_advance(self._h, int(n))
+
+    def ncells(self, l):
+        if l == 0:
+            return self.case.ncells
+        lo, hi = self.boxes[l - 1]
+        dim = self.case.lattice.dim
+        return int(np.prod([hi[d] - lo[d] + 1 for d in range(dim)]))
+
+    def _lev(self, l):
+        return lib().rc_amr_level(self._h, int(l))
+
+    def f(self, l) -> np.ndarray:
+        out = np.empty(self.case.lattice.q * self.ncells(l))
+        lib().rc_get_f(self._lev(l), dptr(out))
+        return out.reshape(self.case.lattice.q, -1)
+
+    def macro(self, l, all_comps=False) -> np.ndarray:
+        nc = self.case.nmac if all_comps else 4
+        out = np.empty(nc * self.ncells(l))
+        if all_comps:
+            lib().rc_get_macro_all(self._lev(l), dptr(out))
+        else:
+            lib().rc_get_macro(self._lev(l), dptr(out))
+        return out.reshape(nc, -1)
+
+    def forcing(self, l) -> np.ndarray:
+        out = np.empty(3 * self.ncells(l))
+        lib().rc_get_forcing(self._lev(l), dptr(out))
+        return out.reshape(3, -1)
+
+    def mass_sum(self) -> float:
+        return lib().rc_mass_sum(self._lev(0))
+
+    def force_record(self) -> np.ndarray:
+        out = np.zeros(40)
+        n = lib().rc_force_record(self._lev(self.nlev - 1), dptr(out), 40)
+        return out[:n]
+
+    def num_markers(self) -> int:
+        return lib().rc_num_markers(self._lev(self.nlev - 1))
+
+
+def cons_interp(crse: np.ndarray, clo, dim: int, fiv) -> np.ndarray:
+    """cell_cons_interp of the oracle at fine cell fiv from a coarse block
+    crse[ncomp][nz][ny][nx] whose first cell is clo."""
+    crse = np.ascontiguousarray(crse, dtype=np.float64)
+    nc = crse.shape[0]
+    shp = crse.shape[1:][::-1]
+    lo = (C.c_int * 3)(*clo)
+    hi = (C.c_int * 3)(*[clo[d] + shp[d] - 1 for d in range(3)])
+    f = (C.c_int * 3)(*fiv)
+    out = np.zeros(nc)
+    lib().rc_cons_interp_test(dptr(crse.reshape(-1)), lo, hi, dim, nc, f, dptr(out))
+    return out
 
 
 def build_markers(case, time: float = 0.0):
