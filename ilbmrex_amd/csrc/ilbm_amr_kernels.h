@@ -1,0 +1,262 @@
+// Coarse-fine transfer kernels of the AMR hierarchy (one box per level,
+// refinement ratio 2), following Source/Core/AmrCoreLBM.cpp:
+//
+//   k_cf_fill        fine ghost layer = feq of FillPatchMacro(lev, time): the
+//                    coarse macros of a window around the fine box, linearly
+//                    interpolated in time (GetDataMacro, :2239-2269) and in
+//                    space with AMReX's cell_cons_interp (FillPatchTwoLevels,
+//                    :2180-2197), then the equilibrium that
+//                    FillPatchMesoscopic writes into every ghost (:1562-1624)
+//   k_average_down   AverageDownTo (:1520-1526): covered coarse cells get the
+//                    mean of their fine children's f and macros; their
+//                    post-collision state is recomputed from those values, as
+//                    the next coarse step collides them
+//   k_derived        UpdateDerivedMacroFields / visPara (:150-168,
+//                    visPara.H:10-54) on the window: vorticity, pressure
+//
+// The window of a coarse level holds the macro components the fine ghost
+// interpolation reads: rho, u, v, w (written by the coarse stream-collide
+// kernel as its band) and vorticity, pressure (k_derived).  Two copies, old
+// and new, as macro_old / macro_new.
+#pragma once
+
+#include "ilbm_lbm_kernels.h"
+
+namespace ilbm {
+
+struct Window {
+  const double* Mo;   // [4][cells] macro_old
+  const double* Mn;   // [4][cells] macro_new
+  const double* Do;   // [2][cells] vorticity, pressure of macro_old
+  const double* Dn;
+  int lo[3];          // window box in coarse global cell indices
+  int n[3];
+  int64_t cells;
+  int mode;           // 0: old, 1: new, 2: a*old + b*new
+  double a, b;
+};
+
+__device__ inline double window_val(const Window& W, int i, int j, int k, int comp) {
+  const int64_t w = ((int64_t)(k - W.lo[2]) * W.n[1] + (j - W.lo[1])) * W.n[0] + (i - W.lo[0]);
+  const double* o = (comp < 4) ? W.Mo + comp * W.cells : W.Do + (comp - 4) * W.cells;
+  const double* nw = (comp < 4) ? W.Mn + comp * W.cells : W.Dn + (comp - 4) * W.cells;
+  if (W.mode == 0) return o[w];
+  if (W.mode == 1) return nw[w];
+  return W.a * o[w] + W.b * nw[w];
+}
+
+__device__ inline int floordiv2(int i) { return (i >= 0) ? i / 2 : -((1 - i) / 2); }
+
+// cell_cons_interp at fine global cell (I, J, K): slopes limited per
+// direction with one factor shared by the ncomp components.
+__device__ inline void cons_interp(const Window& W, int dim, int ncomp, int I, int J, int K, double* out) {
+  const int ic = floordiv2(I), jc = floordiv2(J), kc = (dim == 3) ? floordiv2(K) : 0;
+  double u0[6], s[3][6];
+  double sf[3] = {1.0, 1.0, 1.0};
+  for (int n = 0; n < ncomp; ++n) u0[n] = window_val(W, ic, jc, kc, n);
+  for (int d = 0; d < dim; ++d) {
+    const int pi = ic + (d == 0), pj = jc + (d == 1), pk = kc + (d == 2);
+    const int mi = ic - (d == 0), mj = jc - (d == 1), mk = kc - (d == 2);
+    for (int n = 0; n < ncomp; ++n) {
+      const double up = window_val(W, pi, pj, pk, n);
+      const double um = window_val(W, mi, mj, mk, n);
+      const double dc = 0.5 * (up - um);
+      const double df = 2.0 * (up - u0[n]);
+      const double db = 2.0 * (u0[n] - um);
+      double sx = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+      sx = copysign(1.0, dc) * fmin(sx, fabs(dc));
+      if (dc != 0.0) sf[d] = fmin(sf[d], sx / dc);
+      s[d][n] = dc;
+    }
+  }
+  const double ox = ((double)(I - ic * 2) + 0.5) / 2.0 - 0.5;
+  const double oy = ((double)(J - jc * 2) + 0.5) / 2.0 - 0.5;
+  const double oz = (dim == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
+  for (int n = 0; n < 4; ++n) {
+    double v = u0[n] + ox * (s[0][n] * sf[0]);
+    v = v + oy * (s[1][n] * sf[1]);
+    if (dim == 3) v = v + oz * (s[2][n] * sf[2]);
+    out[n] = v;
+  }
+}
+
+struct CfArgs {
+  GridGeom g;          // fine level
+  Slabs slabs;
+  void* gbuf;          // fine populations; the ghost slabs follow the Q planes
+  int ioff[3];         // global index of the fine level's local cell 0
+  int ncomp;           // macro components interpolated together (nmac)
+  Window W;
+};
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  const int64_t npad = a.slabs.face_off[a.slabs.nface];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= npad) return;
+  int iv[3];
+  {
+    const Slabs& S = a.slabs;
+    int f = 0;
+    while (f + 1 < S.nface && t >= S.face_off[f + 1]) ++f;
+    int64_t r = t - S.face_off[f];
+    for (int e = 0; e < 3; ++e) {
+      if (e == S.face_d[f]) { iv[e] = S.face_side[f] ? a.g.n[e] : -1; continue; }
+      const int n = S.face_n[f][e];
+      iv[e] = S.face_lo[f][e] + (int)(r % n);
+      r /= n;
+    }
+  }
+  double m[4];
+  cons_interp(a.W, LAT::D, a.ncomp, iv[0] + a.ioff[0], iv[1] + a.ioff[1], iv[2] + a.ioff[2], m);
+  // FillPatchMesoscopic's equilibrium (AmrCoreLBM.cpp:1605-1620)
+  const double r = fmax(m[0], 1.0e-12);
+  const double u0 = m[1], v0 = m[2], w0 = m[3];
+  const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+  R* out = static_cast<R*>(a.gbuf) + (int64_t)LAT::Q * a.g.qs;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
+    const double fe = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+    ST::store(out, (int64_t)q * npad + t, T(fe), T(LAT::w[q]));
+  }
+}
+
+struct AvgArgs {
+  GridGeom gc;          // coarse level
+  void* gbuf;           // coarse post-collision populations (overwritten on covered cells)
+  int clo[3], cn[3];    // covered coarse cells (coarse local indices)
+  int64_t ncov;
+  const double* fout;   // fine f_new [Q][fine cells]
+  const double* mout;   // fine macro_new [4][fine cells]
+  int fn[3];            // fine level extents
+  int fofs[3];          // fine local index of the first child of coarse cell clo
+  double* Mw;           // coarse window macro_new [4][window cells]
+  int wofs[3];          // window-local index of coarse cell clo
+  int wn[3];
+  int64_t wcells;
+  double* cfout;        // optional coarse exports [Q][cells], [4][cells]
+  double* cmout;
+  double tau, inv_tau;
+  double Fcol[3];       // uniform force of the coarse collision
+  double* dpartial;     // per-block sum of (rho_avg - rho) for Sum(Rho)
+};
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(256) k_average_down(AvgArgs a) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  __shared__ double lds[4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double drho = 0.0;
+  if (t < a.ncov) {
+    const int ci = (int)(t % a.cn[0]);
+    const int cj = (int)((t / a.cn[0]) % a.cn[1]);
+    const int ck = (int)(t / ((int64_t)a.cn[0] * a.cn[1]));
+    const int fk0 = (LAT::D == 3) ? a.fofs[2] + 2 * ck : 0;
+    const int fj0 = a.fofs[1] + 2 * cj, fi0 = a.fofs[0] + 2 * ci;
+    const int nkr = (LAT::D == 3) ? 2 : 1;
+    const int64_t nf = (int64_t)a.fn[0] * a.fn[1] * a.fn[2];
+    const double volfrac = (LAT::D == 3) ? 0.125 : 0.25;
+    double fa[LAT::Q], ma[4];
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      double c = 0.0;
+      for (int kr = 0; kr < nkr; ++kr)
+        for (int jr = 0; jr < 2; ++jr)
+          for (int ir = 0; ir < 2; ++ir)
+            c += a.fout[(int64_t)q * nf + ((int64_t)(fk0 + kr) * a.fn[1] + (fj0 + jr)) * a.fn[0] + fi0 + ir];
+      fa[q] = volfrac * c;
+    }
+    for (int n = 0; n < 4; ++n) {
+      double c = 0.0;
+      for (int kr = 0; kr < nkr; ++kr)
+        for (int jr = 0; jr < 2; ++jr)
+          for (int ir = 0; ir < 2; ++ir)
+            c += a.mout[(int64_t)n * nf + ((int64_t)(fk0 + kr) * a.fn[1] + (fj0 + jr)) * a.fn[0] + fi0 + ir];
+      ma[n] = volfrac * c;
+    }
+    const int64_t w = ((int64_t)(a.wofs[2] + ck) * a.wn[1] + (a.wofs[1] + cj)) * a.wn[0] + a.wofs[0] + ci;
+    drho = ma[0] - a.Mw[w];
+    for (int n = 0; n < 4; ++n) a.Mw[n * a.wcells + w] = ma[n];
+    const int i = a.clo[0] + ci, j = a.clo[1] + cj, k = a.clo[2] + ck;
+    const int64_t cell = ((int64_t)k * a.gc.n[1] + j) * a.gc.n[0] + i;
+    if (a.cmout) {
+      const int64_t nc = (int64_t)a.gc.n[0] * a.gc.n[1] * a.gc.n[2];
+      for (int n = 0; n < 4; ++n) a.cmout[n * nc + cell] = ma[n];
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) a.cfout[(int64_t)q * nc + cell] = fa[q];
+    }
+    // the coarse collision of the averaged state (AdvanceLBMAtLevel.cpp:196-240)
+    const T r = T(ma[0]), u = T(ma[1]), v = T(ma[2]), ww = T(ma[3]);
+    const T uv = usq<LAT, T>(u, v, ww);
+    const T cfx = T(a.Fcol[0]), cfy = T(a.Fcol[1]), cfz = T(a.Fcol[2]);
+    const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
+    const T fac = T(1.0) - T(0.5) / T(a.tau);
+    R* gb = static_cast<R*>(a.gbuf);
+    const int64_t self = a.gc.idx(i, j, k);
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const T fe = feq_q<LAT, T>(q, r, u, v, ww, uv);
+      T gq = bgk<ST>(T(fa[q]), fe, a.tau, a.inv_tau);
+      if (forced) gq += guo_q<LAT, T>(q, fac, u, v, ww, cfx, cfy, cfz);
+      ST::store(gb, (int64_t)q * a.gc.qs + self, gq, T(LAT::w[q]));
+    }
+  }
+  const double s = wave_sum(drho);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) a.dpartial[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+// vorticity and pressure on the window shrunk by one cell (visPara.H:10-54)
+struct DerArgs {
+  const double* M;   // [4][window]
+  double* D;         // [2][window]
+  int n[3];
+  int64_t cells;
+  int dim;
+  double dx, dy, dz;
+};
+
+__global__ void __launch_bounds__(256) k_derived(DerArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int ni = a.n[0] - 2, nj = a.n[1] - 2, nk = (a.dim == 3) ? a.n[2] - 2 : 1;
+  if (t >= (int64_t)ni * nj * nk) return;
+  const int i = 1 + (int)(t % ni), j = 1 + (int)((t / ni) % nj);
+  const int k = (a.dim == 3) ? 1 + (int)(t / ((int64_t)ni * nj)) : 0;
+  auto at = [&](int ii, int jj, int kk, int c) {
+    return a.M[c * a.cells + ((int64_t)kk * a.n[1] + jj) * a.n[0] + ii];
+  };
+  const int64_t w = ((int64_t)k * a.n[1] + j) * a.n[0] + i;
+  const double P = (at(i, j, k, 0) - 1.0) / 3.0;
+  const double dv_dx = (at(i + 1, j, k, 2) - at(i - 1, j, k, 2)) / (2.0 * a.dx);
+  const double du_dy = (at(i, j + 1, k, 1) - at(i, j - 1, k, 1)) / (2.0 * a.dy);
+  double vort;
+  if (a.dim == 3) {
+    const double du_dz = (at(i, j, k + 1, 1) - at(i, j, k - 1, 1)) / (2.0 * a.dz);
+    const double dw_dx = (at(i + 1, j, k, 3) - at(i - 1, j, k, 3)) / (2.0 * a.dx);
+    const double dv_dz = (at(i, j, k + 1, 2) - at(i, j, k - 1, 2)) / (2.0 * a.dz);
+    const double dw_dy = (at(i, j + 1, k, 3) - at(i, j - 1, k, 3)) / (2.0 * a.dy);
+    const double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+    vort = sqrt(ox * ox + oy * oy + oz * oz);
+  } else {
+    vort = dv_dx - du_dy;
+  }
+  a.D[w] = vort;
+  a.D[a.cells + w] = P;
+}
+
+// Sum(Rho) of level 0 after the average-down: the stream-collide kernel's
+// block partials plus the covered cells' corrections, in fixed order.
+__global__ void __launch_bounds__(1024) k_mass_total(const double* p0, int64_t n0, const double* p1, int64_t n1,
+                                                     double* out) {
+  const double a = sum_partials_body(p0, n0);
+  __syncthreads();
+  const double b = (n1 > 0) ? sum_partials_body(p1, n1) : 0.0;
+  if (threadIdx.x == 0) *out = a + b;
+}
+
+}  // namespace ilbm

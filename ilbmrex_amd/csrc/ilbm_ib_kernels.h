@@ -79,6 +79,8 @@ struct IbArgs {
   int n_ne;
   int nloop;            // direct-forcing iterations per step
   int use_ivc;          // IVC solve (operator factorized) instead of explicit forcing
+  int ioff[3];          // global cell index of local cell 0 (refined levels)
+  int dlo[3], dhi[3];   // the level's domain in local cell indices
 };
 
 __device__ inline double delta_fn(double xf, double xp, double h, int type) {
@@ -201,9 +203,9 @@ __global__ void k_ib_weights(IbArgs a) {
   for (int d = 0; d < 3; ++d) {
     if (d < a.dim) {
       const double pos = a.xm[d * nm + m];
-      b[d] = (int)floor((pos - a.plo[d]) / a.dx[d]);
+      b[d] = (int)floor((pos - a.plo[d]) / a.dx[d]) - a.ioff[d];
       for (int s = 0; s < 5; ++s) {
-        const double xi = a.plo[d] + ((double)(b[d] + s - 2) + 0.5) * a.dx[d];
+        const double xi = a.plo[d] + ((double)(b[d] + a.ioff[d] + s - 2) + 0.5) * a.dx[d];
         t[d][s] = delta_fn(xi, pos, a.dx[d], a.kernel_type);
       }
     } else {
@@ -265,16 +267,16 @@ __device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, doubl
     double s2 = 0.0;
     for (int ii = 0; ii < 5; ++ii) {
       const int ip = b0 + ii - 2;
-      if (ip < 0 || ip > a.n[0] - 1) continue;
+      if (ip < a.dlo[0] || ip > a.dhi[0]) continue;
       const double tU = a.wts[(0 * 5 + ii) * nm + m];
       for (int jj = 0; jj < 5; ++jj) {
         const int jp = b1 + jj - 2;
-        if (jp < 0 || jp > a.n[1] - 1) continue;
+        if (jp < a.dlo[1] || jp > a.dhi[1]) continue;
         const double tV = a.wts[(1 * 5 + jj) * nm + m];
         if (a.dim == 3) {
           for (int kk = 0; kk < 5; ++kk) {
             const int kp = b2 + kk - 2;
-            if (kp < 0 || kp > a.n[2] - 1) continue;
+            if (kp < a.dlo[2] || kp > a.dhi[2]) continue;
             const double tW = a.wts[(2 * 5 + kk) * nm + m];
             const double dval = tU * tV * tW;
             s2 += dval * dval * dV;

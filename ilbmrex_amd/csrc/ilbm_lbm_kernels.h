@@ -102,6 +102,7 @@ struct StepArgs {
   int nonper[3];
   double* partial;       // per-block mass partials
   int tile_bx;           // tile width of the batched launch (block = tile_bx x kBlock/tile_bx)
+  int band_force;        // the band carries the IB force (else it only exports macros)
 };
 
 // Block of a batched launch: case index into the launch's argument table and
@@ -300,7 +301,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
                          (unsigned)bk < (unsigned)a.band_n[2];
     const int64_t bidx = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
     T Fx = T(a.Fmac[0]), Fy = T(a.Fmac[1]), Fz = T(a.Fmac[2]);
-    if (in_band) {
+    if (in_band && a.band_force) {
       Fx = T(a.Fband[bidx]);
       Fy = T(a.Fband[a.band_cells + bidx]);
       Fz = T(a.Fband[2 * a.band_cells + bidx]);

@@ -345,6 +345,17 @@ struct Level {
   bool own_stream = true;
   double* clock_d = nullptr;     // batched runs: t_new, steps taken, history row base
   int64_t partial_cap = 0;
+  // refined levels of an AMR hierarchy
+  int ioff[3] = {0, 0, 0};       // global cell index of local cell 0
+  int dom_n[3] = {0, 0, 0};      // the level's domain extent (0: the box)
+  bool cf = false;               // every face is a coarse-fine interface
+  bool window = false;           // keeps macros of a window box for the next finer level
+  double* Mw[2] = {nullptr, nullptr};
+  double* Dw[2] = {nullptr, nullptr};
+  int wsel = 0;                  // Mw[wsel] is macro_new
+  bool rec_enable = true;        // the IB pass writes the force record
+  bool rec_override = false;
+  double rec_time = 0.0, rec_step = 0.0;
   Slabs slabs{};
   double* fpartial = nullptr;    // [3][export blocks]
   // exports
@@ -398,6 +409,7 @@ struct Level {
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
     dev_free(ne_d); dev_free(fpartial); dev_free(clock_d);
+    for (int b = 0; b < 2; ++b) { dev_free(Mw[b]); dev_free(Dw[b]); }
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
 
@@ -406,7 +418,11 @@ struct Level {
     if (c.dim == 2) { c.n[2] = 1; c.periodic[2] = 1; c.bc_lo[2] = c.bc_hi[2] = 0; }
     for (int d = 0; d < 3; ++d)
       if (c.n[d] < 1) throw StatusError{ILBM_ERR_INVALID, "n must be positive"};
-    for (int d = 0; d < c.dim; ++d) {
+    for (int d = 0; d < 3; ++d)
+      if (dom_n[d] == 0) dom_n[d] = c.n[d];
+    if (cf)
+      for (int d = 0; d < c.dim; ++d) { c.periodic[d] = 0; c.bc_lo[d] = c.bc_hi[d] = BC_FOEXTRAP; }
+    for (int d = 0; d < c.dim && !cf; ++d) {
       const int ok[5] = {0, 2, 3, 1001, 1002};
       for (int side = 0; side < 2; ++side) {
         const int b = side ? c.bc_hi[d] : c.bc_lo[d];
@@ -428,7 +444,7 @@ struct Level {
   void setup_layout() {
     wrap = true;
     for (int d = 0; d < c.dim; ++d)
-      if (!c.periodic[d]) { wrap = false; nonper[d] = 1; }
+      if (!c.periodic[d] || cf) { wrap = false; nonper[d] = 1; }
     g.dim = c.dim;
     for (int d = 0; d < 3; ++d) g.n[d] = c.n[d];
     g.pad = 0;
@@ -516,7 +532,13 @@ struct Level {
     ib = c.use_ib != 0;
     if (!ib) return;
     if (c.coupling_method < 0 || c.coupling_method > 2) throw StatusError{ILBM_ERR_INVALID, "coupling_method"};
-    mg = build_marker_geometry(c);
+    {
+      ilbm_case_desc cg = c;  // markers are placed in the level's domain
+      for (int d = 0; d < 3; ++d) cg.n[d] = dom_n[d];
+      mg = build_marker_geometry(cg);
+      c.box_lo[0] = cg.box_lo[0]; c.box_hi[0] = cg.box_hi[0];
+      c.box_lo[1] = cg.box_lo[1]; c.box_hi[1] = cg.box_hi[1];
+    }
     nm = mg.nm;
     moving = (c.geometry_type == 2);
     for (int d = 0; d < c.dim; ++d)
@@ -527,7 +549,7 @@ struct Level {
     auto absorb = [&](const std::vector<double>& xm, int margin) {
       for (int m = 0; m < nm; ++m)
         for (int d = 0; d < c.dim; ++d) {
-          const int b = (int)std::floor((xm[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]);
+          const int b = (int)std::floor((xm[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]) - ioff[d];
           lo[d] = std::min(lo[d], b - 2 - margin);
           hi[d] = std::max(hi[d], b + 2 + margin);
         }
@@ -544,6 +566,8 @@ struct Level {
     absorb(xm_h, moving ? 1 : 0);
     for (int d = 0; d < 3; ++d) {
       if (d >= c.dim) { band_lo[d] = 0; band_n[d] = 1; continue; }
+      if (cf && (lo[d] < 0 || hi[d] > c.n[d] - 1))
+        throw StatusError{ILBM_ERR_UNSUPPORTED, "immersed-boundary stencil reaches the coarse-fine boundary"};
       if (c.periodic[d] && (lo[d] < 0 || hi[d] > c.n[d] - 1)) { band_lo[d] = 0; band_n[d] = c.n[d]; continue; }
       const int l = std::max(lo[d], 0), h = std::min(hi[d], c.n[d] - 1);
       if (h < l) throw StatusError{ILBM_ERR_INVALID, "markers outside the domain"};
@@ -602,7 +626,8 @@ struct Level {
     ents.reserve((size_t)nm * (c.dim == 3 ? 125 : 25));
     for (int m = 0; m < nm; ++m) {
       int b[3] = {0, 0, 0};
-      for (int d = 0; d < c.dim; ++d) b[d] = (int)std::floor((xm_h[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]);
+      for (int d = 0; d < c.dim; ++d)
+        b[d] = (int)std::floor((xm_h[(size_t)d * nm + m] - c.plo[d]) / c.dx[d]) - ioff[d];
       const int kmax = (c.dim == 3) ? 5 : 1;
       for (int ii = 0; ii < 5; ++ii)
         for (int jj = 0; jj < 5; ++jj)
@@ -679,10 +704,10 @@ struct Level {
     std::vector<double> A((size_t)n * n, 0.0);
     for (int i = 0; i < n; ++i)
       for (int j = i; j < n; ++j) {
-        const double sx = overlap_1d(xm_h[i], xm_h[j], c.dx[0], kt, c.plo[0], 0, c.n[0] - 1);
-        const double sy = overlap_1d(xm_h[n + i], xm_h[n + j], c.dx[1], kt, c.plo[1], 0, c.n[1] - 1);
+        const double sx = overlap_1d(xm_h[i], xm_h[j], c.dx[0], kt, c.plo[0], 0, dom_n[0] - 1);
+        const double sy = overlap_1d(xm_h[n + i], xm_h[n + j], c.dx[1], kt, c.plo[1], 0, dom_n[1] - 1);
         const double sz = (c.dim == 3)
-                              ? overlap_1d(xm_h[2 * n + i], xm_h[2 * n + j], c.dx[2], kt, c.plo[2], 0, c.n[2] - 1)
+                              ? overlap_1d(xm_h[2 * n + i], xm_h[2 * n + j], c.dx[2], kt, c.plo[2], 0, dom_n[2] - 1)
                               : 1.0;
         double val = mg.dv * sx * sy * sz;
         if (i == j) val += c.ivc_diag_reg;
@@ -741,6 +766,11 @@ struct Level {
     a.fuse_explicit = (c.coupling_method != 1 || !ivc_ready) ? 1 : 0;
     a.nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
     a.first_iter = 1;
+    for (int d = 0; d < 3; ++d) {
+      a.ioff[d] = ioff[d];
+      a.dlo[d] = -ioff[d];
+      a.dhi[d] = dom_n[d] - 1 - ioff[d];
+    }
     a.Ainv = Ainv_d; a.rhs = rhs_d; a.sums = sums_d;
     return a;
   }
@@ -790,8 +820,9 @@ struct Level {
     r.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
     r.U0 = c.U0; r.R = c.R;
     r.clock = nullptr;
+    if (rec_override) { r.time = rec_time; r.step = rec_step; }
     fixfn(dim3(gb), dim3(tb), stream, a, g, gbuf[cur], tau, r);
-    hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gb);
+    if (rec_enable) hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gb);
     HIPCHK(hipGetLastError());
   }
 
@@ -819,8 +850,11 @@ struct Level {
       a.nonper[d] = nonper[d];
     }
     a.Fband = Fband;
-    a.Mband = Mband;
-    a.band_cells = ib ? band_cells : 0;
+    a.Mband = window ? Mw[wsel] : Mband;
+    a.band_force = ib ? 1 : 0;
+    a.band_cells = (ib || window) ? band_cells : 0;
+    if (window)
+      for (int d = 0; d < 3; ++d) { a.band_lo[d] = band_lo[d]; a.band_n[d] = band_n[d]; }
     a.mout = want_state ? mout : nullptr;
     a.fout = want_state ? fout : nullptr;
     a.faces = faces;
@@ -1206,3 +1240,4 @@ int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
 }  // extern "C"
 
 #include "ilbm_batch_impl.h"
+#include "ilbm_amr_impl.h"

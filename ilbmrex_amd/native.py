@@ -60,6 +60,9 @@ EXPORTED = [
     "ilbm_batch_advance", "ilbm_batch_finished", "ilbm_batch_progress",
     "ilbm_batch_force_history", "ilbm_batch_mass_history", "ilbm_batch_remove",
     "ilbm_batch_case_tile", "ilbm_batch_info", "ilbm_device_memory",
+    "ilbm_amr_create", "ilbm_amr_destroy", "ilbm_amr_advance", "ilbm_amr_level_cells",
+    "ilbm_amr_get_f", "ilbm_amr_get_macro", "ilbm_amr_mass_history", "ilbm_amr_force_history",
+    "ilbm_amr_num_markers", "ilbm_amr_info",
 ]
 
 ERR_FULL = 6
@@ -110,6 +113,16 @@ def load():
     L.ilbm_batch_case_tile.argtypes = [vp, C.c_int, P(C.c_int)]
     L.ilbm_batch_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_device_memory.argtypes = [P(C.c_int64), P(C.c_int64)]
+    L.ilbm_amr_create.argtypes = [P(CaseDesc), C.c_int, P(C.c_int), P(C.c_int), C.c_int, P(C.c_double), P(vp)]
+    L.ilbm_amr_destroy.argtypes = [vp]
+    L.ilbm_amr_advance.argtypes = [vp, C.c_int, C.c_int]
+    L.ilbm_amr_level_cells.argtypes = [vp, C.c_int, P(C.c_int64)]
+    L.ilbm_amr_get_f.argtypes = [vp, C.c_int, P(C.c_double)]
+    L.ilbm_amr_get_macro.argtypes = [vp, C.c_int, P(C.c_double)]
+    L.ilbm_amr_mass_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int)]
+    L.ilbm_amr_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
+    L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
+    L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_status_string.restype = C.c_char_p
     L.ilbm_status_string.argtypes = [C.c_int]
     L.ilbm_last_error.restype = C.c_char_p
@@ -389,6 +402,92 @@ class Batch:
         return {"resident": int(out[0]), "active": int(out[1]), "bytes": int(out[2]), "budget": int(out[3]),
                 "graph_launches": int(out[4]), "captures": int(out[5]), "eager_steps": int(out[6]),
                 "lups": float(out[7])}
+
+
+class AmrHierarchy:
+    """Levels 0..nlev-1 at refinement ratio 2 (ilbm_amr_* in include/ilbm.h):
+    level 0 is the case's domain, each box in ``boxes`` (level-l cell
+    indices, inclusive lo / hi) one refined level, advanced with subcycling.
+    The initial state (rho, ux, uy, uz) is uniform."""
+
+    def __init__(self, case: "cf.Case", boxes, regrid_int: int, state=(1.0, 0.0, 0.0, 0.0)):
+        case.validate()
+        self.case = case
+        self.nlev = 1 + len(boxes)
+        L = load()
+        lo = (C.c_int * max(3 * len(boxes), 1))()
+        hi = (C.c_int * max(3 * len(boxes), 1))()
+        for l, (blo, bhi) in enumerate(boxes):
+            for d in range(3):
+                lo[3 * l + d], hi[3 * l + d] = int(blo[d]), int(bhi[d])
+        self._desc = to_desc(case)
+        st = (C.c_double * 4)(*[float(x) for x in state])
+        h = C.c_void_p()
+        _check(L.ilbm_amr_create(C.byref(self._desc), self.nlev, lo, hi, int(regrid_int), st, C.byref(h)),
+               "ilbm_amr_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().ilbm_amr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def advance(self, nsteps: int, want_state: bool = False):
+        _check(load().ilbm_amr_advance(self._h, int(nsteps), int(want_state)), "amr_advance")
+
+    def cells(self, lev: int) -> int:
+        n = C.c_int64(0)
+        _check(load().ilbm_amr_level_cells(self._h, int(lev), C.byref(n)), "amr_level_cells")
+        return n.value
+
+    def f(self, lev: int) -> np.ndarray:
+        out = np.empty(self.case.lattice.q * self.cells(lev))
+        _check(load().ilbm_amr_get_f(self._h, int(lev), _dp(out)), "amr_get_f")
+        return out.reshape(self.case.lattice.q, -1)
+
+    def macro(self, lev: int) -> np.ndarray:
+        out = np.empty(4 * self.cells(lev))
+        _check(load().ilbm_amr_get_macro(self._h, int(lev), _dp(out)), "amr_get_macro")
+        return out.reshape(4, -1)
+
+    def mass_history(self) -> np.ndarray:
+        n = C.c_int(0)
+        _check(load().ilbm_amr_mass_history(self._h, None, 0, C.byref(n)), "amr_mass_history")
+        out = np.empty(max(n.value, 1))
+        _check(load().ilbm_amr_mass_history(self._h, _dp(out), n.value, C.byref(n)), "amr_mass_history")
+        return out[:n.value]
+
+    def force_history(self) -> np.ndarray:
+        nr, nc = C.c_int(0), C.c_int(0)
+        L = load()
+        _check(L.ilbm_amr_force_history(self._h, None, 0, C.byref(nr), C.byref(nc)), "amr_force_history")
+        out = np.empty(max(nr.value * nc.value, 1))
+        _check(L.ilbm_amr_force_history(self._h, _dp(out), nr.value, C.byref(nr), C.byref(nc)),
+               "amr_force_history")
+        return out[:nr.value * nc.value].reshape(nr.value, max(nc.value, 1))
+
+    def num_markers(self) -> int:
+        n = C.c_int(0)
+        _check(load().ilbm_amr_num_markers(self._h, C.byref(n)), "amr_num_markers")
+        return n.value
+
+    def info(self) -> dict:
+        out = np.zeros(8)
+        _check(load().ilbm_amr_info(self._h, _dp(out)), "amr_info")
+        return {"levels": int(out[0]), "steps": int(out[1]), "time": out[2],
+                "cells": [int(x) for x in out[3:3 + int(out[0])]], "tau_finest": out[6], "tau0": out[7]}
 
 
 def device_memory():

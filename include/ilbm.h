@@ -175,6 +175,39 @@ int ilbm_batch_info(ilbm_batch* B, double out[8]);
 /* Free and total device memory (hipMemGetInfo), for admission budgets. */
 int ilbm_device_memory(int64_t* free_bytes, int64_t* total_bytes);
 
+/* ------------------------------------------------------------------------
+ * AMR hierarchy: level 0 is the case's domain, levels 1..nlev-1 (nlev <= 3)
+ * one box each at refinement ratio 2, advanced with subcycling.  Replaces
+ * timeStepWithSubcycling / AdvancePhiAtLevel for lev > 0 and the
+ * coarse-fine transfer of the reference (Source/Core/AmrCoreLBM.cpp:
+ * FillPatchMesoscopic lev > 0 :1545-1627, FillPatchMacro :2166-2198,
+ * GetDataMacro :2239-2269, AverageDownTo :1520-1526,
+ * MakeNewLevelFromCoarse :751-830, UpdateDerivedMacroFields :150-168);
+ * regridding is left out: the boxes are fixed.  box_lo / box_hi hold, for
+ * level l >= 1, its box in level-l cell indices at [3*(l-1) + d]; a box
+ * covers whole coarse cells and keeps 3 coarse cells from its coarse box.
+ * The immersed body lives on the finest level; the initial state
+ * (rho, ux, uy, uz) is uniform on every level.
+ * ------------------------------------------------------------------------ */
+typedef struct ilbm_amr ilbm_amr;
+
+int ilbm_amr_create(const ilbm_case_desc* desc, int nlev, const int* box_lo, const int* box_hi,
+                    int regrid_int, const double init_state[4], ilbm_amr** out);
+int ilbm_amr_destroy(ilbm_amr* A);
+/* nsteps coarse steps; want_state also exports level 0 on the last one
+ * (refined levels always export their last substep). */
+int ilbm_amr_advance(ilbm_amr* A, int nsteps, int want_state);
+int ilbm_amr_level_cells(ilbm_amr* A, int lev, int64_t* cells);
+int ilbm_amr_get_f(ilbm_amr* A, int lev, double* host_out);      /* [q][box cells] */
+int ilbm_amr_get_macro(ilbm_amr* A, int lev, double* host_out);  /* [4][box cells] */
+/* Sum(Rho) of level 0 and the finest level's force.dat rows per coarse step */
+int ilbm_amr_mass_history(ilbm_amr* A, double* host_out, int max_rows, int* n_out);
+int ilbm_amr_force_history(ilbm_amr* A, double* host_out, int max_rows, int* n_rows, int* n_cols);
+int ilbm_amr_num_markers(ilbm_amr* A, int* out);
+/* [0] levels, [1] coarse steps, [2] time, [3..5] cells of levels 0..2,
+ * [6] tau of the finest level, [7] tau of level 0 */
+int ilbm_amr_info(ilbm_amr* A, double out[8]);
+
 const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */
 const char* ilbm_last_error(void);

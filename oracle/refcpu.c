@@ -1520,8 +1520,10 @@ rc_amr *rc_amr_create(const rc_case *c0, int nlev, const int *box_lo, const int 
   A->L[0] = rc_create(&cc, init_macro);
   A->L[0]->amr = A;
   A->L[0]->lev = 0;
-  A->L[0]->c.use_ib = cc.use_ib;
+  /* MakeNewLevelFromScratch: initdata, UpdateDerivedMacroFields, then
+   * macro_old = macro_new (AmrCoreLBM.cpp:971-983) */
   update_derived(A, A->L[0], 0.0);
+  memcpy(A->L[0]->m_old.p, A->L[0]->m_new.p, sizeof(double) * (size_t)(A->L[0]->m_new.ns * A->L[0]->m_new.nc));
   for (int l = 1; l < nlev; ++l) {
     A->L[l] = amr_new_level(A, l, c0, box_lo + 3 * (l - 1), box_hi + 3 * (l - 1));
     update_derived(A, A->L[l], 0.0);
