@@ -100,6 +100,29 @@ def cpu_baseline(steps):
                       f"FP64, oracle/refcpu.c with OpenMP ({threads} threads)"}
 
 
+def amr_variant(steps):
+    from ilbmrex_amd.native import AmrHierarchy
+    case, boxes = cf.pitching_naca_amr(storage=cf.STORE_F32)
+    log(f"[bench] amr3_pitching_naca_f32: {steps} coarse steps ({4 * steps} finest)")
+    h = AmrHierarchy(case, boxes, 10, (1.0, case.U0, 0.0, 0.0))
+    h.advance(20)
+    info = h.info()
+    t0 = time.perf_counter()
+    h.advance(steps)
+    wall = time.perf_counter() - t0
+    fh = h.force_history()
+    h.close()
+    cells = info["cells"]
+    updates = cells[0] + 2 * cells[1] + 4 * cells[2]
+    return {"mlups_all_levels": updates * steps / wall / 1e6, "ms_per_coarse_step": 1e3 * wall / steps,
+            "finest_steps_per_s": 4 * steps / wall, "cells_per_level": cells,
+            "markers": 256, "cd_mean_last_100": float(np.mean(fh[-100:, 4])),
+            "finite": bool(np.isfinite(fh).all()),
+            "workload": "coarse 512x256 D2Q9, levels at ratio 2 (128x96, 160x128 boxes), chord 64 finest "
+                        "cells, 256 markers, +-10 deg about the quarter chord, k=1.0, Re=1000, U=0.04, "
+                        "explicit direct forcing, FP32, fixed boxes (no regrid)"}
+
+
 def batched_section(args):
     from ilbmrex_amd import serving
     from ilbmrex_amd.native import device_memory
@@ -135,7 +158,7 @@ def batched_section(args):
         sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
     out["serving_f32"] = sweep
     # the same short stream through both paths
-    rate = args.rates[0]
+    rate = args.rates[-1]
     reqs = serving.serving_requests(16, rate=rate, seed=7, steps=(200, 2000))
     log(f"[bench] serving: 16 short requests at {rate}/s, batched FP32 vs plain FP64")
     b = serving.serve(reqs, budget, chunk=args.chunk)
@@ -169,6 +192,7 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--amr-steps", type=int, default=1000)
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
@@ -225,6 +249,17 @@ def main():
             log(f"[bench] {name}: {vs} steps")
             r, _ = run_level(case, pinit, 50, vs)
             variants[name] = r
+        # the reference's CPU-runnable case: D2Q9 cylinder 512x256, FP64
+        for name, coup in (("cylinder512_f64_explicit", cf.COUPLING_EXPLICIT),
+                           ("cylinder512_f64_ivc", cf.COUPLING_IVC)):
+            case = cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0,
+                                  coupling=coup, storage=cf.STORE_F64)
+            log(f"[bench] {name}: 2000 steps")
+            r, fh = run_level(case, cf.uniform_init(case, u=(0.05, 0.0, 0.0)), 100, 2000, want_forces=True)
+            r["cd_last"] = float(fh[-1, 4])
+            variants[name] = r
+        # pitching NACA 0012 on three levels, FP32
+        variants["amr3_pitching_naca_f32"] = amr_variant(args.amr_steps)
         copy_ms = native.load_probe(2 << 30, 20)
         variants["copy_probe_gbs"] = 2 * (2 << 30) / (copy_ms * 1e-3) / 1e9
 

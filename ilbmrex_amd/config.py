@@ -397,3 +397,26 @@ def taylor_green_init(case: Case, u_max: float = 0.02, rho0: float = 1.0):
     m[0] = rho0 + 3.0 * p
     return m.reshape(4, -1)
 
+
+def pitching_naca_amr(storage: int = STORE_F32, U: float = 0.04, Re: float = 1000.0, k_red: float = 1.0):
+    """Pitching NACA 0012 on a three-level hierarchy (ratio 2): coarse
+    512x256, chord 64 finest cells (16 coarse), 256 markers, +-10 deg about
+    the quarter chord at reduced frequency k = pi f c / U, explicit direct
+    forcing on the finest level.  Returns (case, boxes); the boxes cover the
+    swept section with 3-cell nesting margins and stay fixed."""
+    import math
+    chord = 16.0
+    nu = U * chord / Re
+    f = k_red * U / (math.pi * chord)
+    inflow = (1.0, U, 0.0, 0.0)
+    c = Case(lattice=D2Q9, n=(512, 256, 1), nu=nu, U0=U, periodic=(0, 1, 1),
+             bc_lo=(EXT_DIR, INT_DIR, INT_DIR), bc_hi=(FOEXTRAP, INT_DIR, INT_DIR),
+             bc_lo_val=(inflow, (0.0,) * 4, (0.0,) * 4),
+             use_ib=True, x0=128.0, y0=128.0, R=0.5 * chord, coupling_method=COUPLING_EXPLICIT,
+             loop_ns=1, delta_type=4, geometry_type=GEOM_USER, n_marker=256, airfoil_chord=chord,
+             airfoil_thickness=0.12, pitch_mean=0.0, pitch_amplitude=math.radians(10.0),
+             pitch_frequency=f, pitch_phase=0.0, pivot_x=0.25 * chord,
+             force_eval_method=FORCE_EVAL_MARKER, storage=storage, name="pitching_naca_amr")
+    boxes = [((224, 208, 0), (351, 303, 0)), ((480, 448, 0), (639, 575, 0))]
+    return c, boxes
+

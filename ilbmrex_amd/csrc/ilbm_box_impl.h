@@ -1,0 +1,509 @@
+// Per-box entry points on caller-owned device memory in the reference's own
+// layout (compiled as part of ilbm_level.hip).  A box's data is an AMReX
+// Array4 of FP64: one contiguous array per component over the valid box
+// grown by its ghost cells, x fastest (AmrCoreLBM.H:276, nghost = 3).  These
+// are the per-MFIter loops of AdvancePhiAtLevel
+// (Source/Core/AdvanceLBMAtLevel.cpp:196-386) and the BC functors of
+// FillPatchSingleLevel (Source/BC/bc_fill.H:263-347), one kernel each, so a
+// caller that keeps AMReX's MultiFabs (and its ParallelCopy / FillBoundary
+// between boxes) can route the arithmetic of every box here.  The level-
+// resident path (ilbm_level_*) fuses the same steps; both produce the
+// reference's FP64 results.
+#pragma once
+
+namespace ilbm {
+
+struct Fab {
+  double* p;
+  int lo[3], hi[3];
+  int64_t js, ks, ns;
+  int nc;
+  __host__ __device__ double& at(int i, int j, int k, int n) const {
+    return p[(int64_t)(i - lo[0]) + (int64_t)(j - lo[1]) * js + (int64_t)(k - lo[2]) * ks + (int64_t)n * ns];
+  }
+  __host__ __device__ bool contains(int i, int j, int k) const {
+    return i >= lo[0] && i <= hi[0] && j >= lo[1] && j <= hi[1] && k >= lo[2] && k <= hi[2];
+  }
+};
+
+static Fab to_fab(const ilbm_fab& f) {
+  Fab F{};
+  F.p = f.p;
+  for (int d = 0; d < 3; ++d) { F.lo[d] = f.lo[d]; F.hi[d] = f.hi[d]; }
+  F.js = (int64_t)(f.hi[0] - f.lo[0] + 1);
+  F.ks = F.js * (int64_t)(f.hi[1] - f.lo[1] + 1);
+  F.ns = F.ks * (int64_t)(f.hi[2] - f.lo[2] + 1);
+  F.nc = f.ncomp;
+  return F;
+}
+
+struct BoxGeom {
+  int dim;
+  int n[3];                 // domain [0, n-1]
+  int periodic[3];
+  int lo_t[3], hi_t[3];     // BC types
+  double lo_v[3][4], hi_v[3][4];
+  int vlo[3], vhi[3];       // valid box
+};
+
+__device__ inline int nonper_out_cnt(const BoxGeom& G, const int iv[3]) {
+  int c = 0;
+  for (int d = 0; d < G.dim; ++d)
+    if (!G.periodic[d] && (iv[d] < 0 || iv[d] > G.n[d] - 1)) ++c;
+  return c;
+}
+
+// cell of the fab at a linear index
+__device__ inline void fab_cell(const Fab& F, int64_t t, int iv[3]) {
+  iv[0] = F.lo[0] + (int)(t % F.js);
+  iv[1] = F.lo[1] + (int)((t / F.js) % (F.hi[1] - F.lo[1] + 1));
+  iv[2] = F.lo[2] + (int)(t / F.ks);
+}
+
+// InitEquilibrium (AmrCoreLBM.cpp:2146-2161) over a box
+template <class LAT>
+__global__ void k_box_equilibrium(Fab m, Fab f, int lo0, int lo1, int lo2, int nx, int ny, int64_t cells) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = lo0 + (int)(t % nx), j = lo1 + (int)((t / nx) % ny), k = lo2 + (int)(t / ((int64_t)nx * ny));
+  const double r = fmax(m.at(i, j, k, 0), 1.0e-12);
+  const double u0 = m.at(i, j, k, 1), v0 = m.at(i, j, k, 2), w0 = m.at(i, j, k, 3);
+  const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
+    f.at(i, j, k, q) = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+  }
+}
+
+// FillBoundary of one box: ghost cells whose periodic image lies in the
+// valid box take its value (AMReX FillBoundary with the periodicity)
+__global__ void k_box_fill_periodic(Fab F, BoxGeom G) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.ns) return;
+  int iv[3];
+  fab_cell(F, t, iv);
+  bool valid = true, perout = false;
+  int s[3] = {iv[0], iv[1], iv[2]};
+  for (int d = 0; d < 3; ++d) {
+    if (iv[d] >= G.vlo[d] && iv[d] <= G.vhi[d]) continue;
+    valid = false;
+    if (d >= G.dim) return;
+    const bool out = iv[d] < 0 || iv[d] > G.n[d] - 1;
+    if (out && !G.periodic[d]) return;
+    if (out) {
+      perout = true;
+      int r = iv[d] % G.n[d];
+      if (r < 0) r += G.n[d];
+      s[d] = r;
+    }
+  }
+  if (valid || !perout) return;
+  for (int d = 0; d < 3; ++d)
+    if (s[d] < G.vlo[d] || s[d] > G.vhi[d]) return;
+  for (int n = 0; n < F.nc; ++n) F.at(iv[0], iv[1], iv[2], n) = F.at(s[0], s[1], s[2], n);
+}
+
+// One pass of the physical BC fill (cells outside the domain in `pass`
+// non-periodic dimensions): FilccCell's foextrap copies (the last foextrap
+// dimension wins), then mesoscopicBcFill (meso) or macroBcFill.
+template <class LAT>
+__global__ void k_box_fill_physical(Fab F, BoxGeom G, int pass, int meso) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.ns) return;
+  int iv[3];
+  fab_cell(F, t, iv);
+  if (nonper_out_cnt(G, iv) != pass) return;
+  const int nc = F.nc;
+  for (int d = 0; d < G.dim; ++d) {
+    int src[3] = {iv[0], iv[1], iv[2]};
+    bool hit = false;
+    if (iv[d] < 0 && G.lo_t[d] == BC_FOEXTRAP) { src[d] = 0; hit = true; }
+    else if (iv[d] > G.n[d] - 1 && G.hi_t[d] == BC_FOEXTRAP) { src[d] = G.n[d] - 1; hit = true; }
+    if (hit)
+      for (int n = 0; n < nc; ++n) F.at(iv[0], iv[1], iv[2], n) = F.at(src[0], src[1], src[2], n);
+  }
+  // nearest interior cell, clamped in every dimension (bc_fill.H:15-41)
+  int mir[3] = {iv[0], iv[1], iv[2]}, lh[3] = {0, 0, 0}, hh[3] = {0, 0, 0};
+  for (int d = 0; d < G.dim; ++d) {
+    if (iv[d] < 0) { mir[d] = 0; lh[d] = 1; }
+    else if (iv[d] > G.n[d] - 1) { mir[d] = G.n[d] - 1; hh[d] = 1; }
+  }
+  auto first_face = [&](int type, int* bd, int* lo_side) {
+    for (int d = 0; d < G.dim; ++d) {
+      if (lh[d] && G.lo_t[d] == type) { *bd = d; *lo_side = 1; return true; }
+      if (hh[d] && G.hi_t[d] == type) { *bd = d; *lo_side = 0; return true; }
+    }
+    return false;
+  };
+  int bd = -1, los = 1;
+  if (!meso) {  // macroBcFill (bc_fill.H:263-301)
+    if (first_face(BC_USER_2, &bd, &los)) {
+      const double* b = los ? G.lo_v[bd] : G.hi_v[bd];
+      for (int n = 0; n < nc; ++n) F.at(iv[0], iv[1], iv[2], n) = F.at(mir[0], mir[1], mir[2], n);
+      if (nc > 0) F.at(iv[0], iv[1], iv[2], 0) = fmax(1.0e-20, 2.0 * b[0] - F.at(mir[0], mir[1], mir[2], 0));
+      for (int n = 1; n < 4 && n < nc; ++n) F.at(iv[0], iv[1], iv[2], n) = 2.0 * b[n] - F.at(mir[0], mir[1], mir[2], n);
+      return;
+    }
+    if (first_face(BC_USER_1, &bd, &los) || first_face(BC_EXT_DIR, &bd, &los)) {
+      const double* b = los ? G.lo_v[bd] : G.hi_v[bd];
+      for (int n = 0; n < nc; ++n) F.at(iv[0], iv[1], iv[2], n) = F.at(mir[0], mir[1], mir[2], n);
+      for (int n = 0; n < 4 && n < nc; ++n) F.at(iv[0], iv[1], iv[2], n) = b[n];
+    }
+    return;
+  }
+  int kind = 0;
+  if (first_face(BC_USER_2, &bd, &los)) kind = 2;
+  else if (first_face(BC_USER_1, &bd, &los)) kind = 1;
+  else if (first_face(BC_EXT_DIR, &bd, &los)) kind = 3;
+  if (kind == 0) return;
+  if (kind == 1) {  // copyMesoscopicFromMirror (:147-154)
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) F.at(iv[0], iv[1], iv[2], q) = F.at(mir[0], mir[1], mir[2], q);
+    return;
+  }
+  double fnp[LAT::Q];
+  double rhonp = 0.0, unp = 0.0, vnp = 0.0, wnp = 0.0;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    fnp[q] = F.at(mir[0], mir[1], mir[2], q);
+    rhonp += fnp[q];
+    unp += double(LAT::cx[q]) * fnp[q];
+    vnp += double(LAT::cy[q]) * fnp[q];
+    wnp += double(LAT::cz[q]) * fnp[q];
+  }
+  rhonp = fmax(rhonp, 1.0e-20);
+  unp /= rhonp; vnp /= rhonp; wnp /= rhonp;
+  const double* b = los ? G.lo_v[bd] : G.hi_v[bd];
+  double rg = b[0], ug = b[1], vg = b[2], wg = b[3];
+  if (kind == 2) {  // applyFaceMirroredMesoscopic (:209-261)
+    rg = fmax(1.0e-20, 2.0 * b[0] - rhonp);
+    ug = 2.0 * b[1] - unp; vg = 2.0 * b[2] - vnp; wg = 2.0 * b[3] - wnp;
+  }
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {  // applyExtDirMesoscopic (:159-204) / mirrored
+    const double cx = LAT::cx[q], cy = LAT::cy[q], cz = LAT::cz[q];
+    const double eu_np = cx * unp + cy * vnp + cz * wnp;
+    const double uv_np = unp * unp + vnp * vnp + wnp * wnp;
+    const double feqnp = LAT::w[q] * rhonp * (1.0 + 3.0 * eu_np + 4.5 * eu_np * eu_np - 1.5 * uv_np);
+    const double eu_g = cx * ug + cy * vg + cz * wg;
+    const double uv_g = ug * ug + vg * vg + wg * wg;
+    const double feqg = LAT::w[q] * rg * (1.0 + 3.0 * eu_g + 4.5 * eu_g * eu_g - 1.5 * uv_g);
+    F.at(iv[0], iv[1], iv[2], q) = feqg + (fnp[q] - feqnp);
+  }
+}
+
+// collide / collide_forced over the valid box (AdvanceLBMAtLevel.cpp:196-240)
+template <class LAT>
+__global__ void k_box_collide(Fab f, Fab m, Fab F, double tau, BoxGeom G) {
+  const int nx = G.vhi[0] - G.vlo[0] + 1, ny = G.vhi[1] - G.vlo[1] + 1;
+  const int64_t cells = (int64_t)nx * ny * (G.vhi[2] - G.vlo[2] + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = G.vlo[0] + (int)(t % nx), j = G.vlo[1] + (int)((t / nx) % ny);
+  const int k = G.vlo[2] + (int)(t / ((int64_t)nx * ny));
+  const double rho = m.at(i, j, k, 0), u = m.at(i, j, k, 1), v = m.at(i, j, k, 2), w = m.at(i, j, k, 3);
+  const double fx = F.p ? F.at(i, j, k, 0) : 0.0, fy = F.p ? F.at(i, j, k, 1) : 0.0;
+  const double fz = F.p ? F.at(i, j, k, 2) : 0.0;
+  const double uv = usq<LAT, double>(u, v, w);
+  const bool forced = !(fx == 0.0 && fy == 0.0 && fz == 0.0);
+  const double fac = 1.0 - 0.5 / tau;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double fe = feq_q<LAT, double>(q, rho, u, v, w, uv);
+    double g = bgk<StF64>(f.at(i, j, k, q), fe, tau, 1.0 / tau);
+    if (forced) g += guo_q<LAT, double>(q, fac, u, v, w, fx, fy, fz);
+    f.at(i, j, k, q) = g;
+  }
+}
+
+// halfway bounce-back into user_1 ghost cells (AdvanceLBMAtLevel.cpp:265-373)
+template <class LAT>
+__global__ void k_box_bounce_back(Fab f, BoxGeom G) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= f.ns) return;
+  int iv[3];
+  fab_cell(f, t, iv);
+  bool valid = true;
+  for (int d = 0; d < 3; ++d)
+    if (iv[d] < G.vlo[d] || iv[d] > G.vhi[d]) valid = false;
+  if (valid) return;
+  bool has_bb = false, unsup = false;
+  for (int d = 0; d < G.dim; ++d) {
+    if (iv[d] < 0) { if (G.lo_t[d] == BC_USER_1) has_bb = true; else unsup = true; }
+    else if (iv[d] > G.n[d] - 1) { if (G.hi_t[d] == BC_USER_1) has_bb = true; else unsup = true; }
+  }
+  if (!has_bb || unsup) return;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const int it = iv[0] + LAT::cx[q], jt = iv[1] + LAT::cy[q], kt = iv[2] + LAT::cz[q];
+    if (it < G.vlo[0] || it > G.vhi[0] || jt < G.vlo[1] || jt > G.vhi[1] || kt < G.vlo[2] || kt > G.vhi[2]) continue;
+    f.at(iv[0], iv[1], iv[2], q) = f.at(it, jt, kt, opposite<LAT>(q));
+  }
+}
+
+// stream (stream.H:9-22) and calculateMacroForcing
+// (calculateMacroForcing.H:9-32) over the valid box
+template <class LAT>
+__global__ void k_box_stream_macro(Fab fin, Fab fout, Fab mout, Fab F, BoxGeom G) {
+  const int nx = G.vhi[0] - G.vlo[0] + 1, ny = G.vhi[1] - G.vlo[1] + 1;
+  const int64_t cells = (int64_t)nx * ny * (G.vhi[2] - G.vlo[2] + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = G.vlo[0] + (int)(t % nx), j = G.vlo[1] + (int)((t / nx) % ny);
+  const int k = G.vlo[2] + (int)(t / ((int64_t)nx * ny));
+  double r = 0.0, m1 = 0.0, m2 = 0.0, m3 = 0.0;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double fq = fin.at(i - LAT::cx[q], j - LAT::cy[q], k - LAT::cz[q], q);
+    fout.at(i, j, k, q) = fq;
+    r += fq;
+    m1 += double(LAT::cx[q]) * fq;
+    m2 += double(LAT::cy[q]) * fq;
+    m3 += double(LAT::cz[q]) * fq;
+  }
+  const double fx = F.p ? F.at(i, j, k, 0) : 0.0, fy = F.p ? F.at(i, j, k, 1) : 0.0;
+  const double fz = F.p ? F.at(i, j, k, 2) : 0.0;
+  mout.at(i, j, k, 0) = r;
+  mout.at(i, j, k, 1) = (m1 + 0.5 * fx) / r;
+  mout.at(i, j, k, 2) = (m2 + 0.5 * fy) / r;
+  mout.at(i, j, k, 3) = (m3 + 0.5 * fz) / r;
+}
+
+// vorticity and pressure (visPara.H:10-54) from a macro patch with one ghost
+__global__ void k_box_derived(Fab mp, Fab mout, BoxGeom G, double dx, double dy, double dz) {
+  const int nx = G.vhi[0] - G.vlo[0] + 1, ny = G.vhi[1] - G.vlo[1] + 1;
+  const int64_t cells = (int64_t)nx * ny * (G.vhi[2] - G.vlo[2] + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = G.vlo[0] + (int)(t % nx), j = G.vlo[1] + (int)((t / nx) % ny);
+  const int k = G.vlo[2] + (int)(t / ((int64_t)nx * ny));
+  const double P = (mp.at(i, j, k, 0) - 1.0) / 3.0;
+  const double dv_dx = (mp.at(i + 1, j, k, 2) - mp.at(i - 1, j, k, 2)) / (2.0 * dx);
+  const double du_dy = (mp.at(i, j + 1, k, 1) - mp.at(i, j - 1, k, 1)) / (2.0 * dy);
+  double vort;
+  if (G.dim == 3) {
+    const double du_dz = (mp.at(i, j, k + 1, 1) - mp.at(i, j, k - 1, 1)) / (2.0 * dz);
+    const double dw_dx = (mp.at(i + 1, j, k, 3) - mp.at(i - 1, j, k, 3)) / (2.0 * dx);
+    const double dv_dz = (mp.at(i, j, k + 1, 2) - mp.at(i, j, k - 1, 2)) / (2.0 * dz);
+    const double dw_dy = (mp.at(i, j + 1, k, 3) - mp.at(i, j - 1, k, 3)) / (2.0 * dy);
+    const double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+    vort = sqrt(ox * ox + oy * oy + oz * oz);
+  } else {
+    vort = dv_dx - du_dy;
+  }
+  mout.at(i, j, k, 4) = vort;
+  if (mout.nc > 5) mout.at(i, j, k, 5) = P;
+}
+
+// per-block partial sums of one component over a box, fixed order
+__global__ void __launch_bounds__(256) k_box_sum(Fab F, int comp, BoxGeom G, double* partial) {
+  __shared__ double lds[4];
+  const int nx = G.vhi[0] - G.vlo[0] + 1, ny = G.vhi[1] - G.vlo[1] + 1;
+  const int64_t cells = (int64_t)nx * ny * (G.vhi[2] - G.vlo[2] + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (t < cells) {
+    const int i = G.vlo[0] + (int)(t % nx), j = G.vlo[1] + (int)((t / nx) % ny);
+    const int k = G.vlo[2] + (int)(t / ((int64_t)nx * ny));
+    v = F.at(i, j, k, comp);
+  }
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) partial[blockIdx.x] = ((lds[0] + lds[1]) + lds[2]) + lds[3];
+}
+
+static BoxGeom box_geom(const ilbm_case_desc& c, const int* vlo, const int* vhi) {
+  BoxGeom G{};
+  G.dim = c.dim;
+  for (int d = 0; d < 3; ++d) {
+    G.n[d] = (d < c.dim) ? c.n[d] : 1;
+    G.periodic[d] = (d < c.dim) ? c.periodic[d] : 1;
+    G.lo_t[d] = c.bc_lo[d];
+    G.hi_t[d] = c.bc_hi[d];
+    for (int k = 0; k < 4; ++k) { G.lo_v[d][k] = c.bc_lo_val[d][k]; G.hi_v[d][k] = c.bc_hi_val[d][k]; }
+    G.vlo[d] = vlo ? vlo[d] : 0;
+    G.vhi[d] = vhi ? vhi[d] : (G.n[d] - 1);
+    if (d >= c.dim) { G.vlo[d] = 0; G.vhi[d] = 0; }
+  }
+  return G;
+}
+
+static int box_lattice(const ilbm_case_desc& c) {
+  if (lattice_matches<D2Q9>(c)) return 0;
+  if (lattice_matches<D3Q19>(c)) return 1;
+  if (lattice_matches<D3Q27>(c)) return 2;
+  throw StatusError{ILBM_ERR_UNSUPPORTED, "lattice table is not D2Q9/D3Q19/D3Q27 in the reference order"};
+}
+
+template <class F>
+static void with_lattice(int lat, F&& f) {
+  if (lat == 0) f(D2Q9{});
+  else if (lat == 1) f(D3Q19{});
+  else f(D3Q27{});
+}
+
+static dim3 grid_for(int64_t n) { return dim3((unsigned)((n + 255) / 256)); }
+
+static int64_t box_cells(const BoxGeom& G) {
+  return (int64_t)(G.vhi[0] - G.vlo[0] + 1) * (G.vhi[1] - G.vlo[1] + 1) * (G.vhi[2] - G.vlo[2] + 1);
+}
+
+static void check_fab(const ilbm_fab* f, int min_comp, const BoxGeom& G, int ghosts) {
+  if (!f || !f->p) throw StatusError{ILBM_ERR_INVALID, "null fab"};
+  if (f->ncomp < min_comp) throw StatusError{ILBM_ERR_INVALID, "fab has too few components"};
+  for (int d = 0; d < 3; ++d) {
+    const int g = (d < G.dim) ? ghosts : 0;
+    if (f->lo[d] > G.vlo[d] - g || f->hi[d] < G.vhi[d] + g)
+      throw StatusError{ILBM_ERR_INVALID, "fab does not cover the box and the ghost cells the operation reads"};
+  }
+}
+
+}  // namespace ilbm
+
+extern "C" {
+
+int ilbm_box_equilibrium(const ilbm_case_desc* d, const ilbm_fab* macro, ilbm_fab* f, const int lo[3],
+                         const int hi[3], void* stream) {
+  if (!d || !lo || !hi) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, lo, hi);
+    check_fab(macro, 4, G, 0);
+    check_fab(f, d->q, G, 0);
+    const int64_t cells = box_cells(G);
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_equilibrium<LAT>, grid_for(cells), dim3(256), 0, (hipStream_t)stream,
+                         to_fab(*macro), to_fab(*f), G.vlo[0], G.vlo[1], G.vlo[2], G.vhi[0] - G.vlo[0] + 1,
+                         G.vhi[1] - G.vlo[1] + 1, cells);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_fill_periodic(const ilbm_case_desc* d, ilbm_fab* fab, const int vlo[3], const int vhi[3],
+                           void* stream) {
+  if (!d || !fab || !fab->p) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, vlo, vhi);
+    const Fab F = to_fab(*fab);
+    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_fill_physical(const ilbm_case_desc* d, ilbm_fab* fab, int mesoscopic, void* stream) {
+  if (!d || !fab || !fab->p) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, nullptr, nullptr);
+    const Fab F = to_fab(*fab);
+    if (mesoscopic && F.nc < d->q) throw StatusError{ILBM_ERR_INVALID, "mesoscopic fab needs q components"};
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      for (int pass = 1; pass <= d->dim; ++pass)
+        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G,
+                           pass, mesoscopic ? 1 : 0);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_collide(const ilbm_case_desc* d, ilbm_fab* f, const ilbm_fab* macro, const ilbm_fab* force,
+                     const int vlo[3], const int vhi[3], void* stream) {
+  if (!d || !vlo || !vhi) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, vlo, vhi);
+    check_fab(f, d->q, G, 0);
+    check_fab(macro, 4, G, 0);
+    Fab Fo{};
+    if (force) { check_fab(force, 3, G, 0); Fo = to_fab(*force); }
+    const double tau = 3.0 * d->nu / d->dx[0] + 0.5;  // AmrCoreLBM.cpp:225-238
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_collide<LAT>, grid_for(box_cells(G)), dim3(256), 0, (hipStream_t)stream,
+                         to_fab(*f), to_fab(*macro), Fo, tau, G);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_bounce_back(const ilbm_case_desc* d, ilbm_fab* f, const int vlo[3], const int vhi[3], void* stream) {
+  if (!d || !vlo || !vhi) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, vlo, vhi);
+    check_fab(f, d->q, G, 1);
+    const Fab F = to_fab(*f);
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_bounce_back<LAT>, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_stream_macro(const ilbm_case_desc* d, const ilbm_fab* f_in, ilbm_fab* f_out, ilbm_fab* macro_out,
+                          const ilbm_fab* force, const int vlo[3], const int vhi[3], void* stream) {
+  if (!d || !vlo || !vhi) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, vlo, vhi);
+    check_fab(f_in, d->q, G, 1);
+    check_fab(f_out, d->q, G, 0);
+    check_fab(macro_out, 4, G, 0);
+    Fab Fo{};
+    if (force) { check_fab(force, 3, G, 0); Fo = to_fab(*force); }
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_stream_macro<LAT>, grid_for(box_cells(G)), dim3(256), 0, (hipStream_t)stream,
+                         to_fab(*f_in), to_fab(*f_out), to_fab(*macro_out), Fo, G);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_derived(const ilbm_case_desc* d, const ilbm_fab* macro_patch, ilbm_fab* macro_out, const int vlo[3],
+                     const int vhi[3], void* stream) {
+  if (!d || !vlo || !vhi) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, vlo, vhi);
+    check_fab(macro_patch, 4, G, 1);
+    check_fab(macro_out, 5, G, 0);
+    hipLaunchKernelGGL(k_box_derived, grid_for(box_cells(G)), dim3(256), 0, (hipStream_t)stream,
+                       to_fab(*macro_patch), to_fab(*macro_out), G, d->dx[0], d->dx[1], d->dx[2]);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_sum(const ilbm_fab* fab, int comp, const int vlo[3], const int vhi[3], double* host_out,
+                 void* stream) {
+  if (!fab || !fab->p || !vlo || !vhi || !host_out || comp < 0 || comp >= fab->ncomp) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    BoxGeom G{};
+    G.dim = 3;
+    for (int dd = 0; dd < 3; ++dd) { G.vlo[dd] = vlo[dd]; G.vhi[dd] = vhi[dd]; }
+    const Fab F = to_fab(*fab);
+    if (!F.contains(vlo[0], vlo[1], vlo[2]) || !F.contains(vhi[0], vhi[1], vhi[2]))
+      throw StatusError{ILBM_ERR_INVALID, "box outside the fab"};
+    const int64_t cells = box_cells(G);
+    const int64_t nb = (cells + 255) / 256;
+    double* part = dev_alloc<double>(nb + 1);
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_box_sum, dim3((unsigned)nb), dim3(256), 0, s, F, comp, G, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, (const double*)part, nb, part + nb);
+    const hipError_t e1 = hipMemcpyAsync(host_out, part + nb, sizeof(double), hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    dev_free(part);
+    HIPCHK(e1);
+    HIPCHK(e2);
+  });
+}
+
+}  // extern "C"

@@ -483,6 +483,62 @@ __device__ __forceinline__ void spread_body(const IbArgs& a, int blk) {
 
 __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread(IbArgs a) { spread_body(a, blockIdx.x); }
 
+// Spreading for a moving body, whose stencils change every step: every
+// band cell gathers the contributions of all markers whose 5-point stencil
+// covers it, in marker order -- the order of the serial scatter
+// (IBMarkerDF.cpp:1405-1512).  The band does not cross a periodic boundary.
+__global__ void __launch_bounds__(256) k_ib_spread_all(IbArgs a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.band_cells) return;
+  const int nm = a.nm;
+  const int i = a.band_lo[0] + (int)(b % a.band_n[0]);
+  const int j = a.band_lo[1] + (int)((b / a.band_n[0]) % a.band_n[1]);
+  const int k = a.band_lo[2] + (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int m = 0; m < nm; ++m) {
+    const int ii = i - a.base[m] + 2, jj = j - a.base[nm + m] + 2;
+    const int kk = (a.dim == 3) ? k - a.base[2 * nm + m] + 2 : 0;
+    if ((unsigned)ii > 4u || (unsigned)jj > 4u || (unsigned)kk > 4u) continue;
+    const double tU = a.wts[(0 * 5 + ii) * nm + m];
+    const double tV = a.wts[(1 * 5 + jj) * nm + m];
+    const double sc = a.scale[nm + m];
+    const double dval = (a.dim == 3) ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
+    s0 += dval * (a.F[m] * a.dv);
+    s1 += dval * (a.F[nm + m] * a.dv);
+    if (a.dim == 3) s2 += dval * (a.F[2 * nm + m] * a.dv);
+  }
+  a.EFI[b] = s0;
+  a.EFI[a.band_cells + b] = s1;
+  a.EFI[2 * a.band_cells + b] = s2;
+}
+
+// Positions and target velocities of the pitching section at one time
+// (marker_state of Examples/Pitching_airfoil/IBMUserDefinedGeometry.H:144-175,
+// with the pivot offset along the chord as a parameter).  The angle terms
+// (cos, sin of the pitch angle and the angular rate) come from the host.
+struct MoveArgs {
+  const double* xref;   // [2][nm] section outline in body coordinates
+  double* xm;           // [3][nm]
+  double* ut;           // [3][nm]
+  int nm;
+  double px, py, pivot_x, cs, sn, omega, z;
+};
+
+__global__ void k_ib_move(MoveArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nm) return;
+  const int nm = a.nm;
+  const double xrr = a.xref[m] - a.pivot_x, yr = a.xref[nm + m];
+  const double x = a.px + a.cs * xrr - a.sn * yr;
+  const double y = a.py + a.sn * xrr + a.cs * yr;
+  a.xm[m] = x;
+  a.xm[nm + m] = y;
+  a.xm[2 * nm + m] = a.z;
+  a.ut[m] = -a.omega * (y - a.py);
+  a.ut[nm + m] = a.omega * (x - a.px);
+  a.ut[2 * nm + m] = 0.0;
+}
+
 // Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
 // iterations) and the moments r x (F dv) (:1500-1503), with their sum as
 // the body torque.  One block, fixed reduction tree.

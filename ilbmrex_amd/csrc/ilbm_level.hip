@@ -392,6 +392,8 @@ struct Level {
   double* frstate_d = nullptr;
   std::vector<double> xm_h, ut_h;
   bool moving = false;
+  bool dev_kin = false;          // moving body advanced on the device (explicit forcing)
+  double* xref_d = nullptr;      // [2][nm] section outline in body coordinates
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev_start, ev_stop;
@@ -408,7 +410,7 @@ struct Level {
     dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
-    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d);
+    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d); dev_free(xref_d);
     for (int b = 0; b < 2; ++b) { dev_free(Mw[b]); dev_free(Dw[b]); }
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -541,6 +543,7 @@ struct Level {
     }
     nm = mg.nm;
     moving = (c.geometry_type == 2);
+    dev_kin = moving && c.coupling_method != 1;
     for (int d = 0; d < c.dim; ++d)
       if (c.periodic[d] && c.n[d] < 5) throw StatusError{ILBM_ERR_UNSUPPORTED, "periodic extent < 5 with IB"};
     // band: bounding box of the 5-point stencils (over the motion for a moving body)
@@ -575,6 +578,16 @@ struct Level {
       band_n[d] = h - l + 1;
     }
     band_cells = (int64_t)band_n[0] * band_n[1] * band_n[2];
+    if (dev_kin) {
+      for (int d = 0; d < c.dim; ++d)
+        if (c.periodic[d] && (lo[d] < 0 || hi[d] > c.n[d] - 1))
+          throw StatusError{ILBM_ERR_UNSUPPORTED, "moving body across a periodic boundary"};
+      std::vector<double> xr(2 * (size_t)nm);
+      for (int m = 0; m < nm; ++m) { xr[m] = mg.xref[m + 1]; xr[nm + m] = mg.yref[m + 1]; }
+      xref_d = dev_alloc<double>(2 * nm);
+      HIPCHK(hipMemcpyAsync(xref_d, xr.data(), sizeof(double) * 2 * nm, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipStreamSynchronize(stream));
+    }
     Mband = dev_alloc<double>(4 * band_cells);
     Fband = dev_alloc<double>(3 * band_cells);
     EV = dev_alloc<double>(3 * band_cells);
@@ -609,7 +622,7 @@ struct Level {
   void upload_markers() {
     HIPCHK(hipMemcpyAsync(xm_d, xm_h.data(), sizeof(double) * 3 * nm, hipMemcpyHostToDevice, stream));
     HIPCHK(hipMemcpyAsync(ut_d, ut_h.data(), sizeof(double) * 3 * nm, hipMemcpyHostToDevice, stream));
-    build_spread_lists();
+    if (!dev_kin) build_spread_lists();
     if (c.coupling_method == 1) build_ivc();
     // delta weights depend on positions only: once for a fixed body, every
     // step for a moving one
@@ -789,7 +802,22 @@ struct Level {
   // with the export of F fused with the Guo term of the coming collision and
   // with the ComputeIBForce record of the step (row `row` of the history).
   void ib_update_forcing(double time, int row) {
-    if (moving) {
+    if (dev_kin) {
+      // the pitching kinematics on the device; the angle terms as
+      // marker_positions computes them
+      const double twopi = 2.0 * kPi;
+      const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
+      const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
+      MoveArgs mv{};
+      mv.xref = xref_d; mv.xm = xm_d; mv.ut = ut_d; mv.nm = nm;
+      mv.omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
+      mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
+      mv.cs = std::cos(angle); mv.sn = std::sin(angle);
+      mv.z = 0.0;
+      hipLaunchKernelGGL(k_ib_move, dim3((nm + 255) / 256), dim3(256), 0, stream, mv);
+      const IbArgs aw = ib_args();
+      hipLaunchKernelGGL(k_ib_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, aw);
+    } else if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
     }
@@ -807,7 +835,9 @@ struct Level {
         hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
         hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
       }
-      if (n_ne)
+      if (dev_kin)
+        hipLaunchKernelGGL(k_ib_spread_all, dim3(gb), dim3(tb), 0, stream, a);
+      else if (n_ne)
         hipLaunchKernelGGL(k_ib_spread, dim3((n_ne + kSpreadWaves - 1) / kSpreadWaves), dim3(64 * kSpreadWaves), 0,
                            stream, a);
       hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
@@ -1241,3 +1271,4 @@ int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
 
 #include "ilbm_batch_impl.h"
 #include "ilbm_amr_impl.h"
+#include "ilbm_box_impl.h"

@@ -63,9 +63,16 @@ EXPORTED = [
     "ilbm_amr_create", "ilbm_amr_destroy", "ilbm_amr_advance", "ilbm_amr_level_cells",
     "ilbm_amr_get_f", "ilbm_amr_get_macro", "ilbm_amr_mass_history", "ilbm_amr_force_history",
     "ilbm_amr_num_markers", "ilbm_amr_info",
+    "ilbm_box_equilibrium", "ilbm_box_fill_periodic", "ilbm_box_fill_physical", "ilbm_box_collide",
+    "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
 ]
 
 ERR_FULL = 6
+
+class Fab(C.Structure):
+    """ilbm_fab (include/ilbm.h): an Array4 of FP64 in device memory."""
+    _fields_ = [("p", C.c_void_p), ("lo", I3), ("hi", I3), ("ncomp", C.c_int)]
+
 
 _lib = None
 
@@ -123,6 +130,15 @@ def load():
     L.ilbm_amr_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
     L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
     L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
+    PF = P(Fab)
+    L.ilbm_box_equilibrium.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_fill_periodic.argtypes = [P(CaseDesc), PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_fill_physical.argtypes = [P(CaseDesc), PF, C.c_int, vp]
+    L.ilbm_box_collide.argtypes = [P(CaseDesc), PF, PF, PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_bounce_back.argtypes = [P(CaseDesc), PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_stream_macro.argtypes = [P(CaseDesc), PF, PF, PF, PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_derived.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_sum.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
     L.ilbm_status_string.restype = C.c_char_p
     L.ilbm_status_string.argtypes = [C.c_int]
     L.ilbm_last_error.restype = C.c_char_p

@@ -208,6 +208,54 @@ int ilbm_amr_num_markers(ilbm_amr* A, int* out);
  * [6] tau of the finest level, [7] tau of level 0 */
 int ilbm_amr_info(ilbm_amr* A, double out[8]);
 
+/* ------------------------------------------------------------------------
+ * Per-box entry points on caller-owned device memory in the reference's
+ * layout: an AMReX Array4 of FP64 (one contiguous array per component over
+ * the box grown by its ghost cells, x fastest; AmrCoreLBM.H:276).  One call
+ * per MFIter box of AdvancePhiAtLevel, in the reference's order; AMReX's
+ * ParallelCopy / FillBoundary between boxes stay with the caller
+ * (ilbm_box_fill_periodic covers a level of one box).  The domain is
+ * [0, n-1] of desc (the level's cell counts); tau = 3 nu / dx + 1/2.
+ * `stream` is a hipStream_t (NULL: the default stream).
+ * ------------------------------------------------------------------------ */
+typedef struct {
+  double* p;          /* device pointer to component 0 at cell lo */
+  int lo[3], hi[3];   /* fab box (valid box grown by the ghost cells) */
+  int ncomp;
+} ilbm_fab;
+
+/* InitEquilibrium (AmrCoreLBM.cpp:2117-2164): f = feq(rho, u) on [lo, hi] */
+int ilbm_box_equilibrium(const ilbm_case_desc* desc, const ilbm_fab* macro, ilbm_fab* f,
+                         const int lo[3], const int hi[3], void* stream);
+/* FillBoundary with the domain's periodicity for a level of one box */
+int ilbm_box_fill_periodic(const ilbm_case_desc* desc, ilbm_fab* fab, const int valid_lo[3],
+                           const int valid_hi[3], void* stream);
+/* PhysBCFunct<GpuBndryFuncFab<mesoscopicBcFill / macroBcFill>> on every fab
+ * cell outside the domain: FilccCell foextrap, then the user functor
+ * (Source/BC/bc_fill.H:263-347), faces before edges before corners */
+int ilbm_box_fill_physical(const ilbm_case_desc* desc, ilbm_fab* fab, int mesoscopic, void* stream);
+/* collide / collide_forced on the valid box, in place on f, with feq of
+ * macro (rho, ux, uy, uz) and the force fab (NULL: no force)
+ * (Source/LBM/collide_forced.H:10-51, AdvanceLBMAtLevel.cpp:196-240) */
+int ilbm_box_collide(const ilbm_case_desc* desc, ilbm_fab* f, const ilbm_fab* macro,
+                     const ilbm_fab* force, const int valid_lo[3], const int valid_hi[3], void* stream);
+/* halfway bounce-back into the user_1 ghost cells (AdvanceLBMAtLevel.cpp:265-373) */
+int ilbm_box_bounce_back(const ilbm_case_desc* desc, ilbm_fab* f, const int valid_lo[3],
+                         const int valid_hi[3], void* stream);
+/* stream + calculateMacroForcing on the valid box (stream.H:9-22,
+ * calculateMacroForcing.H:9-32) */
+int ilbm_box_stream_macro(const ilbm_case_desc* desc, const ilbm_fab* f_in, ilbm_fab* f_out,
+                          ilbm_fab* macro_out, const ilbm_fab* force, const int valid_lo[3],
+                          const int valid_hi[3], void* stream);
+/* visPara (Source/LBM/visPara.H:10-54): vorticity into comp 4 and pressure
+ * into comp 5 of macro_out from a macro patch with one ghost cell */
+int ilbm_box_derived(const ilbm_case_desc* desc, const ilbm_fab* macro_patch, ilbm_fab* macro_out,
+                     const int valid_lo[3], const int valid_hi[3], void* stream);
+/* sum of one component over a box (MultiFab::sum of one box; Sum(Rho),
+ * the force integral of ComputeIBForce), fixed order; synchronizes */
+int ilbm_box_sum(const ilbm_fab* fab, int comp, const int lo[3], const int hi[3], double* host_out,
+                 void* stream);
+
 const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */
 const char* ilbm_last_error(void);

@@ -89,3 +89,26 @@ def test_two_level_fp32_cd_close_to_fp64():
         h.close()
     assert np.isfinite(res[cf.STORE_F32])
     assert abs(res[cf.STORE_F32] - res[cf.STORE_F64]) <= 1e-3 * abs(res[cf.STORE_F64])
+
+
+def test_two_level_pitching_section_bitexact():
+    """Moving body on the finest level: device kinematics and gather
+    spreading against the oracle's host kinematics and serial scatter."""
+    c = cf.cylinder_2d(nx=64, ny=48, D=10.0, xc=20.0, yc=24.0, U=0.03, Re=100.0).replace(
+        geometry_type=cf.GEOM_USER, airfoil_chord=10.0, n_marker=96, pitch_amplitude=0.2,
+        pitch_frequency=0.01, pivot_x=2.5, force_eval_method=cf.FORCE_EVAL_MARKER, name="pitching_amr")
+    gpu, orc = run_pair(c, [((28, 28, 0), (75, 67, 0))], 8)
+    check_levels(gpu, orc)
+    check_record(gpu, orc)
+
+
+def test_pitching_naca_three_levels_runs():
+    from ilbmrex_amd.native import AmrHierarchy
+    c, boxes = cf.pitching_naca_amr()
+    h = AmrHierarchy(c, boxes, 10, (1.0, c.U0, 0.0, 0.0))
+    assert h.num_markers() == 256
+    h.advance(20)
+    fh = h.force_history()
+    assert fh.shape == (20, 23) and np.isfinite(fh).all()
+    info = h.info()
+    assert info["cells"] == [512 * 256, 128 * 96, 160 * 128]
