@@ -280,6 +280,41 @@ struct Batch {
     return cases.back().id;
   }
 
+  // a case admitted from a checkpoint of the same case description
+  // (preempted and resumed); nsteps more steps
+  int add_from_checkpoint(const ilbm_case_desc& desc, const char* buf, int64_t size, int nsteps) {
+    if (nsteps <= 0) throw StatusError{ILBM_ERR_INVALID, "nsteps must be positive"};
+    if (desc.use_ib && desc.geometry_type == 2)
+      throw StatusError{ILBM_ERR_UNSUPPORTED, "moving bodies are not batched"};
+    const int64_t bytes = case_footprint(desc, nsteps);
+    if ((int)cases.size() >= max_cases || used + bytes > budget)
+      throw StatusError{ILBM_ERR_FULL, "batch full (case slots or HBM budget)"};
+    BatchCase bc;
+    bc.L.reset(new Level());
+    Level& L = *bc.L;
+    L.c = desc;
+    L.stream = stream;
+    L.own_stream = false;
+    L.validate();
+    L.setup_layout();
+    L.setup_ib();
+    std::vector<double> m0(4 * (size_t)L.ncells, 0.0);
+    std::fill(m0.begin(), m0.begin() + L.ncells, 1.0);
+    L.init(m0.data());
+    L.restore(buf, size);
+    bc.tile_bx = tile_for(L);
+    L.ensure_partial(L.tile_blocks(bc.tile_bx));
+    L.grow_histories(nsteps, nsteps);
+    HIPCHK(hipStreamSynchronize(stream));
+    bc.id = next_id++;
+    bc.target = nsteps;
+    bc.bytes = bytes;
+    used += bytes;
+    cases.push_back(std::move(bc));
+    dirty = true;
+    return cases.back().id;
+  }
+
   void remove(int id) {
     for (size_t i = 0; i < cases.size(); ++i) {
       if (cases[i].id != id) continue;
@@ -621,6 +656,33 @@ int ilbm_batch_mass_history(ilbm_batch* h, int case_id, double* out, int max_row
   ilbm::BatchCase* c = h->B.find(case_id);
   if (!c) return ILBM_ERR_INVALID;
   return mass_history_of(*c->L, out, max_rows, n_out);
+}
+
+int ilbm_batch_checkpoint_size(ilbm_batch* h, int case_id, int64_t* bytes) {
+  if (!h || !bytes) return ILBM_ERR_INVALID;
+  ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  *bytes = c->L->checkpoint_size();
+  return ILBM_OK;
+}
+
+int ilbm_batch_checkpoint(ilbm_batch* h, int case_id, void* host_buf, int64_t bytes) {
+  if (!h || !host_buf) return ILBM_ERR_INVALID;
+  ilbm::BatchCase* c = h->B.find(case_id);
+  if (!c) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    HIPCHK(hipStreamSynchronize(h->B.stream));
+    c->L->checkpoint(static_cast<char*>(host_buf), bytes);
+  });
+}
+
+int ilbm_batch_add_from_checkpoint(ilbm_batch* h, const ilbm_case_desc* desc, const void* host_buf, int64_t bytes,
+                                   int nsteps, int* case_id) {
+  if (!h || !desc || !host_buf || !case_id) return ILBM_ERR_INVALID;
+  *case_id = -1;
+  return guarded([&] {
+    *case_id = h->B.add_from_checkpoint(*desc, static_cast<const char*>(host_buf), bytes, nsteps);
+  });
 }
 
 int ilbm_batch_remove(ilbm_batch* h, int case_id) {

@@ -899,6 +899,107 @@ struct Level {
     if (!fout) fout = dev_alloc<double>((size_t)c.q * ncells);
   }
 
+  // ---------------- checkpoint / restart ----------------
+  // The dynamic state a step reads: the current populations with their
+  // ghost slabs, the pre-collision boundary layer, the IB band macros and
+  // exported force, the time-centred force state, and the clock.  Marker
+  // geometry, IVC operator and layout are rebuilt from the case.
+  struct CkptHeader {
+    char magic[8];
+    int abi, q, storage, ib;
+    int64_t pop_bytes, face_bytes, band_cells;
+    double t_new, t_old;
+    int istep, pad;
+  };
+
+  int64_t face_bytes_total() const {
+    int64_t b = 0;
+    for (int d = 0; d < c.dim; ++d) {
+      if (!nonper[d]) continue;
+      const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
+                                                                     : (int64_t)c.n[0] * c.n[1];
+      b += 2 * (int64_t)popbytes * fn * c.q;
+    }
+    return b;
+  }
+
+  int64_t pop_bytes_total() const {
+    return (int64_t)popbytes * ((int64_t)g.qs * c.q + slabs.face_off[slabs.nface] * c.q);
+  }
+
+  int64_t checkpoint_size() const {
+    int64_t b = (int64_t)sizeof(CkptHeader) + pop_bytes_total() + face_bytes_total();
+    if (ib) b += (int64_t)sizeof(double) * (7 * band_cells + 4);
+    return b;
+  }
+
+  void checkpoint(char* out, int64_t size) {
+    if (size < checkpoint_size()) throw StatusError{ILBM_ERR_INVALID, "checkpoint buffer too small"};
+    HIPCHK(hipStreamSynchronize(stream));
+    CkptHeader h{};
+    std::memcpy(h.magic, "ILBMCKP1", 8);
+    h.abi = ILBM_ABI_VERSION; h.q = c.q; h.storage = st; h.ib = ib ? 1 : 0;
+    h.pop_bytes = pop_bytes_total(); h.face_bytes = face_bytes_total(); h.band_cells = ib ? band_cells : 0;
+    h.t_new = t_new; h.t_old = t_old; h.istep = istep;
+    std::memcpy(out, &h, sizeof(h));
+    char* p = out + sizeof(h);
+    HIPCHK(hipMemcpy(p, gbuf[cur], h.pop_bytes, hipMemcpyDeviceToHost));
+    p += h.pop_bytes;
+    for (int f = 0; f < 6; ++f) {
+      if (!faces.p[f]) continue;
+      const int d = f / 2;
+      const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
+                                                                     : (int64_t)c.n[0] * c.n[1];
+      const int64_t b = (int64_t)popbytes * fn * c.q;
+      HIPCHK(hipMemcpy(p, faces.p[f], b, hipMemcpyDeviceToHost));
+      p += b;
+    }
+    if (ib) {
+      HIPCHK(hipMemcpy(p, Mband, sizeof(double) * 4 * band_cells, hipMemcpyDeviceToHost));
+      p += sizeof(double) * 4 * band_cells;
+      HIPCHK(hipMemcpy(p, Fband, sizeof(double) * 3 * band_cells, hipMemcpyDeviceToHost));
+      p += sizeof(double) * 3 * band_cells;
+      HIPCHK(hipMemcpy(p, frstate_d, sizeof(double) * 4, hipMemcpyDeviceToHost));
+    }
+  }
+
+  void restore(const char* in, int64_t size) {
+    if (size < (int64_t)sizeof(CkptHeader)) throw StatusError{ILBM_ERR_INVALID, "checkpoint too small"};
+    CkptHeader h;
+    std::memcpy(&h, in, sizeof(h));
+    if (std::memcmp(h.magic, "ILBMCKP1", 8) != 0 || h.abi != ILBM_ABI_VERSION || h.q != c.q || h.storage != st ||
+        h.ib != (ib ? 1 : 0) || h.pop_bytes != pop_bytes_total() || h.face_bytes != face_bytes_total() ||
+        h.band_cells != (ib ? band_cells : 0) || size < checkpoint_size())
+      throw StatusError{ILBM_ERR_INVALID, "checkpoint does not match this case"};
+    HIPCHK(hipStreamSynchronize(stream));
+    const char* p = in + sizeof(h);
+    cur = 0;
+    HIPCHK(hipMemcpy(gbuf[cur], p, h.pop_bytes, hipMemcpyHostToDevice));
+    p += h.pop_bytes;
+    for (int f = 0; f < 6; ++f) {
+      if (!faces.p[f]) continue;
+      const int d = f / 2;
+      const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
+                                                                     : (int64_t)c.n[0] * c.n[1];
+      const int64_t b = (int64_t)popbytes * fn * c.q;
+      HIPCHK(hipMemcpy(faces.p[f], p, b, hipMemcpyHostToDevice));
+      p += b;
+    }
+    if (ib) {
+      HIPCHK(hipMemcpy(Mband, p, sizeof(double) * 4 * band_cells, hipMemcpyHostToDevice));
+      p += sizeof(double) * 4 * band_cells;
+      HIPCHK(hipMemcpy(Fband, p, sizeof(double) * 3 * band_cells, hipMemcpyHostToDevice));
+      p += sizeof(double) * 3 * band_cells;
+      HIPCHK(hipMemcpy(frstate_d, p, sizeof(double) * 4, hipMemcpyHostToDevice));
+    }
+    t_new = h.t_new; t_old = h.t_old; istep = h.istep;
+    const double clk[4] = {t_new, (double)istep, (double)istep, 0.0};
+    HIPCHK(hipMemcpy(clock_d, clk, sizeof(clk), hipMemcpyHostToDevice));
+    state_valid = false;
+    mass_host.clear();
+    force_host.clear();
+  }
+
   void init(const double* macro_host) {
     grow_histories(1, 1);
     double* m0 = dev_alloc<double>(4 * ncells);
@@ -1205,6 +1306,22 @@ int ilbm_level_marker_torque(ilbm_level* h, double out[3]) {
     HIPCHK(hipMemcpy(s, h->L.sums_d, sizeof(double) * 8, hipMemcpyDeviceToHost));
     out[0] = s[3]; out[1] = s[4]; out[2] = s[5];
   });
+}
+
+int ilbm_level_checkpoint_size(ilbm_level* h, int64_t* bytes) {
+  if (!h || !bytes) return ILBM_ERR_INVALID;
+  *bytes = h->L.checkpoint_size();
+  return ILBM_OK;
+}
+
+int ilbm_level_checkpoint(ilbm_level* h, void* host_buf, int64_t bytes) {
+  if (!h || !host_buf) return ILBM_ERR_INVALID;
+  return guarded([&] { h->L.checkpoint(static_cast<char*>(host_buf), bytes); });
+}
+
+int ilbm_level_restore(ilbm_level* h, const void* host_buf, int64_t bytes) {
+  if (!h || !host_buf) return ILBM_ERR_INVALID;
+  return guarded([&] { h->L.restore(static_cast<const char*>(host_buf), bytes); });
 }
 
 int ilbm_level_info(ilbm_level* h, double out[8]) {

@@ -65,6 +65,8 @@ EXPORTED = [
     "ilbm_amr_num_markers", "ilbm_amr_info",
     "ilbm_box_equilibrium", "ilbm_box_fill_periodic", "ilbm_box_fill_physical", "ilbm_box_collide",
     "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
+    "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore",
+    "ilbm_batch_checkpoint_size", "ilbm_batch_checkpoint", "ilbm_batch_add_from_checkpoint",
 ]
 
 ERR_FULL = 6
@@ -130,6 +132,12 @@ def load():
     L.ilbm_amr_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
     L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
     L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_checkpoint_size.argtypes = [vp, P(C.c_int64)]
+    L.ilbm_level_checkpoint.argtypes = [vp, C.c_void_p, C.c_int64]
+    L.ilbm_level_restore.argtypes = [vp, C.c_void_p, C.c_int64]
+    L.ilbm_batch_checkpoint_size.argtypes = [vp, C.c_int, P(C.c_int64)]
+    L.ilbm_batch_checkpoint.argtypes = [vp, C.c_int, C.c_void_p, C.c_int64]
+    L.ilbm_batch_add_from_checkpoint.argtypes = [vp, P(CaseDesc), C.c_void_p, C.c_int64, C.c_int, P(C.c_int)]
     PF = P(Fab)
     L.ilbm_box_equilibrium.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), vp]
     L.ilbm_box_fill_periodic.argtypes = [P(CaseDesc), PF, P(C.c_int), P(C.c_int), vp]
@@ -305,6 +313,19 @@ class Level:
     def set_timing(self, enable: bool):
         _check(load().ilbm_level_set_timing(self._h, int(enable)), "set_timing")
 
+    def checkpoint(self) -> bytes:
+        """The level's dynamic state (ilbm_level_checkpoint)."""
+        L = load()
+        n = C.c_int64(0)
+        _check(L.ilbm_level_checkpoint_size(self._h, C.byref(n)), "checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        _check(L.ilbm_level_checkpoint(self._h, buf, n.value), "checkpoint")
+        return buf.raw
+
+    def restore(self, data: bytes):
+        buf = C.create_string_buffer(data, len(data))
+        _check(load().ilbm_level_restore(self._h, buf, len(data)), "restore")
+
     def kernel_time(self):
         ms, n = C.c_double(0), C.c_int(0)
         _check(load().ilbm_level_kernel_time(self._h, C.byref(ms), C.byref(n)), "kernel_time")
@@ -402,6 +423,25 @@ class Batch:
         out = np.empty(max(n.value, 1))
         _check(L.ilbm_batch_mass_history(self._h, int(cid), _dp(out), n.value, C.byref(n)), "mass_history")
         return out[:n.value]
+
+    def checkpoint(self, cid: int) -> bytes:
+        L = load()
+        n = C.c_int64(0)
+        _check(L.ilbm_batch_checkpoint_size(self._h, int(cid), C.byref(n)), "checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        _check(L.ilbm_batch_checkpoint(self._h, int(cid), buf, n.value), "checkpoint")
+        return buf.raw
+
+    def add_from_checkpoint(self, case: "cf.Case", data: bytes, nsteps: int) -> int:
+        d = to_desc(case)
+        buf = C.create_string_buffer(data, len(data))
+        cid = C.c_int(-1)
+        rc = load().ilbm_batch_add_from_checkpoint(self._h, C.byref(d), buf, len(data), int(nsteps), C.byref(cid))
+        if rc == ERR_FULL:
+            raise BatchFull("batch full")
+        _check(rc, "add_from_checkpoint")
+        self._cases[cid.value] = case
+        return cid.value
 
     def remove(self, cid: int):
         _check(load().ilbm_batch_remove(self._h, int(cid)), "remove")

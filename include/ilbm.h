@@ -120,6 +120,16 @@ int ilbm_level_marker_state(ilbm_level* L, double* xyz, double* uvw, double* fxy
 int ilbm_level_marker_force_sum(ilbm_level* L, double out[3]);
 int ilbm_level_marker_torque(ilbm_level* L, double out[3]);
 
+/* Checkpoint / restart of a level's dynamic state (populations, ghost and
+ * boundary layers, IB band macros and force, time-centred force state,
+ * clock) into a host buffer, restored into a level created from the same
+ * case description.  The reference's checkpoint writer is stubbed out
+ * (AmrCoreLBM.cpp:1932-2115); this is a compact per-case dump for
+ * preemption.  Histories restart empty after a restore. */
+int ilbm_level_checkpoint_size(ilbm_level* L, int64_t* bytes);
+int ilbm_level_checkpoint(ilbm_level* L, void* host_buf, int64_t bytes);
+int ilbm_level_restore(ilbm_level* L, const void* host_buf, int64_t bytes);
+
 /* Level facts: [0] tau, [1] dt, [2] time, [3] steps, [4] band cells,
  * [5] storage bytes per population, [6] padded (1) or wrapped (0) layout. */
 int ilbm_level_info(ilbm_level* L, double out[8]);
@@ -166,6 +176,12 @@ int ilbm_batch_force_history(ilbm_batch* B, int case_id, double* host_out, int m
 int ilbm_batch_mass_history(ilbm_batch* B, int case_id, double* host_out, int max_rows,
                             int* n_out);
 int ilbm_batch_remove(ilbm_batch* B, int case_id);
+/* Preemption: dump a resident case, and admit a case from such a dump
+ * (same case description) for nsteps more steps. */
+int ilbm_batch_checkpoint_size(ilbm_batch* B, int case_id, int64_t* bytes);
+int ilbm_batch_checkpoint(ilbm_batch* B, int case_id, void* host_buf, int64_t bytes);
+int ilbm_batch_add_from_checkpoint(ilbm_batch* B, const ilbm_case_desc* desc, const void* host_buf,
+                                   int64_t bytes, int nsteps, int* case_id);
 /* Tile width (cells along x per workgroup) the autotuner chose for a case. */
 int ilbm_batch_case_tile(ilbm_batch* B, int case_id, int* tile_bx);
 /* [0] resident cases, [1] unfinished cases, [2] bytes charged, [3] budget,
