@@ -8,6 +8,7 @@ import re
 import subprocess
 import sys
 
+import numpy as np
 import pytest
 
 from ilbmrex_amd import config as cf
@@ -116,3 +117,51 @@ def test_bench_refuses_several_gpus():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"],
                        capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_footprint_and_argument_checks_without_gpu():
+    """Entry points that reject their arguments on the host return a status
+    before any device work."""
+    from ilbmrex_amd import native
+    lib = native.load()
+    c = cf.cylinder_2d(nx=256, ny=128, storage=cf.STORE_F32)
+    b100 = native.Batch.footprint(c, 100)
+    b1000 = native.Batch.footprint(c, 1000)
+    big = native.Batch.footprint(c.replace(n=(512, 256, 1)), 100)
+    assert c.ncells * 9 * 4 * 2 < b100 < b1000 < big
+    assert native.Batch.footprint(c.replace(storage=cf.STORE_F64), 100) > b100
+    d = native.to_desc(c)
+    lo = (C.c_int * 3)(0, 0, 0)
+    hi = (C.c_int * 3)(255, 127, 0)
+    assert lib.ilbm_box_collide(C.byref(d), None, None, None, lo, hi, None) == 1
+    f = native.Fab()
+    f.p = 0
+    assert lib.ilbm_box_fill_physical(C.byref(d), C.byref(f), 1, None) == 1
+    assert lib.ilbm_level_create(None, None, None) == 1
+    assert lib.ilbm_amr_create(C.byref(d), 2, None, None, 2, None, None) == 1
+    assert lib.ilbm_batch_create(0, 1, 8, None) == 1
+
+
+def test_serving_workloads_are_seeded_and_in_range():
+    from ilbmrex_amd import serving
+    a = serving.ensemble_cases(64, seed=3)
+    b = serving.ensemble_cases(64, seed=3)
+    assert [c.n for c in a] == [c.n for c in b]
+    for c in a:
+        nx, ny = c.n[0], c.n[1]
+        assert nx == 2 * ny and 64 <= ny <= 512
+        D = 2 * c.R
+        assert 8.0 <= D <= max(8.0, ny / 4.0) + 1e-9 or D >= 0.08 * 20
+        assert c.n_marker == round(3.141592653589793 * D)
+        assert c.nu >= 0.004 - 1e-12 and c.storage == cf.STORE_F32
+    reqs = serving.serving_requests(400, rate=10.0, seed=5)
+    t = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(t) > 0) and abs(len(reqs) / t[-1] - 10.0) < 2.0
+    kinds = [r.case.lattice.dim for r in reqs]
+    assert 0.6 < kinds.count(2) / len(kinds) < 0.8
+    assert all(2000 <= r.nsteps <= 20000 for r in reqs)
+    for r in reqs:
+        if r.case.lattice.dim == 3:
+            assert 32 <= r.case.n[0] <= 192 and r.case.n[0] == r.case.n[1] == r.case.n[2]
+        else:
+            assert 64 <= r.case.n[0] <= 1024 and r.case.n[0] == r.case.n[1]
