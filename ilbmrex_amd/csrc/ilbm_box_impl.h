@@ -360,9 +360,71 @@ static void check_fab(const ilbm_fab* f, int min_comp, const BoxGeom& G, int gho
   }
 }
 
+// macro (rho, ux, uy, uz; [4][cells], no ghosts) into the interior of a fab
+__global__ void k_fab_from_cells(Fab F, const double* m, int nx, int ny, int64_t cells) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+  for (int n = 0; n < 4; ++n) F.at(i, j, k, n) = m[n * cells + t];
+}
+
+__global__ void k_cells_from_fab(Fab F, int comp0, int ncomp, double* out, int nx, int ny, int64_t cells) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+  for (int n = 0; n < ncomp; ++n) out[n * cells + t] = F.at(i, j, k, comp0 + n);
+}
+
 }  // namespace ilbm
 
 extern "C" {
+
+// UpdateDerivedMacroFields of a resident level (AmrCoreLBM.cpp:150-168):
+// FillPatchMacro of the exported macros with one ghost, then visPara.
+int ilbm_level_get_derived(ilbm_level* h, double* host_out) {
+  if (!h || !host_out) return ILBM_ERR_INVALID;
+  ilbm::Level& L = h->L;
+  if (!L.state_valid || !L.mout) return ILBM_ERR_STATE;
+  if (L.cf) return ILBM_ERR_UNSUPPORTED;
+  return guarded([&] {
+    using namespace ilbm;
+    const ilbm_case_desc& c = L.c;
+    const int g[3] = {1, 1, c.dim == 3 ? 1 : 0};
+    ilbm_fab pf{}, of{};
+    for (int d = 0; d < 3; ++d) {
+      pf.lo[d] = -g[d];
+      pf.hi[d] = c.n[d] - 1 + g[d];
+      of.lo[d] = 0;
+      of.hi[d] = c.n[d] - 1;
+    }
+    pf.ncomp = 4;
+    of.ncomp = 6;
+    Fab P = to_fab(pf), O = to_fab(of);
+    double* buf = dev_alloc<double>(4 * P.ns + 6 * O.ns);
+    P.p = buf;
+    O.p = buf + 4 * P.ns;
+    const BoxGeom G = box_geom(c, nullptr, nullptr);
+    const int64_t cells = L.ncells;
+    hipStream_t s = L.stream;
+    hipLaunchKernelGGL(k_fab_from_cells, grid_for(cells), dim3(256), 0, s, P, (const double*)L.mout, c.n[0], c.n[1],
+                       cells);
+    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(P.ns), dim3(256), 0, s, P, G);
+    with_lattice(L.lat, [&](auto lat) {
+      using LAT = decltype(lat);
+      for (int pass = 1; pass <= c.dim; ++pass)
+        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(P.ns), dim3(256), 0, s, P, G, pass, 0);
+    });
+    hipLaunchKernelGGL(k_box_derived, grid_for(cells), dim3(256), 0, s, P, O, G, c.dx[0], c.dx[1], c.dx[2]);
+    double* outd = dev_alloc<double>(2 * cells);
+    hipLaunchKernelGGL(k_cells_from_fab, grid_for(cells), dim3(256), 0, s, O, 4, 2, outd, c.n[0], c.n[1], cells);
+    const hipError_t e1 = hipMemcpyAsync(host_out, outd, sizeof(double) * 2 * cells, hipMemcpyDeviceToHost, s);
+    const hipError_t e2 = hipStreamSynchronize(s);
+    dev_free(outd);
+    dev_free(buf);
+    HIPCHK(e1);
+    HIPCHK(e2);
+  });
+}
 
 int ilbm_box_equilibrium(const ilbm_case_desc* d, const ilbm_fab* macro, ilbm_fab* f, const int lo[3],
                          const int hi[3], void* stream) {

@@ -65,7 +65,7 @@ EXPORTED = [
     "ilbm_amr_num_markers", "ilbm_amr_info",
     "ilbm_box_equilibrium", "ilbm_box_fill_periodic", "ilbm_box_fill_physical", "ilbm_box_collide",
     "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
-    "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore",
+    "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore", "ilbm_level_get_derived",
     "ilbm_batch_checkpoint_size", "ilbm_batch_checkpoint", "ilbm_batch_add_from_checkpoint",
 ]
 
@@ -132,6 +132,7 @@ def load():
     L.ilbm_amr_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
     L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
     L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
+    L.ilbm_level_get_derived.argtypes = [vp, P(C.c_double)]
     L.ilbm_level_checkpoint_size.argtypes = [vp, P(C.c_int64)]
     L.ilbm_level_checkpoint.argtypes = [vp, C.c_void_p, C.c_int64]
     L.ilbm_level_restore.argtypes = [vp, C.c_void_p, C.c_int64]
@@ -256,6 +257,12 @@ class Level:
         out = np.empty(4 * self.case.ncells)
         _check(load().ilbm_level_get_macro(self._h, _dp(out)), "get_macro")
         return out.reshape(4, -1)
+
+    def derived(self) -> np.ndarray:
+        """vorticity and pressure of the exported state, [2][cells]."""
+        out = np.empty(2 * self.case.ncells)
+        _check(load().ilbm_level_get_derived(self._h, _dp(out)), "get_derived")
+        return out.reshape(2, -1)
 
     def forcing(self) -> np.ndarray:
         out = np.empty(3 * self.case.ncells)

@@ -107,6 +107,9 @@ int ilbm_level_synchronize(ilbm_level* L);
 int ilbm_level_get_f(ilbm_level* L, double* host_out);       /* [q][cells] */
 int ilbm_level_get_macro(ilbm_level* L, double* host_out);   /* [4][cells] */
 int ilbm_level_get_forcing(ilbm_level* L, double* host_out); /* [3][cells] */
+/* UpdateDerivedMacroFields (AmrCoreLBM.cpp:150-168, visPara.H:10-54) of the
+ * exported state: [0] vorticity, [1] pressure (rho - 1)/3, [2][cells]. */
+int ilbm_level_get_derived(ilbm_level* L, double* host_out);
 
 /* Sum(Rho) after each step since creation; *n_out = steps recorded. */
 int ilbm_level_mass_history(ilbm_level* L, double* host_out, int max_rows, int* n_out);

@@ -70,3 +70,23 @@ def test_box_derived_fields_taylor_green():
     # centred differences: exact up to sin(k)/k
     assert np.abs(vort.reshape(64, 64) - w_exact * math.sin(k) / k).max() < 1e-15 * 64
     assert np.array_equal(P, (init[0] - 1.0) / 3.0)
+
+
+@pytest.mark.parametrize("which", ["cylinder", "duct3d"])
+def test_level_derived_fields_match_oracle(which):
+    """UpdateDerivedMacroFields of a resident level against the oracle's
+    (a one-level hierarchy refreshes them every step with regrid_int = 1)."""
+    from ilbmrex_amd.native import Level
+    if which == "cylinder":
+        c = cf.cylinder_2d(nx=64, ny=40, D=8.0, xc=20.0, yc=21.0, U=0.05, Re=50.0)
+        init, n = cf.uniform_init(c, u=(0.05, 0.0, 0.0)), 20
+    else:
+        c = cf.square_duct().replace(n=(6, 12, 12))
+        init, n = cf.random_init(c, seed=9, du=0.01), 6
+    lev = Level(c, init)
+    lev.advance(n, want_state=True)
+    got = lev.derived()
+    orc = refcpu.AmrOracle(c, [], 1, init)
+    orc.advance(n)
+    ref = orc.macro(0, all_comps=True)[4:6]
+    _same(got, ref, "vorticity / pressure")
