@@ -258,6 +258,14 @@ def main():
             r, fh = run_level(case, cf.uniform_init(case, u=(0.05, 0.0, 0.0)), 100, 2000, want_forces=True)
             r["cd_last"] = float(fh[-1, 4])
             variants[name] = r
+        # the same case as a one-case batch: hipGraph replay of the step sequence
+        case = cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0,
+                              storage=cf.STORE_F64)
+        from ilbmrex_amd import serving
+        log("[bench] cylinder512_f64_explicit_graph: 2000 steps")
+        wall, lups, _, _ = serving.run_ensemble([case], 2000, 8 << 30)
+        variants["cylinder512_f64_explicit_graph"] = {"mlups": lups / wall / 1e6,
+                                                      "ms_per_step": 1e3 * wall / 2000}
         # pitching NACA 0012 on three levels, FP32
         variants["amr3_pitching_naca_f32"] = amr_variant(args.amr_steps)
         copy_ms = native.load_probe(2 << 30, 20)

@@ -2,9 +2,9 @@
 
 BoxLevel keeps the reference's own state on the device -- f_new / f_old
 (ndir components) and macro_new / macro_old (rho, ux, uy, uz), AMReX Array4
-layout with 3 ghost cells (AmrCoreLBM.H:175-180, 276) -- in torch tensors,
-and steps it with one ilbm_box_* call per stage in the order of
-Source/Core/AdvanceLBMAtLevel.cpp:20-386 for a level of one box:
+layout with 3 ghost cells (AmrCoreLBM.H:175-180, 276) -- in buffers of the
+library's allocator, and steps it with one ilbm_box_* call per stage in the
+order of Source/Core/AdvanceLBMAtLevel.cpp:20-386 for a level of one box:
 
     swap f_new/f_old, macro_new/macro_old
     FillPatch(f_old) -> sfSborder   copy + periodic images + mesoscopicBcFill
@@ -15,7 +15,6 @@ Source/Core/AdvanceLBMAtLevel.cpp:20-386 for a level of one box:
 
 It is the reference-layout counterpart of the fused level (ilbm_level_*): the
 same FP64 results, with the intermediate arrays of the reference visible.
-torch is used for device memory only.
 """
 from __future__ import annotations
 
@@ -32,9 +31,6 @@ NG = 3
 
 class BoxLevel:
     def __init__(self, case: "cf.Case", init_macro: np.ndarray, force: Optional[tuple] = None):
-        import torch
-        if not torch.cuda.is_available():
-            raise native.IlbmError("BoxLevel needs a GPU")
         case.validate()
         self.case = case
         self.L = native.load()
@@ -47,25 +43,26 @@ class BoxLevel:
         g = [NG if d < dim else 0 for d in range(3)]
         self.flo = tuple(self.vlo[d] - g[d] for d in range(3))
         self.fhi = tuple(self.vhi[d] + g[d] for d in range(3))
-        shape3 = tuple(self.fhi[d] - self.flo[d] + 1 for d in (2, 1, 0))
-        dev = torch.device("cuda:0")
-        z = lambda nc: torch.zeros((nc,) + shape3, dtype=torch.float64, device=dev)
-        self.f_new, self.f_old, self.sfS = z(self.q), z(self.q), z(self.q)
-        self.m_new, self.m_old, self.smS = z(4), z(4), z(4)
+        self.shape3 = tuple(self.fhi[d] - self.flo[d] + 1 for d in (2, 1, 0))
+        arr = lambda nc: native.DeviceArray((nc,) + self.shape3)
+        self.f_new, self.f_old, self.sfS = arr(self.q), arr(self.q), arr(self.q)
+        self.m_new, self.m_old, self.smS = arr(4), arr(4), arr(4)
         self.force = None
         if force is not None and any(x != 0.0 for x in force):
-            self.force = z(3)
+            self.force = arr(3)
+            h = np.zeros((3,) + self.shape3)
             for c in range(3):
-                self.force[c].fill_(float(force[c]))
-        sl = self._valid()
-        m = torch.as_tensor(np.ascontiguousarray(init_macro, dtype=np.float64).reshape((4,) + tuple(
-            case.n[d] if d < dim else 1 for d in (2, 1, 0))), device=dev)
-        self.m_new[(slice(None),) + sl] = m
+                h[c] = float(force[c])
+            self.force.upload(h)
+        m = np.zeros((4,) + self.shape3)
+        m[(slice(None),) + self._valid()] = np.asarray(init_macro, dtype=np.float64).reshape(
+            (4,) + tuple(case.n[d] if d < dim else 1 for d in (2, 1, 0)))
+        self.m_new.upload(m)
+        self.m_old.upload(m)
         self._check(self.L.ilbm_box_equilibrium(C.byref(self.desc), self._fab(self.m_new), self._fab(self.f_new),
                                                 self._i3(self.vlo), self._i3(self.vhi), None), "equilibrium")
-        self.f_old.copy_(self.f_new)
-        self.m_old.copy_(self.m_new)
-        torch.cuda.synchronize()
+        self._check(self.L.ilbm_box_copy(self._fab(self.f_new), self._fab(self.f_old), self._i3(self.vlo),
+                                         self._i3(self.vhi), self.q, None), "copy")
 
     def _valid(self):
         return tuple(slice(self.vlo[d] - self.flo[d], self.vhi[d] - self.flo[d] + 1) for d in (2, 1, 0))
@@ -74,24 +71,24 @@ class BoxLevel:
     def _i3(v):
         return (C.c_int * 3)(*v)
 
-    def _fab(self, t):
-        if t is None:
+    def _fab(self, a):
+        if a is None:
             return None
         f = native.Fab()
-        f.p = t.data_ptr()
+        f.p = a.ptr
         f.lo = native.I3(*self.flo)
         f.hi = native.I3(*self.fhi)
-        f.ncomp = t.shape[0]
+        f.ncomp = a.shape[0]
         return C.byref(f)
 
     def _check(self, rc, what):
         native._check(rc, f"ilbm_box_{what}")
 
     def _fill_patch(self, dst, src, meso):
-        dst.zero_()
-        sl = (slice(None),) + self._valid()
-        dst[sl] = src[sl]                                            # ParallelCopy
         d = C.byref(self.desc)
+        self._check(self.L.ilbm_box_setval(self._fab(dst), 0.0, None), "setval")
+        self._check(self.L.ilbm_box_copy(self._fab(src), self._fab(dst), self._i3(self.vlo), self._i3(self.vhi),
+                                         dst.shape[0], None), "copy")              # ParallelCopy
         self._check(self.L.ilbm_box_fill_periodic(d, self._fab(dst), self._i3(self.vlo), self._i3(self.vhi), None),
                     "fill_periodic")
         self._check(self.L.ilbm_box_fill_physical(d, self._fab(dst), int(meso), None), "fill_physical")
@@ -114,15 +111,15 @@ class BoxLevel:
         for _ in range(n):
             self.step()
 
+    def _valid_of(self, a, nc):
+        native._check(self.L.ilbm_device_synchronize(), "synchronize")
+        return a.download()[(slice(0, nc),) + self._valid()].reshape(nc, -1)
+
     def f(self) -> np.ndarray:
-        import torch
-        torch.cuda.synchronize()
-        return self.f_new[(slice(None),) + self._valid()].reshape(self.q, -1).cpu().numpy()
+        return self._valid_of(self.f_new, self.q)
 
     def macro(self) -> np.ndarray:
-        import torch
-        torch.cuda.synchronize()
-        return self.m_new[(slice(None),) + self._valid()].reshape(4, -1).cpu().numpy()
+        return self._valid_of(self.m_new, 4)
 
     def mass_sum(self) -> float:
         out = C.c_double(0.0)
@@ -132,12 +129,14 @@ class BoxLevel:
 
     def derived(self) -> np.ndarray:
         """vorticity and pressure of macro_new (UpdateDerivedMacroFields)."""
-        import torch
-        patch = torch.zeros_like(self.m_new)
+        patch = native.DeviceArray((4,) + self.shape3)
         self._fill_patch(patch, self.m_new, False)
-        out = torch.zeros((6,) + tuple(self.m_new.shape[1:]), dtype=torch.float64, device=self.m_new.device)
+        out = native.DeviceArray((6,) + self.shape3)
+        self._check(self.L.ilbm_box_setval(self._fab(out), 0.0, None), "setval")
         d = C.byref(self.desc)
         self._check(self.L.ilbm_box_derived(d, self._fab(patch), self._fab(out), self._i3(self.vlo),
                                             self._i3(self.vhi), None), "derived")
-        torch.cuda.synchronize()
-        return out[(slice(4, 6),) + self._valid()].reshape(2, -1).cpu().numpy()
+        res = self._valid_of(out, 6)[4:6]
+        patch.free()
+        out.free()
+        return res

@@ -375,9 +375,80 @@ __global__ void k_cells_from_fab(Fab F, int comp0, int ncomp, double* out, int n
   for (int n = 0; n < ncomp; ++n) out[n * cells + t] = F.at(i, j, k, comp0 + n);
 }
 
+// ParallelCopy of a box region between two fabs (same index space)
+__global__ void k_box_copy(Fab S, Fab D, BoxGeom G, int ncomp) {
+  const int nx = G.vhi[0] - G.vlo[0] + 1, ny = G.vhi[1] - G.vlo[1] + 1;
+  const int64_t cells = (int64_t)nx * ny * (G.vhi[2] - G.vlo[2] + 1);
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = G.vlo[0] + (int)(t % nx), j = G.vlo[1] + (int)((t / nx) % ny);
+  const int k = G.vlo[2] + (int)(t / ((int64_t)nx * ny));
+  for (int n = 0; n < ncomp; ++n) D.at(i, j, k, n) = S.at(i, j, k, n);
+}
+
+__global__ void k_fab_setval(Fab F, double v) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= F.ns * F.nc) return;
+  F.p[t] = v;
+}
+
 }  // namespace ilbm
 
 extern "C" {
+
+int ilbm_box_copy(const ilbm_fab* src, ilbm_fab* dst, const int lo[3], const int hi[3], int ncomp, void* stream) {
+  if (!src || !dst || !src->p || !dst->p || !lo || !hi || ncomp < 1 || ncomp > src->ncomp || ncomp > dst->ncomp)
+    return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    BoxGeom G{};
+    G.dim = 3;
+    for (int d = 0; d < 3; ++d) { G.vlo[d] = lo[d]; G.vhi[d] = hi[d]; }
+    const Fab S = to_fab(*src), D = to_fab(*dst);
+    for (int d = 0; d < 3; ++d)
+      if (lo[d] > hi[d]) throw StatusError{ILBM_ERR_INVALID, "empty box"};
+    if (!S.contains(lo[0], lo[1], lo[2]) || !S.contains(hi[0], hi[1], hi[2]) || !D.contains(lo[0], lo[1], lo[2]) ||
+        !D.contains(hi[0], hi[1], hi[2]))
+      throw StatusError{ILBM_ERR_INVALID, "box outside a fab"};
+    hipLaunchKernelGGL(k_box_copy, grid_for(box_cells(G)), dim3(256), 0, (hipStream_t)stream, S, D, G, ncomp);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_setval(ilbm_fab* f, double value, void* stream) {
+  if (!f || !f->p || f->ncomp < 1) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const Fab F = to_fab(*f);
+    hipLaunchKernelGGL(k_fab_setval, grid_for(F.ns * F.nc), dim3(256), 0, (hipStream_t)stream, F, value);
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_device_alloc(int64_t bytes, void** out) {
+  if (!out || bytes <= 0) return ILBM_ERR_INVALID;
+  *out = nullptr;
+  return guarded([&] { *out = ilbm::dev_alloc_bytes((size_t)bytes); });
+}
+
+int ilbm_device_free(void* p) {
+  if (!p) return ILBM_ERR_INVALID;
+  return guarded([&] { HIPCHK(hipFree(p)); });
+}
+
+int ilbm_copy_to_device(void* dst, const void* src, int64_t bytes) {
+  if (!dst || !src || bytes < 0) return ILBM_ERR_INVALID;
+  return guarded([&] { HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice)); });
+}
+
+int ilbm_copy_to_host(void* dst, const void* src, int64_t bytes) {
+  if (!dst || !src || bytes < 0) return ILBM_ERR_INVALID;
+  return guarded([&] { HIPCHK(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost)); });
+}
+
+int ilbm_device_synchronize(void) {
+  return guarded([&] { HIPCHK(hipDeviceSynchronize()); });
+}
 
 // UpdateDerivedMacroFields of a resident level (AmrCoreLBM.cpp:150-168):
 // FillPatchMacro of the exported macros with one ghost, then visPara.

@@ -67,6 +67,8 @@ EXPORTED = [
     "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
     "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore", "ilbm_level_get_derived",
     "ilbm_batch_checkpoint_size", "ilbm_batch_checkpoint", "ilbm_batch_add_from_checkpoint",
+    "ilbm_box_copy", "ilbm_box_setval", "ilbm_device_alloc", "ilbm_device_free", "ilbm_copy_to_device",
+    "ilbm_copy_to_host", "ilbm_device_synchronize",
 ]
 
 ERR_FULL = 6
@@ -148,6 +150,12 @@ def load():
     L.ilbm_box_stream_macro.argtypes = [P(CaseDesc), PF, PF, PF, PF, P(C.c_int), P(C.c_int), vp]
     L.ilbm_box_derived.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), vp]
     L.ilbm_box_sum.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
+    L.ilbm_box_copy.argtypes = [PF, PF, P(C.c_int), P(C.c_int), C.c_int, vp]
+    L.ilbm_box_setval.argtypes = [PF, C.c_double, vp]
+    L.ilbm_device_alloc.argtypes = [C.c_int64, P(vp)]
+    L.ilbm_device_free.argtypes = [vp]
+    L.ilbm_copy_to_device.argtypes = [vp, vp, C.c_int64]
+    L.ilbm_copy_to_host.argtypes = [vp, vp, C.c_int64]
     L.ilbm_status_string.restype = C.c_char_p
     L.ilbm_status_string.argtypes = [C.c_int]
     L.ilbm_last_error.restype = C.c_char_p
@@ -551,6 +559,38 @@ class AmrHierarchy:
         _check(load().ilbm_amr_info(self._h, _dp(out)), "amr_info")
         return {"levels": int(out[0]), "steps": int(out[1]), "time": out[2],
                 "cells": [int(x) for x in out[3:3 + int(out[0])]], "tau_finest": out[6], "tau0": out[7]}
+
+
+class DeviceArray:
+    """FP64 device buffer of the library's allocator, shaped [ncomp][nz][ny][nx]."""
+
+    def __init__(self, shape):
+        self.shape = tuple(int(x) for x in shape)
+        self.nbytes = 8 * int(np.prod(self.shape))
+        p = C.c_void_p()
+        _check(load().ilbm_device_alloc(self.nbytes, C.byref(p)), "device_alloc")
+        self.ptr = p.value
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.size * 8 == self.nbytes
+        _check(load().ilbm_copy_to_device(self.ptr, a.ctypes.data, self.nbytes), "copy_to_device")
+
+    def download(self) -> np.ndarray:
+        out = np.empty(self.shape)
+        _check(load().ilbm_copy_to_host(out.ctypes.data, self.ptr, self.nbytes), "copy_to_host")
+        return out
+
+    def free(self):
+        if self.ptr:
+            load().ilbm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_memory():

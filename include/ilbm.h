@@ -270,6 +270,19 @@ int ilbm_box_stream_macro(const ilbm_case_desc* desc, const ilbm_fab* f_in, ilbm
  * into comp 5 of macro_out from a macro patch with one ghost cell */
 int ilbm_box_derived(const ilbm_case_desc* desc, const ilbm_fab* macro_patch, ilbm_fab* macro_out,
                      const int valid_lo[3], const int valid_hi[3], void* stream);
+/* ParallelCopy of [lo, hi] between two fabs of the same index space, and
+ * setVal of a whole fab (the copy / fill parts of FillPatchSingleLevel and
+ * the force scratch's setVal, AdvanceLBMAtLevel.cpp:43-56) */
+int ilbm_box_copy(const ilbm_fab* src, ilbm_fab* dst, const int lo[3], const int hi[3], int ncomp,
+                  void* stream);
+int ilbm_box_setval(ilbm_fab* fab, double value, void* stream);
+/* Device memory for callers without an allocator of their own (hipMalloc /
+ * hipMemcpy / hipDeviceSynchronize); measurement and test plumbing. */
+int ilbm_device_alloc(int64_t bytes, void** out);
+int ilbm_device_free(void* p);
+int ilbm_copy_to_device(void* dst, const void* src, int64_t bytes);
+int ilbm_copy_to_host(void* dst, const void* src, int64_t bytes);
+int ilbm_device_synchronize(void);
 /* sum of one component over a box (MultiFab::sum of one box; Sum(Rho),
  * the force integral of ComputeIBForce), fixed order; synchronizes */
 int ilbm_box_sum(const ilbm_fab* fab, int comp, const int lo[3], const int hi[3], double* host_out,
