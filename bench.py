@@ -197,6 +197,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--workload", choices=["sphere", "ensemble", "amr"], default="sphere",
+                    help="ensemble / amr: time only that workload (profiler runs) and print its summary")
     ap.add_argument("--probe", action="store_true",
                     help="also run the 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
@@ -206,6 +208,16 @@ def main():
         sys.exit(2)
     from ilbmrex_amd import native
     native.load()
+    if args.workload == "ensemble":
+        from ilbmrex_amd import serving
+        cases = serving.ensemble_cases(256, seed=20261202)
+        wall, lups, _, info = serving.run_ensemble(cases, args.ensemble_steps, 100 << 30)
+        print(json.dumps({"workload": "ensemble of 256 D2Q9 cylinder cases, FP32", "steps": args.ensemble_steps,
+                          "mlups": round(lups / wall / 1e6, 1), "wall_s": round(wall, 4)}), flush=True)
+        return
+    if args.workload == "amr":
+        print(json.dumps(amr_variant(args.amr_steps)), flush=True)
+        return
 
     # headline: sphere, FP32 storage
     c32 = sphere_case(cf.STORE_F32)

@@ -39,18 +39,24 @@ static void launch_fix_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef*
 }
 
 template <class LAT, class ST>
-static void pick_b(bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff) {
+static void launch_fixw_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef* m) {
+  hipLaunchKernelGGL((k_ib_spread_export_w_b<LAT, ST>), g, dim3(64 * kSpreadWaves), 0, s, a, m);
+}
+
+template <class LAT, class ST>
+static void pick_b(bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
   sf = wrap ? launch_step_b<LAT, ST, true> : launch_step_b<LAT, ST, false>;
   gf = launch_ghost_b<LAT, ST>;
   ff = launch_fix_b<LAT, ST>;
+  fw = launch_fixw_b<LAT, ST>;
 }
 
-static void pick_class(int lat, int st, bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff) {
+static void pick_class(int lat, int st, bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
   auto pk = [&](auto latt) {
     using L = decltype(latt);
-    if (st == 0) pick_b<L, StF64>(wrap, sf, gf, ff);
-    else if (st == 1) pick_b<L, StF32>(wrap, sf, gf, ff);
-    else pick_b<L, StF16S>(wrap, sf, gf, ff);
+    if (st == 0) pick_b<L, StF64>(wrap, sf, gf, ff, fw);
+    else if (st == 1) pick_b<L, StF32>(wrap, sf, gf, ff, fw);
+    else pick_b<L, StF16S>(wrap, sf, gf, ff, fw);
   };
   if (lat == 0) pk(D2Q9{});
   else if (lat == 1) pk(D3Q19{});
@@ -73,6 +79,7 @@ struct BatchGroup {
   StepBFn step = nullptr;
   GhostBFn ghost = nullptr;
   FixBFn fix = nullptr;
+  FixBFn fixw = nullptr;
   std::vector<int> members;  // indices into Batch::cases
   StepArgs* dstep[2] = {nullptr, nullptr};
   GhostArgs* dghost[2] = {nullptr, nullptr};
@@ -80,7 +87,7 @@ struct BatchGroup {
   CaseTail* dtail = nullptr;
   BlockRef* dmap = nullptr;
   // block maps inside dmap: offset and count of each kernel's grid
-  enum { M_STEP, M_GHOST, M_BAND, M_INTERP, M_SPREAD, M_RHS, M_MATVEC, M_IBCASE, M_TAIL, M_N };
+  enum { M_STEP, M_GHOST, M_BAND, M_INTERP, M_SPREAD, M_RHS, M_MATVEC, M_IBCASE, M_TAIL, M_FIX, M_FIXW, M_N };
   int64_t moff[M_N] = {}, mcnt[M_N] = {};
   int max_loop = 0;
   bool any_ivc = false, any_iter = false;
@@ -212,8 +219,8 @@ struct Batch {
       int64_t maxblk = 0;
       for (int bx : cand) maxblk = std::max(maxblk, T.tile_blocks(bx));
       T.ensure_partial(maxblk);
-      StepBFn sf; GhostBFn gf; FixBFn ff;
-      pick_class(T.lat, T.st, T.wrap, sf, gf, ff);
+      StepBFn sf; GhostBFn gf; FixBFn ff, fw;
+      pick_class(T.lat, T.st, T.wrap, sf, gf, ff, fw);
       StepArgs* da = dev_alloc<StepArgs>(1);
       BlockRef* dm = dev_alloc<BlockRef>(maxblk);
       hipEvent_t e0, e1;
@@ -346,7 +353,7 @@ struct Batch {
       if (gi < 0) {
         BatchGroup g;
         g.lat = L.lat; g.st = L.st; g.wrap = L.wrap;
-        pick_class(g.lat, g.st, g.wrap, g.step, g.ghost, g.fix);
+        pick_class(g.lat, g.st, g.wrap, g.step, g.ghost, g.fix, g.fixw);
         groups.push_back(g);
         gi = (int)groups.size() - 1;
       }
@@ -386,7 +393,7 @@ struct Batch {
           r.dV = (L.c.dim == 3) ? L.c.dx[0] * L.c.dx[1] * L.c.dx[2] : L.c.dx[0] * L.c.dx[1];
           r.U0 = L.c.U0; r.R = L.c.R;
           r.clock = L.clock_d;
-          ic.nblk_fix = (int)((L.band_cells + 255) / 256);
+          ic.nblk_fix = (int)L.fix_blocks();
         }
         ia[p][ci] = ic;
       }
@@ -404,8 +411,10 @@ struct Batch {
         for (int64_t b = 0; b < (L.band_cells + 255) / 256; ++b) maps[BatchGroup::M_BAND].push_back({c, (int)b});
         for (int b = 0; b < (L.nm + kInterpWaves - 1) / kInterpWaves; ++b)
           maps[BatchGroup::M_INTERP].push_back({c, b});
-        for (int b = 0; b < (L.n_ne + kSpreadWaves - 1) / kSpreadWaves; ++b)
-          maps[BatchGroup::M_SPREAD].push_back({c, b});
+        if (a.nloop > 1)
+          for (int b = 0; b < (L.n_ne + 255) / 256; ++b) maps[BatchGroup::M_SPREAD].push_back({c, b});
+        const int fk = (L.fix_mode() == 3) ? BatchGroup::M_FIXW : BatchGroup::M_FIX;
+        for (unsigned b = 0; b < L.fix_blocks(); ++b) maps[fk].push_back({c, (int)b});
         if (a.use_ivc) {
           for (int b = 0; b < (L.nm + 255) / 256; ++b) maps[BatchGroup::M_RHS].push_back({c, b});
           for (int b = 0; b < (L.nm + 3) / 4; ++b) maps[BatchGroup::M_MATVEC].push_back({c, b});
@@ -464,7 +473,7 @@ struct Batch {
                                m + G.moff[BatchGroup::M_MATVEC], it);
           }
           if (G.mcnt[BatchGroup::M_SPREAD] > 0)
-            hipLaunchKernelGGL(k_ib_spread_b, grid(BatchGroup::M_SPREAD), dim3(64 * kSpreadWaves), 0, stream, A,
+            hipLaunchKernelGGL(k_ib_spread_t_b, grid(BatchGroup::M_SPREAD), dim3(256), 0, stream, A,
                                m + G.moff[BatchGroup::M_SPREAD], it);
           hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
                              m + G.moff[BatchGroup::M_IBCASE], it);
@@ -472,8 +481,10 @@ struct Batch {
             hipLaunchKernelGGL(k_ib_correct_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
                                m + G.moff[BatchGroup::M_BAND], it);
         }
-        G.fix(grid(BatchGroup::M_BAND), stream, A, m + G.moff[BatchGroup::M_BAND]);
-        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(256), 0, stream, A,
+        if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
+        if (G.mcnt[BatchGroup::M_FIXW] > 0)
+          G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
+        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
                            m + G.moff[BatchGroup::M_IBCASE]);
       }
       if (G.mcnt[BatchGroup::M_GHOST] > 0)

@@ -81,7 +81,10 @@ struct IbArgs {
   int use_ivc;          // IVC solve (operator factorized) instead of explicit forcing
   int ioff[3];          // global cell index of local cell 0 (refined levels)
   int dlo[3], dhi[3];   // the level's domain in local cell indices
+  int* chunk_box;       // moving body: [6][nchunks] min / max marker cell per dim of each marker chunk
 };
+
+constexpr int kMarkerChunk = 32;
 
 __device__ inline double delta_fn(double xf, double xp, double h, int type) {
   const double rr = fabs(xf - xp) / h;
@@ -234,6 +237,26 @@ __global__ void k_ib_weights(IbArgs a) {
   a.scale[nm + m] = (a.renorm_spread && part > 0.0) ? 1.0 / part : 1.0;  // spreading factor
 }
 
+// cell bounds of every chunk of kMarkerChunk consecutive markers (moving
+// bodies), so the gather spreading visits only the chunks that reach a cell
+__global__ void k_ib_chunk_bounds(IbArgs a) {
+  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
+  const int nch = (a.nm + kMarkerChunk - 1) / kMarkerChunk;
+  if (ch >= nch) return;
+  const int nm = a.nm;
+  const int m0 = ch * kMarkerChunk, m1 = (m0 + kMarkerChunk < nm) ? m0 + kMarkerChunk : nm;
+  for (int d = 0; d < 3; ++d) {
+    int lo = 1 << 30, hi = -(1 << 30);
+    for (int m = m0; m < m1; ++m) {
+      const int b = a.base[d * nm + m];
+      lo = b < lo ? b : lo;
+      hi = b > hi ? b : hi;
+    }
+    a.chunk_box[(2 * d) * nch + ch] = lo;
+    a.chunk_box[(2 * d + 1) * nch + ch] = hi;
+  }
+}
+
 __device__ inline void target_velocity(const IbArgs& a, int m, double* ux, double* uy, double* uz) {
   const int nm = a.nm;
   if (a.geometry == 2) {
@@ -360,6 +383,7 @@ __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
 
 __global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp(IbArgs a) { interp_body(a, blockIdx.x); }
 
+
 __global__ void k_ib_force_explicit(IbArgs a) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.nm) return;
@@ -416,6 +440,56 @@ __global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
   const double tV = a.wts[(1 * 5 + jj) * nm + m];
   const double sc = a.scale[nm + m];
   a.ent_w[e] = (a.dim == 3) ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
+}
+
+// The spreading sum of one band cell from its CSR list: entries in list
+// order (direct contributions first, then each periodic image group in
+// marker order, added as a whole as SumBoundary adds ghost sums) -- the
+// serial scatter's order.
+__device__ inline void spread_cell_sum(const IbArgs& a, int64_t b, double tot[3]) {
+  const int nm = a.nm;
+  const int e0 = a.rowptr[b], e1 = a.rowptr[b + 1];
+  double gs[3] = {0.0, 0.0, 0.0};
+  tot[0] = tot[1] = tot[2] = 0.0;
+  int cur = 0;
+  for (int e = e0; e < e1; ++e) {
+    const int m = a.ent_m[e];
+    const double dval = a.ent_w[e];
+    const int grp = a.ent_code[e] >> 9;
+    const double p0 = dval * (a.F[m] * a.dv);
+    const double p1 = dval * (a.F[nm + m] * a.dv);
+    const double p2 = dval * (a.F[2 * nm + m] * a.dv);
+    if (grp != cur) {
+      if (cur != 0)
+        for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+      gs[0] = gs[1] = gs[2] = 0.0;
+      cur = grp;
+    }
+    if (grp == 0) {
+      tot[0] += p0; tot[1] += p1;
+      if (a.dim == 3) tot[2] += p2;
+    } else {
+      gs[0] += p0; gs[1] += p1;
+      if (a.dim == 3) gs[2] += p2;
+    }
+  }
+  if (cur != 0)
+    for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+}
+
+// multi-iteration forcing: EulerForceIter of the non-empty band cells
+__device__ __forceinline__ void spread_thread_body(const IbArgs& a, int ci) {
+  if (ci >= a.n_ne) return;
+  const int b = a.ne_cells[ci];
+  double tot[3];
+  spread_cell_sum(a, b, tot);
+  a.EFI[b] = tot[0];
+  a.EFI[a.band_cells + b] = tot[1];
+  a.EFI[2 * a.band_cells + b] = tot[2];
+}
+
+__global__ void __launch_bounds__(256) k_ib_spread_t(IbArgs a) {
+  spread_thread_body(a, blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // One wavefront per band cell that receives contributions: lanes form the
@@ -607,7 +681,43 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const doub
 // term of the coming collision on the same cell (collide_forced.H:22-50),
 // which the stream-collide kernel left out because F was not yet known.
 // Each block also sums its exported force for the step's force record.
-template <class LAT, class ST>
+// Per band cell totals of a moving body's gather spreading: the markers
+// whose stencil covers the cell, in marker order (k_ib_spread_all's
+// arithmetic), visiting only the marker chunks whose cell bounds reach it.
+__device__ inline void gather_all_sum(const IbArgs& a, int64_t b, bool active, double tot[3]) {
+  tot[0] = tot[1] = tot[2] = 0.0;
+  if (!active) return;
+  const int nm = a.nm;
+  const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
+  const int i = a.band_lo[0] + (int)(b % a.band_n[0]);
+  const int j = a.band_lo[1] + (int)((b / a.band_n[0]) % a.band_n[1]);
+  const int k = a.band_lo[2] + (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+  const bool d3 = (a.dim == 3);
+  for (int ch = 0; ch < nch; ++ch) {
+    if (i < a.chunk_box[ch] - 2 || i > a.chunk_box[nch + ch] + 2) continue;
+    if (j < a.chunk_box[2 * nch + ch] - 2 || j > a.chunk_box[3 * nch + ch] + 2) continue;
+    if (d3 && (k < a.chunk_box[4 * nch + ch] - 2 || k > a.chunk_box[5 * nch + ch] + 2)) continue;
+    const int m0 = ch * kMarkerChunk, m1 = (m0 + kMarkerChunk < nm) ? m0 + kMarkerChunk : nm;
+    for (int m = m0; m < m1; ++m) {
+      const int ii = i - a.base[m] + 2, jj = j - a.base[nm + m] + 2;
+      const int kk = d3 ? k - a.base[2 * nm + m] + 2 : 0;
+      if ((unsigned)ii > 4u || (unsigned)jj > 4u || (unsigned)kk > 4u) continue;
+      const double tU = a.wts[(0 * 5 + ii) * nm + m];
+      const double tV = a.wts[(1 * 5 + jj) * nm + m];
+      const double sc = a.scale[nm + m];
+      const double dval = d3 ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
+      tot[0] += dval * (a.F[m] * a.dv);
+      tot[1] += dval * (a.F[nm + m] * a.dv);
+      if (d3) tot[2] += dval * (a.F[2 * nm + m] * a.dv);
+    }
+  }
+}
+
+// Export of the force of band cell b: MODE 0 reads EulerForceIter /
+// EulerForceTotal, MODE 1 sums the cell's CSR spreading list first, MODE 2
+// gathers a moving body's markers (single-iteration forcing, where the
+// iteration's force is the total).
+template <class LAT, class ST, int MODE>
 __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
                                                   const RecArgs& rec, int blk, int nblk) {
   using T = typename ST::compute_t;
@@ -615,12 +725,21 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   __shared__ double lds[3][4];
   const int64_t b = (int64_t)blk * 256 + threadIdx.x;
   double fx = 0.0, fy = 0.0, fz = 0.0;
+  double cell_tot[3] = {0.0, 0.0, 0.0};
+  if constexpr (MODE == 2) gather_all_sum(a, b, b < a.band_cells, cell_tot);
   if (b < a.band_cells) {
-    const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
     const double rho = a.Mband[b];
-    fx = a.dt * rho * tot[b];
-    fy = a.dt * rho * tot[a.band_cells + b];
-    fz = a.dt * rho * tot[2 * a.band_cells + b];
+    if constexpr (MODE == 0) {
+      const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
+      cell_tot[0] = tot[b];
+      cell_tot[1] = tot[a.band_cells + b];
+      cell_tot[2] = tot[2 * a.band_cells + b];
+    } else if constexpr (MODE == 1) {
+      spread_cell_sum(a, b, cell_tot);
+    }
+    fx = a.dt * rho * cell_tot[0];
+    fy = a.dt * rho * cell_tot[1];
+    fz = a.dt * rho * cell_tot[2];
     a.Fband[b] = fx;
     a.Fband[a.band_cells + b] = fy;
     a.Fband[2 * a.band_cells + b] = fz;
@@ -658,15 +777,146 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   }
 }
 
-template <class LAT, class ST>
+template <class LAT, class ST, int MODE>
 __global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, void* gbuf, double tau, RecArgs rec) {
-  export_fixup_body<LAT, ST>(a, g, gbuf, tau, rec, blockIdx.x, gridDim.x);
+  export_fixup_body<LAT, ST, MODE>(a, g, gbuf, tau, rec, blockIdx.x, gridDim.x);
+}
+
+// 3D single-iteration forcing: one wavefront per non-empty band cell sums
+// its CSR list (lanes form the products, lane 0 adds them in list order as
+// k_ib_spread does), exports F = dt rho EulerForce and adds the Guo term of
+// the coming collision, one population per lane.  Band cells without
+// entries keep F = 0.  Block partials of F in wave order for the record.
+template <class LAT, class ST>
+__device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
+                                                        const RecArgs& rec, int blk, int nblk) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  __shared__ double pr[kSpreadWaves][3][kSpreadChunk];
+  __shared__ int gp[kSpreadWaves][kSpreadChunk];
+  __shared__ int rounds[kSpreadWaves];
+  __shared__ double fsh[kSpreadWaves][3];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int ci = blk * kSpreadWaves + wv;
+  const bool active = ci < a.n_ne;
+  const int nm = a.nm;
+  const int b = active ? a.ne_cells[ci] : 0;
+  const int e0 = active ? a.rowptr[b] : 0, e1 = active ? a.rowptr[b + 1] : 0;
+  double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
+  int cur = 0;
+  if (lane == 0) rounds[wv] = (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
+  __syncthreads();
+  int nr = 0;
+  for (int w2 = 0; w2 < kSpreadWaves; ++w2) nr = rounds[w2] > nr ? rounds[w2] : nr;
+  for (int r = 0; r < nr; ++r) {
+    const int base = e0 + r * kSpreadChunk;
+    for (int t = lane; t < kSpreadChunk; t += 64) {
+      const int e = base + t;
+      if (e < e1) {
+        const int m = a.ent_m[e];
+        const double dval = a.ent_w[e];
+        pr[wv][0][t] = dval * (a.F[m] * a.dv);
+        pr[wv][1][t] = dval * (a.F[nm + m] * a.dv);
+        pr[wv][2][t] = dval * (a.F[2 * nm + m] * a.dv);
+        gp[wv][t] = a.ent_code[e] >> 9;
+      }
+    }
+    __syncthreads();
+    if (active && lane == 0) {
+      const int cnt = (e1 - base < kSpreadChunk) ? e1 - base : kSpreadChunk;
+      for (int t = 0; t < cnt; ++t) {
+        const int grp = gp[wv][t];
+        if (grp != cur) {
+          if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+          gs[0] = gs[1] = gs[2] = 0.0;
+          cur = grp;
+        }
+        double* acc = (grp == 0) ? tot : gs;
+        acc[0] += pr[wv][0][t];
+        acc[1] += pr[wv][1][t];
+        if (a.dim == 3) acc[2] += pr[wv][2][t];
+      }
+    }
+    __syncthreads();
+  }
+  if (lane == 0) {
+    double fx = 0.0, fy = 0.0, fz = 0.0;
+    if (active) {
+      if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+      const double rho = a.Mband[b];
+      fx = a.dt * rho * tot[0];
+      fy = a.dt * rho * tot[1];
+      fz = a.dt * rho * tot[2];
+      a.Fband[b] = fx;
+      a.Fband[a.band_cells + b] = fy;
+      a.Fband[2 * a.band_cells + b] = fz;
+    }
+    fsh[wv][0] = fx; fsh[wv][1] = fy; fsh[wv][2] = fz;
+  }
+  __syncthreads();
+  if (active && lane < LAT::Q) {
+    const double fx = fsh[wv][0], fy = fsh[wv][1], fz = fsh[wv][2];
+    if (!(fx == 0.0 && fy == 0.0 && fz == 0.0)) {
+      const int bi = (int)(b % a.band_n[0]);
+      const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+      const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+      const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
+      const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
+      const T uz = T(a.Mband[3 * a.band_cells + b]);
+      const T fac = T(1.0) - T(0.5) / T(tau);
+      R* gb = static_cast<R*>(gbuf);
+      const int q = lane;
+      const int64_t o = (int64_t)q * g.qs + g.idx(i, j, k);
+      T gq = ST::load(gb, o, T(LAT::w[q]));
+      gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
+      ST::store(gb, o, gq, T(LAT::w[q]));
+    }
+  }
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 3; ++c) {
+      double sum = 0.0;
+      for (int w2 = 0; w2 < kSpreadWaves; ++w2) sum += fsh[w2][c];
+      rec.partial[(int64_t)c * nblk + blk] = sum;
+    }
+  }
+}
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_export_w(IbArgs a, GridGeom g, void* gbuf, double tau,
+                                                                          RecArgs rec) {
+  spread_export_wave_body<LAT, ST>(a, g, gbuf, tau, rec, blockIdx.x, gridDim.x);
 }
 
 // Adds the export kernel's block sums in block order and writes the
-// force.dat row of the step.
+// force.dat row of the step.  The first 256 threads reduce, whatever the
+// block size, so the tree is the same in every launch.
 __device__ __forceinline__ void force_record_body(const IbArgs& a, const RecArgs& rec, int nblk) {
   __shared__ double lds[3][4];
+  double acc[3] = {0.0, 0.0, 0.0};
+  if (threadIdx.x < 256)
+    for (int t = threadIdx.x; t < nblk; t += 256)
+      for (int c = 0; c < 3; ++c) acc[c] += rec.partial[(int64_t)c * nblk + t];
+  for (int c = 0; c < 3; ++c) {
+    const double r = wave_sum(acc[c]);
+    if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) lds[c][threadIdx.x >> 6] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sum[3] = {0.0, 0.0, 0.0};
+    for (int c = 0; c < 3; ++c)
+      for (int wv = 0; wv < 4; ++wv) sum[c] += lds[c][wv];
+    write_force_record(a, rec, sum);
+  }
+}
+
+__global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) { force_record_body(a, rec, nblk); }
+
+// Single-iteration forcing: the marker sums (k_ib_sums, same 1024-thread
+// tree) and the record in one launch.
+__device__ __forceinline__ void sums_record_body(const IbArgs& a, const RecArgs& rec, int nblk) {
+  __shared__ double lds[3][16];
+  sums_body(a, 1);
+  __syncthreads();
   double acc[3] = {0.0, 0.0, 0.0};
   for (int t = threadIdx.x; t < nblk; t += blockDim.x)
     for (int c = 0; c < 3; ++c) acc[c] += rec.partial[(int64_t)c * nblk + t];
@@ -683,7 +933,7 @@ __device__ __forceinline__ void force_record_body(const IbArgs& a, const RecArgs
   }
 }
 
-__global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) { force_record_body(a, rec, nblk); }
+__global__ void __launch_bounds__(1024) k_sums_record(IbArgs a, RecArgs rec, int nblk) { sums_record_body(a, rec, nblk); }
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of
 // the exported force and the marker sums, appended to the history.
@@ -763,6 +1013,14 @@ __global__ void __launch_bounds__(64 * kInterpWaves) k_ib_interp_b(const IbCase*
   interp_body(a, r.b);
 }
 
+__global__ void __launch_bounds__(256) k_ib_spread_t_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map,
+                                                       int it) {
+  const BlockRef r = map[blockIdx.x];
+  const IbArgs& a = A[r.c].ib;
+  if (it >= a.nloop || a.nloop == 1) return;
+  spread_thread_body(a, r.b * 256 + threadIdx.x);
+}
+
 __global__ void k_ib_ivc_rhs_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map, int it) {
   const BlockRef r = map[blockIdx.x];
   const IbArgs& a = A[r.c].ib;
@@ -789,7 +1047,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_b(const IbCase*
 __global__ void __launch_bounds__(1024) k_ib_sums_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map,
                                                     int it) {
   const IbArgs& a = A[map[blockIdx.x].c].ib;
-  if (it >= a.nloop) return;
+  if (it >= a.nloop || a.nloop == 1) return;
   sums_body(a, it == 0);
 }
 
@@ -805,12 +1063,22 @@ __global__ void __launch_bounds__(256) k_ib_export_fixup_b(const IbCase* __restr
                                                            const BlockRef* __restrict__ map) {
   const BlockRef r = map[blockIdx.x];
   const IbCase& c = A[r.c];
-  export_fixup_body<LAT, ST>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
+  if (c.ib.nloop == 1) export_fixup_body<LAT, ST, 1>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
+  else export_fixup_body<LAT, ST, 0>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
 }
 
-__global__ void __launch_bounds__(256) k_force_record_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map) {
+template <class LAT, class ST>
+__global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_export_w_b(const IbCase* __restrict__ A,
+                                                                            const BlockRef* __restrict__ map) {
+  const BlockRef r = map[blockIdx.x];
+  const IbCase& c = A[r.c];
+  spread_export_wave_body<LAT, ST>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
+}
+
+__global__ void __launch_bounds__(1024) k_force_record_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map) {
   const IbCase& c = A[map[blockIdx.x].c];
-  force_record_body(c.ib, c.rec, c.nblk_fix);
+  if (c.ib.nloop == 1) sums_record_body(c.ib, c.rec, c.nblk_fix);
+  else force_record_body(c.ib, c.rec, c.nblk_fix);
 }
 
 }  // namespace ilbm

@@ -281,7 +281,7 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
 // ---------------------------------------------------------------------------
 using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&);
 using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
-using FixFn = void (*)(dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
+using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 
 template <class LAT, class ST, bool W>
@@ -292,10 +292,16 @@ template <class LAT, class ST, bool W>
 static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
   hipLaunchKernelGGL((k_init<LAT, ST, W>), g, b, 0, s, a);
 }
+// mode 0: export of EulerForceIter / Total; 1: fused CSR spreading (thread
+// per band cell); 2: fused gather spreading of a moving body; 3: fused CSR
+// spreading, one wavefront per non-empty band cell
 template <class LAT, class ST>
-static void launch_fix(dim3 g, dim3 b, hipStream_t s, const IbArgs& a, const GridGeom& gg, void* buf,
+static void launch_fix(int mode, dim3 g, dim3 b, hipStream_t s, const IbArgs& a, const GridGeom& gg, void* buf,
                        double tau, const RecArgs& r) {
-  hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST>), g, b, 0, s, a, gg, buf, tau, r);
+  if (mode == 3) hipLaunchKernelGGL((k_ib_spread_export_w<LAT, ST>), g, b, 0, s, a, gg, buf, tau, r);
+  else if (mode == 1) hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 1>), g, b, 0, s, a, gg, buf, tau, r);
+  else if (mode == 2) hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 2>), g, b, 0, s, a, gg, buf, tau, r);
+  else hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 0>), g, b, 0, s, a, gg, buf, tau, r);
 }
 template <class LAT, class ST>
 static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
@@ -394,6 +400,7 @@ struct Level {
   bool moving = false;
   bool dev_kin = false;          // moving body advanced on the device (explicit forcing)
   double* xref_d = nullptr;      // [2][nm] section outline in body coordinates
+  int* chunk_d = nullptr;        // [6][marker chunks] cell bounds of the moving markers
   // timing
   bool timing = false;
   std::vector<hipEvent_t> ev_start, ev_stop;
@@ -410,7 +417,7 @@ struct Level {
     dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
-    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d); dev_free(xref_d);
+    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d); dev_free(xref_d); dev_free(chunk_d);
     for (int b = 0; b < 2; ++b) { dev_free(Mw[b]); dev_free(Dw[b]); }
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -585,6 +592,7 @@ struct Level {
       std::vector<double> xr(2 * (size_t)nm);
       for (int m = 0; m < nm; ++m) { xr[m] = mg.xref[m + 1]; xr[nm + m] = mg.yref[m + 1]; }
       xref_d = dev_alloc<double>(2 * nm);
+      chunk_d = dev_alloc<int>(6 * ((nm + kMarkerChunk - 1) / kMarkerChunk));
       HIPCHK(hipMemcpyAsync(xref_d, xr.data(), sizeof(double) * 2 * nm, hipMemcpyHostToDevice, stream));
       HIPCHK(hipStreamSynchronize(stream));
     }
@@ -613,7 +621,7 @@ struct Level {
     HIPCHK(hipMemsetAsync(U_d, 0, sizeof(double) * 3 * nm, stream));
     HIPCHK(hipMemsetAsync(Mm_d, 0, sizeof(double) * 3 * nm, stream));
     rowptr_d = dev_alloc<int>(band_cells + 1);
-    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + 1));
+    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + 2));
     upload_markers();
     ncol = (c.dim == 3) ? 33 : 23;
   }
@@ -773,6 +781,7 @@ struct Level {
     a.Mband = Mband; a.EV = EV; a.EFI = EFI; a.EFT = EFT; a.Fband = Fband;
     a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
     a.ne_cells = ne_d; a.n_ne = n_ne;
+    a.chunk_box = chunk_d;
     a.ev_from_macro = (c.coupling_method == 1 || std::max(c.loop_ns, 1) == 1) ? 1 : 0;
     a.use_ivc = (c.coupling_method == 1 && ivc_ready) ? 1 : 0;
     // IVC whose LU failed falls back to explicit forcing (IBMarkerDF.cpp:1253-1259)
@@ -786,6 +795,18 @@ struct Level {
     }
     a.Ainv = Ainv_d; a.rhs = rhs_d; a.sums = sums_d;
     return a;
+  }
+
+  // how the IB force is exported (see launch_fix) and its block count
+  int fix_mode() const {
+    const int nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
+    if (nloop > 1) return 0;
+    if (dev_kin) return 2;
+    return (c.dim == 3) ? 3 : 1;
+  }
+  unsigned fix_blocks() const {
+    if (fix_mode() == 3) return (unsigned)((n_ne + kSpreadWaves - 1) / kSpreadWaves);
+    return (unsigned)((band_cells + 255) / 256);
   }
 
   BcSpec bc_spec() const {
@@ -817,6 +838,8 @@ struct Level {
       hipLaunchKernelGGL(k_ib_move, dim3((nm + 255) / 256), dim3(256), 0, stream, mv);
       const IbArgs aw = ib_args();
       hipLaunchKernelGGL(k_ib_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, aw);
+      const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
+      hipLaunchKernelGGL(k_ib_chunk_bounds, dim3((nch + 63) / 64), dim3(64), 0, stream, aw);
     } else if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
@@ -835,11 +858,11 @@ struct Level {
         hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
         hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
       }
+      if (nloop == 1) break;  // single iteration: spreading fused into the export
       if (dev_kin)
         hipLaunchKernelGGL(k_ib_spread_all, dim3(gb), dim3(tb), 0, stream, a);
       else if (n_ne)
-        hipLaunchKernelGGL(k_ib_spread, dim3((n_ne + kSpreadWaves - 1) / kSpreadWaves), dim3(64 * kSpreadWaves), 0,
-                           stream, a);
+        hipLaunchKernelGGL(k_ib_spread_t, dim3((n_ne + 255) / 256), dim3(256), 0, stream, a);
       hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
       if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
     }
@@ -851,8 +874,15 @@ struct Level {
     r.U0 = c.U0; r.R = c.R;
     r.clock = nullptr;
     if (rec_override) { r.time = rec_time; r.step = rec_step; }
-    fixfn(dim3(gb), dim3(tb), stream, a, g, gbuf[cur], tau, r);
-    if (rec_enable) hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gb);
+    const int mode = fix_mode();
+    const unsigned gfix = fix_blocks();
+    if (gfix) fixfn(mode, dim3(gfix), dim3(tb), stream, a, g, gbuf[cur], tau, r);
+    if (nloop == 1) {
+      if (rec_enable) hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
+      else hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
+    } else if (rec_enable) {
+      hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gfix);
+    }
     HIPCHK(hipGetLastError());
   }
 
