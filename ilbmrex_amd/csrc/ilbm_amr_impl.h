@@ -1,8 +1,9 @@
 // AMR hierarchy on one device (compiled as part of ilbm_level.hip): level 0
 // is the case's domain, each finer level one box at refinement ratio 2,
 // advanced with subcycling as timeStepWithSubcycling does
-// (Source/Core/AmrCoreLBM.cpp:1688-1754), regridding left out (the boxes
-// are fixed).  Every level is a resident Level; a refined level has ghost
+// (Source/Core/AmrCoreLBM.cpp:1688-1754), with the regrid of :1692-1720
+// when it is switched on (ErrorEst tags on the device, one box per level
+// chosen on the host, RemakeLevel on the device).  Every level is a resident Level; a refined level has ghost
 // slabs on all faces, filled from its coarser level by k_cf_fill, and the
 // immersed body lives on the finest level only (AdvanceLBMAtLevel.cpp:87-99).
 //
@@ -71,35 +72,40 @@ struct Amr {
   int mass_cap = 0;
   std::vector<double> mass_host, force_host;
   int ncol = 0;
+  ilbm_case_desc d0{};            // the case as given (refinement-region parameters)
+  // regrid: ErrorEst + one box per level + RemakeLevel
+  bool regrid_on = false;
+  ilbm_regrid_params rg{};
+  int last_regrid[kMaxLev] = {0, 0, 0};
+  int nregrid = 0;
+  double* Dfull[kMaxLev][2] = {};   // windowed levels: [2][cells] vorticity, pressure after step s in [s & 1]
+  double* patch[kMaxLev] = {};      // one-ghost macro patch [4][patch cells]
+  int64_t patch_cap[kMaxLev] = {};
+  double* scratch = nullptr;        // [2][cells] derived fields for tagging
+  int64_t scratch_cap = 0;
+  int* tag_d = nullptr;             // count, bbox lo[3], hi[3]
+  unsigned long long* vmax_d = nullptr;
 
   ~Amr() {
     if (stream) (void)hipStreamSynchronize(stream);
     L.clear();
-    for (int l = 0; l < kMaxLev; ++l) dev_free(dpartial[l]);
+    for (int l = 0; l < kMaxLev; ++l) {
+      dev_free(dpartial[l]);
+      dev_free(Dfull[l][0]);
+      dev_free(Dfull[l][1]);
+      dev_free(patch[l]);
+    }
+    dev_free(scratch);
+    dev_free(tag_d);
+    dev_free(vmax_d);
     dev_free(mass_dev);
     if (stream) (void)hipStreamDestroy(stream);
   }
 
-  void create(const ilbm_case_desc& d0, int nl, const int* box_lo, const int* box_hi, int rint,
-              const double st[4]) {
-    if (nl < 1 || nl > kMaxLev) throw StatusError{ILBM_ERR_UNSUPPORTED, "1 to 3 levels"};
-    if (rint < 1) throw StatusError{ILBM_ERR_INVALID, "regrid_int must be >= 1"};
-    if (d0.nmac != 4 && d0.nmac != 6) throw StatusError{ILBM_ERR_UNSUPPORTED, "nmac must be 4 or 6"};
-    nlev = nl;
-    regrid_int = rint;
-    dim = d0.dim;
-    if (dim != 2 && dim != 3) throw StatusError{ILBM_ERR_INVALID, "dim"};
-    for (int d = 0; d < 3; ++d) {
-      blo[0][d] = 0;
-      bhi[0][d] = (d < dim) ? d0.n[d] - 1 : 0;
-    }
-    for (int l = 1; l < nlev; ++l)
-      for (int d = 0; d < 3; ++d) {
-        blo[l][d] = (d < dim) ? box_lo[3 * (l - 1) + d] : 0;
-        bhi[l][d] = (d < dim) ? box_hi[3 * (l - 1) + d] : 0;
-      }
-    // nesting: a fine box covers whole coarse cells and its window (the
-    // coarse footprint grown by 3) lies in the coarse box
+  // covered cells and windows from the boxes: a fine box covers whole
+  // coarse cells and its window (the coarse footprint grown by 3) lies in
+  // the coarse box
+  void nest_geometry() {
     for (int l = 1; l < nlev; ++l)
       for (int d = 0; d < dim; ++d) {
         if (blo[l][d] % 2 != 0 || bhi[l][d] % 2 != 1 || bhi[l][d] < blo[l][d])
@@ -114,47 +120,89 @@ struct Amr {
       }
     for (int l = 0; l < nlev - 1; ++l)
       for (int d = dim; d < 3; ++d) { clo[l][d] = 0; cn[l][d] = 1; wlo[l][d] = 0; wn[l][d] = 1; }
+  }
+
+  // window storage of level l for the current box of level l+1 (contents
+  // left to the caller), and the average-down partials of its footprint
+  void alloc_window(int l) {
+    Level& V = *L[l];
+    V.window = true;
+    int64_t cells = 1;
+    for (int d = 0; d < 3; ++d) {
+      V.band_lo[d] = wlo[l][d] - V.ioff[d];
+      V.band_n[d] = wn[l][d];
+      cells *= wn[l][d];
+    }
+    V.band_cells = cells;
+    for (int b = 0; b < 2; ++b) {
+      dev_free(V.Mw[b]);
+      dev_free(V.Dw[b]);
+      V.Mw[b] = dev_alloc<double>(4 * cells);
+      V.Dw[b] = dev_alloc<double>(2 * cells);
+      HIPCHK(hipMemsetAsync(V.Dw[b], 0, sizeof(double) * 2 * cells, stream));
+    }
+    const int64_t ncov = (int64_t)cn[l][0] * cn[l][1] * cn[l][2];
+    dev_free(dpartial[l]);
+    dblocks[l] = (ncov + 255) / 256;
+    dpartial[l] = dev_alloc<double>(dblocks[l]);
+  }
+
+  // a Level for the current box of level l, with its window (l < nlev-1)
+  // and its marker set (finest), not yet initialised
+  std::unique_ptr<Level> new_level(int l) {
+    std::unique_ptr<Level> V(new Level());
+    ilbm_case_desc c = d0;
+    const int fac = 1 << l;
+    for (int d = 0; d < dim; ++d) {
+      c.dx[d] = d0.dx[d] / (double)fac;
+      c.n[d] = bhi[l][d] - blo[l][d] + 1;
+      V->ioff[d] = blo[l][d];
+      V->dom_n[d] = d0.n[d] * fac;
+    }
+    c.use_ib = (l == nlev - 1) ? d0.use_ib : 0;
+    V->c = c;
+    V->cf = (l > 0);
+    V->stream = stream;
+    V->own_stream = false;
+    V->validate();
+    V->setup_layout();
+    if ((int)L.size() <= l) L.resize(l + 1);
+    std::swap(L[l], V);   // alloc_window addresses L[l]
+    if (l < nlev - 1) alloc_window(l);
+    L[l]->setup_ib();
+    std::swap(L[l], V);
+    return V;
+  }
+
+  void create(const ilbm_case_desc& dd, int nl, const int* box_lo, const int* box_hi, int rint,
+              const double st[4]) {
+    if (nl < 1 || nl > kMaxLev) throw StatusError{ILBM_ERR_UNSUPPORTED, "1 to 3 levels"};
+    if (rint < 1) throw StatusError{ILBM_ERR_INVALID, "regrid_int must be >= 1"};
+    if (dd.nmac != 4 && dd.nmac != 6) throw StatusError{ILBM_ERR_UNSUPPORTED, "nmac must be 4 or 6"};
+    d0 = dd;
+    nlev = nl;
+    regrid_int = rint;
+    dim = d0.dim;
+    if (dim != 2 && dim != 3) throw StatusError{ILBM_ERR_INVALID, "dim"};
+    for (int d = 0; d < 3; ++d) {
+      blo[0][d] = 0;
+      bhi[0][d] = (d < dim) ? d0.n[d] - 1 : 0;
+    }
+    for (int l = 1; l < nlev; ++l)
+      for (int d = 0; d < 3; ++d) {
+        blo[l][d] = (d < dim) ? box_lo[3 * (l - 1) + d] : 0;
+        bhi[l][d] = (d < dim) ? box_hi[3 * (l - 1) + d] : 0;
+      }
+    nest_geometry();
+    L.clear();
+    L.resize(nlev);
     for (int l = 0; l < nlev; ++l) {
-      std::unique_ptr<Level> V(new Level());
-      ilbm_case_desc c = d0;
-      const int fac = 1 << l;
-      for (int d = 0; d < dim; ++d) {
-        c.dx[d] = d0.dx[d] / (double)fac;
-        c.n[d] = bhi[l][d] - blo[l][d] + 1;
-        V->ioff[d] = blo[l][d];
-        V->dom_n[d] = d0.n[d] * fac;
-      }
-      c.use_ib = (l == nlev - 1) ? d0.use_ib : 0;
-      V->c = c;
-      V->cf = (l > 0);
-      V->stream = stream;
-      V->own_stream = false;
-      V->validate();
-      V->setup_layout();
-      if (l < nlev - 1) {
-        V->window = true;
-        int64_t cells = 1;
-        for (int d = 0; d < 3; ++d) {
-          V->band_lo[d] = wlo[l][d] - V->ioff[d];
-          V->band_n[d] = wn[l][d];
-          cells *= wn[l][d];
-        }
-        V->band_cells = cells;
-        for (int b = 0; b < 2; ++b) {
-          V->Mw[b] = dev_alloc<double>(4 * cells);
-          V->Dw[b] = dev_alloc<double>(2 * cells);
-          HIPCHK(hipMemsetAsync(V->Dw[b], 0, sizeof(double) * 2 * cells, stream));
-        }
-        const int64_t ncov = (int64_t)cn[l][0] * cn[l][1] * cn[l][2];
-        dblocks[l] = (ncov + 255) / 256;
-        dpartial[l] = dev_alloc<double>(dblocks[l]);
-      }
-      V->setup_ib();
+      std::unique_ptr<Level> V = new_level(l);
       std::vector<double> m0(4 * (size_t)V->ncells);
       for (int n = 0; n < 4; ++n) std::fill(m0.begin() + n * V->ncells, m0.begin() + (n + 1) * V->ncells, st[n]);
       V->init(m0.data());
       if (l > 0) V->ensure_exports();
-      L.push_back(std::move(V));
+      L[l] = std::move(V);
     }
     // macro_old = macro_new on the windows, derived fields of both
     for (int l = 0; l < nlev - 1; ++l) {
@@ -168,6 +216,26 @@ struct Amr {
     ncol = F.ib ? F.ncol : 0;
     pick_amr(L[0]->lat, L[0]->st, cffn, avgfn);
     HIPCHK(hipStreamSynchronize(stream));
+  }
+
+  // before the first step: the derived fields of the initial state fill
+  // both slots (InitData's UpdateDerivedMacroFields, macro_old = macro_new)
+  void set_regrid(const ilbm_regrid_params& p) {
+    if (p.blocking_factor < 2 || p.blocking_factor % 2 != 0)
+      throw StatusError{ILBM_ERR_INVALID, "blocking_factor must be even and >= 2"};
+    if (p.n_error_buf < 0) throw StatusError{ILBM_ERR_INVALID, "n_error_buf < 0"};
+    if (istep != 0) throw StatusError{ILBM_ERR_STATE, "regridding must be switched on before the first advance"};
+    for (int l = 0; l < nlev - 1; ++l) {
+      Level& V = *L[l];
+      if (!V.mout) throw StatusError{ILBM_ERR_UNSUPPORTED, "level too large for an exported initial state"};
+      ensure_dfull(l);
+      patch_derived(l, V.t_new, Dfull[l][0]);
+      HIPCHK(hipMemcpyAsync(Dfull[l][1], Dfull[l][0], sizeof(double) * 2 * V.ncells, hipMemcpyDeviceToDevice,
+                            stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    rg = p;
+    regrid_on = true;
   }
 
   void derived(int l, int sel) {
@@ -245,8 +313,326 @@ struct Amr {
     avgfn(dim3((unsigned)dblocks[l]), stream, a);
   }
 
-  void substep(int l, double time, bool last_chain, int row, bool export_state) {
+  // ---------------- regrid ----------------
+  // FillPatchMacro(l, time) of the 4 macro components with one ghost layer
+  // from the level's exported macros (physical / periodic ghosts on level
+  // 0, coarse interpolation on refined levels), then visPara on every
+  // valid cell into D[2][cells]
+  void patch_derived(int l, double time, double* D) {
     Level& V = *L[l];
+    if (!V.mout) throw StatusError{ILBM_ERR_STATE, "level macros not exported"};
+    PatchFab P{};
+    for (int d = 0; d < 3; ++d) {
+      const int g = (d < dim) ? 1 : 0;
+      P.lo[d] = -g;
+      P.hi[d] = V.c.n[d] - 1 + g;
+    }
+    P.js = P.hi[0] - P.lo[0] + 1;
+    P.ks = P.js * (P.hi[1] - P.lo[1] + 1);
+    P.ns = P.ks * (P.hi[2] - P.lo[2] + 1);
+    if (4 * P.ns > patch_cap[l]) {
+      dev_free(patch[l]);
+      patch[l] = dev_alloc<double>(4 * P.ns);
+      patch_cap[l] = 4 * P.ns;
+    }
+    P.p = patch[l];
+    Fab F{};
+    F.p = P.p;
+    for (int d = 0; d < 3; ++d) { F.lo[d] = P.lo[d]; F.hi[d] = P.hi[d]; }
+    F.js = P.js; F.ks = P.ks; F.ns = P.ns; F.nc = 4;
+    const int64_t cells = V.ncells;
+    hipLaunchKernelGGL(k_fab_from_cells, grid_for(cells), dim3(256), 0, stream, F, (const double*)V.mout, V.c.n[0],
+                       V.c.n[1], cells);
+    if (l == 0) {
+      const BoxGeom G = box_geom(V.c, nullptr, nullptr);
+      hipLaunchKernelGGL(k_box_fill_periodic, grid_for(P.ns), dim3(256), 0, stream, F, G);
+      with_lattice(V.lat, [&](auto lat) {
+        using LAT = decltype(lat);
+        for (int pass = 1; pass <= dim; ++pass)
+          hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(P.ns), dim3(256), 0, stream, F, G, pass, 0);
+      });
+    } else {
+      PatchCfArgs a{};
+      a.P = P;
+      for (int d = 0; d < 3; ++d) { a.n[d] = V.c.n[d]; a.ioff[d] = (d < dim) ? V.ioff[d] : 0; }
+      a.dim = dim;
+      window_at(l - 1, time, a.W);
+      hipLaunchKernelGGL(k_patch_cf, grid_for(P.ns), dim3(256), 0, stream, a);
+    }
+    hipLaunchKernelGGL(k_patch_derived, grid_for(cells), dim3(256), 0, stream, P, D, V.c.n[0], V.c.n[1], cells, dim,
+                       V.c.dx[0], V.c.dx[1], V.c.dx[2]);
+    HIPCHK(hipGetLastError());
+  }
+
+  void ensure_dfull(int l) {
+    if (!Dfull[l][0]) {
+      Dfull[l][0] = dev_alloc<double>(2 * L[l]->ncells);
+      Dfull[l][1] = dev_alloc<double>(2 * L[l]->ncells);
+    }
+  }
+
+  // immersed_boundary_refinement_region (AmrCoreLBM.cpp:60-104); the user
+  // (pitching) section is Examples/Pitching_airfoil/IBMUserDefinedGeometry.H:
+  // 25-57 with the vertical reach of the point farthest from the pivot
+  bool refine_region(double dx, double lo[3], double hi[3]) const {
+    const ilbm_case_desc& c = d0;
+    const double pad = std::max(6.0 * dx, 2.0);
+    if (c.geometry_type == 1) {
+      lo[0] = c.box_lo[0] - pad; hi[0] = c.box_hi[0] + pad;
+      lo[1] = (c.box_lid_only > 0) ? (c.box_hi[1] - pad) : (c.box_lo[1] - pad);
+      hi[1] = c.box_hi[1] + pad;
+      lo[2] = c.box_lo[2] - pad; hi[2] = c.box_hi[2] + pad;
+      return true;
+    }
+    if (c.geometry_type == 2) {
+      if (c.dim != 2) return false;
+      const double chord = std::max(c.airfoil_chord, 1.0e-12);
+      const double ht = 0.5 * c.airfoil_thickness * chord;
+      const double arm = std::max(c.pivot_x, chord - c.pivot_x);
+      const double yext = arm * std::sin(std::fabs(c.pitch_amplitude)) + ht;
+      lo[0] = c.x0 - pad; hi[0] = c.x0 + chord + pad;
+      lo[1] = c.y0 - yext - pad; hi[1] = c.y0 + yext + pad;
+      lo[2] = c.z0 - pad; hi[2] = c.z0 + pad;
+      return true;
+    }
+    if (c.R > 0.0) {
+      const bool ext = rg.refine_upstream > 0.0 && rg.refine_downstream > 0.0 && rg.refine_cross > 0.0;
+      const double up = ext ? rg.refine_upstream * c.R : c.R, dn = ext ? rg.refine_downstream * c.R : c.R;
+      const double cr = ext ? rg.refine_cross * c.R : c.R;
+      lo[0] = c.x0 - up - pad; hi[0] = c.x0 + dn + pad;
+      lo[1] = c.y0 - cr - pad; hi[1] = c.y0 + cr + pad;
+      lo[2] = c.z0 - cr - pad; hi[2] = c.z0 + cr + pad;
+      return true;
+    }
+    return false;
+  }
+
+  // ErrorEst on level l at `time`: tag count and the bounding box of the
+  // tags grown by n_error_buf (level-l global indices)
+  int64_t tag_bbox(int l, double time, int tlo[3], int thi[3]) {
+    Level& V = *L[l];
+    const int64_t cells = V.ncells;
+    if (2 * cells > scratch_cap) {
+      dev_free(scratch);
+      scratch = dev_alloc<double>(2 * cells);
+      scratch_cap = 2 * cells;
+    }
+    if (!tag_d) tag_d = dev_alloc<int>(8);
+    if (!vmax_d) vmax_d = dev_alloc<unsigned long long>(1);
+    patch_derived(l, time, scratch);
+    const int init[8] = {0, 1 << 30, 1 << 30, 1 << 30, -(1 << 30), -(1 << 30), -(1 << 30), 0};
+    HIPCHK(hipMemcpyAsync(tag_d, init, sizeof(init), hipMemcpyHostToDevice, stream));
+    const unsigned long long zero = 0;
+    HIPCHK(hipMemcpyAsync(vmax_d, &zero, sizeof(zero), hipMemcpyHostToDevice, stream));
+    hipLaunchKernelGGL(k_vort_max, grid_for(cells), dim3(256), 0, stream, (const double*)scratch, cells, vmax_d);
+    TagArgs a{};
+    a.vort = scratch;
+    a.vmax_key = vmax_d;
+    for (int d = 0; d < 3; ++d) {
+      a.n[d] = V.c.n[d];
+      a.ioff[d] = (d < dim) ? V.ioff[d] : 0;
+      a.plo[d] = d0.plo[d];
+      a.dx[d] = V.c.dx[d];
+    }
+    a.dim = dim;
+    a.cells = cells;
+    a.ratio = rg.threshold_ratio[std::min(l, 3)];
+    a.probhi0 = d0.plo[0] + d0.dx[0] * (double)d0.n[0];
+    a.nr_lo = rg.no_refine_xlo;
+    a.nr_hi = rg.no_refine_xhi;
+    a.region = d0.use_ib && refine_region(V.c.dx[0], a.rlo, a.rhi);
+    a.lset = d0.use_ib && d0.geometry_type == 0;
+    a.x0 = d0.x0; a.y0 = d0.y0; a.z0 = d0.z0; a.R = d0.R;
+    a.band = (5.0 + 0.5) * V.c.dx[0];
+    a.buf = rg.n_error_buf;
+    a.out = tag_d;
+    hipLaunchKernelGGL(k_tag_bbox, grid_for(cells), dim3(256), 0, stream, a);
+    HIPCHK(hipGetLastError());
+    int out[8];
+    HIPCHK(hipMemcpyAsync(out, tag_d, sizeof(out), hipMemcpyDeviceToHost, stream));
+    HIPCHK(hipStreamSynchronize(stream));
+    for (int d = 0; d < 3; ++d) {
+      tlo[d] = (d < dim) ? out[1 + d] : 0;
+      thi[d] = (d < dim) ? out[4 + d] : 0;
+    }
+    return out[0];
+  }
+
+  static int fdiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+  // one box for level l+1 (level-(l+1) indices) from a tag bounding box in
+  // level-l indices: clipped to the parent box shrunk by 3 cells, aligned
+  // outward to the blocking unit, inward where that leaves the clip region
+  bool box_from_tags(const int tlo[3], const int thi[3], const int plo[3], const int phi[3], int bu, int olo[3],
+                     int ohi[3]) const {
+    for (int d = 0; d < 3; ++d) { olo[d] = 0; ohi[d] = 0; }
+    for (int d = 0; d < dim; ++d) {
+      const int nlo = plo[d] + 3, nhi = phi[d] - 3;
+      const int lo = std::max(tlo[d], nlo), hi = std::min(thi[d], nhi);
+      if (lo > hi) return false;
+      int alo = fdiv(lo, bu) * bu, ahi = (fdiv(hi, bu) + 1) * bu - 1;
+      if (alo < nlo) alo = -fdiv(-nlo, bu) * bu;
+      if (ahi > nhi) ahi = fdiv(nhi + 1, bu) * bu - 1;
+      if (alo > ahi) return false;
+      olo[d] = 2 * alo;
+      ohi[d] = 2 * ahi + 1;
+    }
+    return true;
+  }
+
+  // crop of the window box of level l out of a [nc][level cells] array
+  void crop_window(int l, const double* src, double* dst, int nc) {
+    Level& V = *L[l];
+    const int64_t wc = V.band_cells;
+    hipLaunchKernelGGL(k_crop, grid_for(wc), dim3(256), 0, stream, src, V.c.n[0], V.c.n[1], (int64_t)V.ncells, dst,
+                       (int)V.band_n[0], (int)V.band_n[1], wc, (int)V.band_lo[0], (int)V.band_lo[1],
+                       (int)V.band_lo[2], nc);
+  }
+
+  // window of level l (not re-boxed) moved to the new box of level l+1:
+  // macro_new from the level's exported macros, the derived fields of the
+  // last two steps (macro_new's, and macro_old's stale ones)
+  void move_window(int l) {
+    Level& V = *L[l];
+    alloc_window(l);
+    crop_window(l, V.mout, V.Mw[V.wsel], 4);
+    crop_window(l, V.mout, V.Mw[V.wsel ^ 1], 4);
+    crop_window(l, Dfull[l][V.istep & 1], V.Dw[V.wsel], 2);
+    crop_window(l, Dfull[l][(V.istep + 1) & 1], V.Dw[V.wsel ^ 1], 2);
+    HIPCHK(hipGetLastError());
+  }
+
+  // RemakeLevel (AmrCoreLBM.cpp:835-914) of level l on its new box
+  void remake(int l, double time) {
+    Level& O = *L[l];
+    const int nmac = d0.nmac;
+    std::unique_ptr<Level> N = new_level(l);
+    double* M = dev_alloc<double>((size_t)nmac * N->ncells);
+    RemakeArgs ra{};
+    ra.M = M;
+    for (int d = 0; d < 3; ++d) {
+      ra.n[d] = N->c.n[d];
+      ra.lo[d] = (d < dim) ? blo[l][d] : 0;
+      ra.on[d] = O.c.n[d];
+      ra.olo[d] = (d < dim) ? O.ioff[d] : 0;
+    }
+    if (!O.mout) throw StatusError{ILBM_ERR_STATE, "level macros not exported"};
+    ra.om = O.mout;
+    ra.od = O.window ? Dfull[l][O.istep & 1] : nullptr;
+    ra.dim = dim;
+    ra.nmac = nmac;
+    window_at(l - 1, time, ra.W);
+    hipLaunchKernelGGL(k_remake_macro, grid_for(N->ncells), dim3(256), 0, stream, ra);
+    HIPCHK(hipGetLastError());
+    N->init_dev(M);   // synchronizes: the old level's derived fields are free to go
+    for (int b = 0; b < 2; ++b) {
+      dev_free(Dfull[l][b]);
+      Dfull[l][b] = nullptr;
+    }
+    N->ensure_exports();
+    HIPCHK(hipMemcpyAsync(N->mout, M, sizeof(double) * 4 * N->ncells, hipMemcpyDeviceToDevice, stream));
+    N->t_new = O.t_new;
+    N->t_old = O.t_old;
+    N->istep = O.istep;
+    if (N->ib && O.ib) {   // the force record carries on (ComputeIBForce state is the hierarchy's)
+      N->grow_histories(1, O.force_cap);
+      if (O.force_cap && O.force_dev)
+        HIPCHK(hipMemcpyAsync(N->force_dev, O.force_dev, sizeof(double) * (size_t)O.force_cap * O.ncol,
+                              hipMemcpyDeviceToDevice, stream));
+      HIPCHK(hipMemcpyAsync(N->frstate_d, O.frstate_d, sizeof(double) * 4, hipMemcpyDeviceToDevice, stream));
+    }
+    L[l] = std::move(N);   // the old level's memory is released here
+    Level& V = *L[l];
+    if (V.window) {
+      ensure_dfull(l);
+      crop_window(l, M, V.Mw[V.wsel], 4);
+      crop_window(l, M, V.Mw[V.wsel ^ 1], 4);
+      derived(l, V.wsel);                 // UpdateDerivedMacroFields of macro_new
+      if (nmac > 4) crop_window(l, M + 4 * V.ncells, V.Dw[V.wsel ^ 1], 2);   // macro_old = FillPatch copy
+      patch_derived(l, time, Dfull[l][V.istep & 1]);
+      if (nmac > 4)
+        HIPCHK(hipMemcpyAsync(Dfull[l][(V.istep + 1) & 1], M + 4 * V.ncells, sizeof(double) * 2 * V.ncells,
+                              hipMemcpyDeviceToDevice, stream));
+    }
+    HIPCHK(hipStreamSynchronize(stream));
+    dev_free(M);
+  }
+
+  // regrid(lbase, time) (AmrCore::regrid with ErrorEst and RemakeLevel):
+  // boxes of levels lbase+1.. made finest first from the tags of their
+  // parent (old data) and the new child footprint grown by 3, then the
+  // changed levels remade coarse to fine
+  void regrid(int lbase, double time) {
+    const int bu = rg.blocking_factor / 2;
+    int tlo[kMaxLev][3], thi[kMaxLev][3];
+    int64_t tcnt[kMaxLev] = {0, 0, 0};
+    for (int l = lbase; l < nlev - 1; ++l) tcnt[l] = tag_bbox(l, time, tlo[l], thi[l]);
+    int nlo[kMaxLev][3], nhi[kMaxLev][3];
+    for (int l = 0; l < nlev; ++l)
+      for (int d = 0; d < 3; ++d) { nlo[l][d] = blo[l][d]; nhi[l][d] = bhi[l][d]; }
+    for (int lf = nlev - 1; lf > lbase; --lf) {
+      const int lc = lf - 1;
+      int qlo[3], qhi[3];
+      bool have = tcnt[lc] > 0;
+      for (int d = 0; d < 3; ++d) { qlo[d] = tlo[lc][d]; qhi[d] = thi[lc][d]; }
+      if (lf < nlev - 1) {
+        for (int d = 0; d < dim; ++d) {
+          const int clo_ = fdiv(fdiv(nlo[lf + 1][d], 2) - 3, 2), chi_ = fdiv(fdiv(nhi[lf + 1][d], 2) + 3, 2);
+          if (!have || clo_ < qlo[d]) qlo[d] = clo_;
+          if (!have || chi_ > qhi[d]) qhi[d] = chi_;
+        }
+        if (!have)
+          for (int d = 0; d < dim; ++d) {
+            qlo[d] = std::min(qlo[d], fdiv(nlo[lf][d], 2));
+            qhi[d] = std::max(qhi[d], fdiv(nhi[lf][d], 2));
+          }
+        have = true;
+      }
+      if (!have) continue;
+      int b0[3], b1[3];
+      if (box_from_tags(qlo, qhi, nlo[lc], nhi[lc], bu, b0, b1))
+        for (int d = 0; d < 3; ++d) { nlo[lf][d] = b0[d]; nhi[lf][d] = b1[d]; }
+    }
+    for (int lf = lbase + 2; lf < nlev; ++lf) {
+      int c0[3], c1[3], b0[3], b1[3];
+      for (int d = 0; d < 3; ++d) { c0[d] = fdiv(nlo[lf][d], 2); c1[d] = fdiv(nhi[lf][d], 2); }
+      if (box_from_tags(c0, c1, nlo[lf - 1], nhi[lf - 1], bu, b0, b1))
+        for (int d = 0; d < 3; ++d) { nlo[lf][d] = b0[d]; nhi[lf][d] = b1[d]; }
+    }
+    bool changed[kMaxLev] = {false, false, false};
+    bool any = false;
+    for (int l = lbase + 1; l < nlev; ++l)
+      for (int d = 0; d < 3; ++d)
+        if (nlo[l][d] != blo[l][d] || nhi[l][d] != bhi[l][d]) changed[l] = true;
+    for (int l = lbase + 1; l < nlev; ++l) any = any || changed[l];
+    if (!any) return;
+    for (int l = lbase + 1; l < nlev; ++l)
+      for (int d = 0; d < 3; ++d) { blo[l][d] = nlo[l][d]; bhi[l][d] = nhi[l][d]; }
+    nest_geometry();
+    for (int l = lbase + 1; l < nlev; ++l) {
+      if (!changed[l]) continue;
+      if (!changed[l - 1]) move_window(l - 1);
+      remake(l, time);
+    }
+    if (changed[nlev - 1] && L[nlev - 1]->ib) ncol = L[nlev - 1]->ncol;
+    ++nregrid;
+  }
+
+  void substep(int l, double time, bool last_chain, int row, bool export_state) {
+    if (regrid_on && l < nlev - 1 && L[l]->istep > last_regrid[l] && L[l]->istep % regrid_int == 0) {
+      regrid(l, time);
+      for (int k = l; k < nlev; ++k) last_regrid[k] = L[k]->istep;
+    }
+    Level& V = *L[l];
+    // the two steps before a possible regrid of this level keep their
+    // macros and whole-level derived fields (tags, window moves, remakes)
+    const int nxt = V.istep + 1;
+    const bool pre = regrid_on && l < nlev - 1 && (nxt % regrid_int == 0 || (nxt + 1) % regrid_int == 0);
+    if (pre) {
+      V.ensure_exports();
+      ensure_dfull(l);
+    }
+    export_state = export_state || pre;
     V.t_old = V.t_new;
     V.t_new += V.dt;
     if (V.window) V.wsel ^= 1;  // swap(macro_new, macro_old)
@@ -276,6 +662,7 @@ struct Amr {
       average_down(l);
       derived(l, V.wsel);
     }
+    if (pre) patch_derived(l, V.t_new, Dfull[l][V.istep & 1]);
     HIPCHK(hipGetLastError());
   }
 
@@ -287,8 +674,7 @@ struct Amr {
       mass_cap = std::max(nsteps, 2 * mass_cap);
       mass_dev = dev_alloc<double>(mass_cap);
     }
-    Level& F = *L[nlev - 1];
-    F.grow_histories(1, nsteps);
+    L[nlev - 1]->grow_histories(1, nsteps);
     Level& C0 = *L[0];
     if (want_state) C0.ensure_exports();
     for (int s = 0; s < nsteps; ++s) {
@@ -305,6 +691,7 @@ struct Amr {
     std::vector<double> mh(nsteps);
     HIPCHK(hipMemcpyAsync(mh.data(), mass_dev, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
     std::vector<double> fh;
+    const Level& F = *L[nlev - 1];
     if (ncol) {
       fh.resize((size_t)nsteps * ncol);
       HIPCHK(hipMemcpyAsync(fh.data(), F.force_dev, sizeof(double) * nsteps * ncol, hipMemcpyDeviceToHost, stream));
@@ -404,6 +791,23 @@ int ilbm_amr_num_markers(ilbm_amr* h, int* out) {
   if (!h || !out) return ILBM_ERR_INVALID;
   const ilbm::Level& F = *h->A.L[h->A.nlev - 1];
   *out = F.ib ? F.nm : 0;
+  return ILBM_OK;
+}
+
+int ilbm_amr_set_regrid(ilbm_amr* h, const ilbm_regrid_params* p) {
+  if (!h || !p) return ILBM_ERR_INVALID;
+  return guarded([&] { h->A.set_regrid(*p); });
+}
+
+int ilbm_amr_boxes(ilbm_amr* h, int* lo, int* hi, int* n_regrids) {
+  if (!h) return ILBM_ERR_INVALID;
+  const ilbm::Amr& A = h->A;
+  for (int l = 1; l < A.nlev; ++l)
+    for (int d = 0; d < 3; ++d) {
+      if (lo) lo[3 * (l - 1) + d] = A.blo[l][d];
+      if (hi) hi[3 * (l - 1) + d] = A.bhi[l][d];
+    }
+  if (n_regrids) *n_regrids = A.nregrid;
   return ILBM_OK;
 }
 

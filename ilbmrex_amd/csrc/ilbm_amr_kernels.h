@@ -24,6 +24,16 @@
 
 namespace ilbm {
 
+// an Array4-style fab: comps planes over [lo, hi], x fastest
+struct PatchFab {
+  double* p;
+  int lo[3], hi[3];
+  int64_t js, ks, ns;
+  __device__ double& at(int i, int j, int k, int n) const {
+    return p[(int64_t)(i - lo[0]) + (int64_t)(j - lo[1]) * js + (int64_t)(k - lo[2]) * ks + (int64_t)n * ns];
+  }
+};
+
 struct Window {
   const double* Mo;   // [4][cells] macro_old
   const double* Mn;   // [4][cells] macro_new
@@ -48,7 +58,8 @@ __device__ inline double window_val(const Window& W, int i, int j, int k, int co
 __device__ inline int floordiv2(int i) { return (i >= 0) ? i / 2 : -((1 - i) / 2); }
 
 // cell_cons_interp at fine global cell (I, J, K): slopes limited per
-// direction with one factor shared by the ncomp components.
+// direction with one factor shared by the ncomp components; all ncomp
+// components are written to out.
 __device__ inline void cons_interp(const Window& W, int dim, int ncomp, int I, int J, int K, double* out) {
   const int ic = floordiv2(I), jc = floordiv2(J), kc = (dim == 3) ? floordiv2(K) : 0;
   double u0[6], s[3][6];
@@ -72,7 +83,7 @@ __device__ inline void cons_interp(const Window& W, int dim, int ncomp, int I, i
   const double ox = ((double)(I - ic * 2) + 0.5) / 2.0 - 0.5;
   const double oy = ((double)(J - jc * 2) + 0.5) / 2.0 - 0.5;
   const double oz = (dim == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
-  for (int n = 0; n < 4; ++n) {
+  for (int n = 0; n < ncomp; ++n) {
     double v = u0[n] + ox * (s[0][n] * sf[0]);
     v = v + oy * (s[1][n] * sf[1]);
     if (dim == 3) v = v + oz * (s[2][n] * sf[2]);
@@ -109,7 +120,7 @@ __global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
       r /= n;
     }
   }
-  double m[4];
+  double m[6];
   cons_interp(a.W, LAT::D, a.ncomp, iv[0] + a.ioff[0], iv[1] + a.ioff[1], iv[2] + a.ioff[2], m);
   // FillPatchMesoscopic's equilibrium (AmrCoreLBM.cpp:1605-1620)
   const double r = fmax(m[0], 1.0e-12);
@@ -247,6 +258,181 @@ __global__ void __launch_bounds__(256) k_derived(DerArgs a) {
   }
   a.D[w] = vort;
   a.D[a.cells + w] = P;
+}
+
+// ---------------------------------------------------------------------------
+// Regrid (ErrorEst AmrCoreLBM.cpp:986-1180, RemakeLevel :835-914)
+// ---------------------------------------------------------------------------
+
+// FillPatchMacro of a refined level with one ghost layer, the ghost cells:
+// cell_cons_interp of the 4 macro components from the coarser level's
+// window (FillPatchTwoLevels).  P holds the level in local indices.
+struct PatchCfArgs {
+  PatchFab P;
+  int n[3];          // valid extents
+  int ioff[3];       // global index of local cell 0
+  int dim;
+  Window W;
+};
+
+__global__ void k_patch_cf(PatchCfArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.P.ns) return;
+  const int i = a.P.lo[0] + (int)(t % a.P.js);
+  const int j = a.P.lo[1] + (int)((t / a.P.js) % (a.P.hi[1] - a.P.lo[1] + 1));
+  const int k = a.P.lo[2] + (int)(t / a.P.ks);
+  if (i >= 0 && i < a.n[0] && j >= 0 && j < a.n[1] && k >= 0 && k < a.n[2]) return;
+  double m[4];
+  cons_interp(a.W, a.dim, 4, i + a.ioff[0], j + a.ioff[1], k + a.ioff[2], m);
+  for (int c = 0; c < 4; ++c) a.P.p[c * a.P.ns + t] = m[c];
+}
+
+// vorticity and pressure (visPara.H:10-54) of every valid cell of a level
+// from its one-ghost macro patch, into D[2][cells]
+__global__ void k_patch_derived(PatchFab P, double* D, int nx, int ny, int64_t cells, int dim, double dx, double dy,
+                                double dz) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int i = (int)(t % nx), j = (int)((t / nx) % ny), k = (int)(t / ((int64_t)nx * ny));
+  const double Pr = (P.at(i, j, k, 0) - 1.0) / 3.0;
+  const double dv_dx = (P.at(i + 1, j, k, 2) - P.at(i - 1, j, k, 2)) / (2.0 * dx);
+  const double du_dy = (P.at(i, j + 1, k, 1) - P.at(i, j - 1, k, 1)) / (2.0 * dy);
+  double vort;
+  if (dim == 3) {
+    const double du_dz = (P.at(i, j, k + 1, 1) - P.at(i, j, k - 1, 1)) / (2.0 * dz);
+    const double dw_dx = (P.at(i + 1, j, k, 3) - P.at(i - 1, j, k, 3)) / (2.0 * dx);
+    const double dv_dz = (P.at(i, j, k + 1, 2) - P.at(i, j, k - 1, 2)) / (2.0 * dz);
+    const double dw_dy = (P.at(i, j + 1, k, 3) - P.at(i, j - 1, k, 3)) / (2.0 * dy);
+    const double ox = dw_dy - dv_dz, oy = du_dz - dw_dx, oz = dv_dx - du_dy;
+    vort = sqrt(ox * ox + oy * oy + oz * oz);
+  } else {
+    vort = dv_dx - du_dy;
+  }
+  D[t] = vort;
+  D[cells + t] = Pr;
+}
+
+// order-preserving 64-bit key of a double (NaN aside), for atomicMax
+__device__ inline unsigned long long order_key(double x) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(x);
+  return (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+}
+__device__ inline double key_value(unsigned long long k) {
+  const unsigned long long u = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)u);
+}
+
+// MultiFab::max of the vorticity: block maxima folded with atomicMax
+__global__ void __launch_bounds__(256) k_vort_max(const double* v, int64_t cells, unsigned long long* key) {
+  __shared__ double lds[4];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double m = (t < cells) ? v[t] : -HUGE_VAL;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  if ((threadIdx.x & 63) == 0) lds[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double b = fmax(fmax(lds[0], lds[1]), fmax(lds[2], lds[3]));
+    atomicMax(key, order_key(b));
+  }
+}
+
+// ErrorEst's tags on one level (valid cells): |vorticity| > ratio * max,
+// the level-set band of a cylinder / sphere (phi = r - R < 5.5 dx), the
+// body refinement region, all cleared in the no-refine x bands.  Output:
+// tag count and the bounding box of the tags grown by n_error_buf, in
+// level-global indices (atomics on ints: order free).
+struct TagArgs {
+  const double* vort;   // [cells]
+  const unsigned long long* vmax_key;
+  int n[3], ioff[3], dim;
+  int64_t cells;
+  double ratio;
+  double plo[3], dx[3];
+  double probhi0;
+  double nr_lo, nr_hi;  // no_refine_xlo / xhi
+  int region;           // body refinement box in use
+  double rlo[3], rhi[3];
+  int lset;             // level-set band of the cylinder / sphere
+  double x0, y0, z0, R, band;
+  int buf;
+  int* out;             // [0] count, [1..3] lo, [4..6] hi
+};
+
+__global__ void k_tag_bbox(TagArgs a) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= a.cells) return;
+  const int i = a.ioff[0] + (int)(t % a.n[0]);
+  const int j = a.ioff[1] + (int)((t / a.n[0]) % a.n[1]);
+  const int k = a.ioff[2] + (int)(t / ((int64_t)a.n[0] * a.n[1]));
+  const double x = a.plo[0] + ((double)i + 0.5) * a.dx[0];
+  const double y = a.plo[1] + ((double)j + 0.5) * a.dx[1];
+  const double z = (a.dim == 3) ? a.plo[2] + ((double)k + 0.5) * a.dx[2] : 0.0;
+  if (a.nr_lo > 0.0 || a.nr_hi > 0.0) {
+    const double xlo_block = a.plo[0] + a.nr_lo, xhi_block = a.probhi0 - a.nr_hi;
+    if (x <= xlo_block || x >= xhi_block) return;
+  }
+  const double thr = a.ratio * key_value(*a.vmax_key);
+  bool tag = fabs(a.vort[t]) > thr;
+  if (a.lset) {
+    double r2 = (x - a.x0) * (x - a.x0) + (y - a.y0) * (y - a.y0);
+    if (a.dim == 3) r2 += (z - a.z0) * (z - a.z0);
+    if (sqrt(r2) - a.R < a.band) tag = true;
+  }
+  if (a.region && x >= a.rlo[0] && x <= a.rhi[0] && y >= a.rlo[1] && y <= a.rhi[1] && z >= a.rlo[2] &&
+      z <= a.rhi[2])
+    tag = true;
+  if (!tag) return;
+  atomicAdd(&a.out[0], 1);
+  const int iv[3] = {i, j, k};
+  for (int d = 0; d < a.dim; ++d) {
+    atomicMin(&a.out[1 + d], iv[d] - a.buf);
+    atomicMax(&a.out[4 + d], iv[d] + a.buf);
+  }
+}
+
+// RemakeLevel's FillPatchMacro over the new box (valid cells): the old
+// level's macro_new where it was valid (comps 0-3 from its export, 4-5
+// from its derived fields), cell_cons_interp of all nmac components from
+// the coarser level's window elsewhere.
+struct RemakeArgs {
+  double* M;            // [nmac][new cells]
+  int n[3], lo[3];      // new box extents and global lo
+  const double* om;     // old [4][old cells]
+  const double* od;     // old [2][old cells] or null
+  int on[3], olo[3];
+  int dim, nmac;
+  Window W;
+};
+
+__global__ void k_remake_macro(RemakeArgs a) {
+  const int64_t cells = (int64_t)a.n[0] * a.n[1] * a.n[2];
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= cells) return;
+  const int I = a.lo[0] + (int)(t % a.n[0]);
+  const int J = a.lo[1] + (int)((t / a.n[0]) % a.n[1]);
+  const int K = a.lo[2] + (int)(t / ((int64_t)a.n[0] * a.n[1]));
+  const int oi = I - a.olo[0], oj = J - a.olo[1], ok = K - a.olo[2];
+  if (oi >= 0 && oi < a.on[0] && oj >= 0 && oj < a.on[1] && ok >= 0 && ok < a.on[2]) {
+    const int64_t ocells = (int64_t)a.on[0] * a.on[1] * a.on[2];
+    const int64_t o = ((int64_t)ok * a.on[1] + oj) * a.on[0] + oi;
+    for (int c = 0; c < 4; ++c) a.M[c * cells + t] = a.om[c * ocells + o];
+    for (int c = 4; c < a.nmac; ++c) a.M[c * cells + t] = a.od ? a.od[(c - 4) * ocells + o] : 0.0;
+    return;
+  }
+  double m[6];
+  cons_interp(a.W, a.dim, a.nmac, I, J, K, m);
+  for (int c = 0; c < a.nmac; ++c) a.M[c * cells + t] = m[c];
+}
+
+// copy of a box of a [nc][src cells] array into a [nc][dst cells] array
+// (dst box inside the src box; offsets in src indices)
+__global__ void k_crop(const double* src, int sn0, int sn1, int64_t scells, double* dst, int dn0, int dn1,
+                       int64_t dcells, int o0, int o1, int o2, int nc) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= dcells) return;
+  const int i = (int)(t % dn0), j = (int)((t / dn0) % dn1), k = (int)(t / ((int64_t)dn0 * dn1));
+  const int64_t s = ((int64_t)(k + o2) * sn1 + (j + o1)) * sn0 + (i + o0);
+  for (int c = 0; c < nc; ++c) dst[c * dcells + t] = src[c * scells + s];
 }
 
 // Sum(Rho) of level 0 after the average-down: the stream-collide kernel's

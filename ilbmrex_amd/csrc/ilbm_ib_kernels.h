@@ -197,15 +197,13 @@ __global__ void k_ib_begin(IbArgs a) { ib_begin_body(a, (int64_t)blockIdx.x * bl
 
 // cell index and 5-point weights of every marker, and the partition sums
 // that the optional renormalisation uses (:640-690, :1358-1403)
-__global__ void k_ib_weights(IbArgs a) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= a.nm) return;
+__device__ inline void marker_weights(const IbArgs& a, int m, const double pos3[3], int b[3]) {
   const int nm = a.nm;
   double t[3][5];
-  int b[3] = {0, 0, 0};
+  b[0] = b[1] = b[2] = 0;
   for (int d = 0; d < 3; ++d) {
     if (d < a.dim) {
-      const double pos = a.xm[d * nm + m];
+      const double pos = pos3[d];
       b[d] = (int)floor((pos - a.plo[d]) / a.dx[d]) - a.ioff[d];
       for (int s = 0; s < 5; ++s) {
         const double xi = a.plo[d] + ((double)(b[d] + a.ioff[d] + s - 2) + 0.5) * a.dx[d];
@@ -237,24 +235,13 @@ __global__ void k_ib_weights(IbArgs a) {
   a.scale[nm + m] = (a.renorm_spread && part > 0.0) ? 1.0 / part : 1.0;  // spreading factor
 }
 
-// cell bounds of every chunk of kMarkerChunk consecutive markers (moving
-// bodies), so the gather spreading visits only the chunks that reach a cell
-__global__ void k_ib_chunk_bounds(IbArgs a) {
-  const int ch = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nch = (a.nm + kMarkerChunk - 1) / kMarkerChunk;
-  if (ch >= nch) return;
-  const int nm = a.nm;
-  const int m0 = ch * kMarkerChunk, m1 = (m0 + kMarkerChunk < nm) ? m0 + kMarkerChunk : nm;
-  for (int d = 0; d < 3; ++d) {
-    int lo = 1 << 30, hi = -(1 << 30);
-    for (int m = m0; m < m1; ++m) {
-      const int b = a.base[d * nm + m];
-      lo = b < lo ? b : lo;
-      hi = b > hi ? b : hi;
-    }
-    a.chunk_box[(2 * d) * nch + ch] = lo;
-    a.chunk_box[(2 * d + 1) * nch + ch] = hi;
-  }
+__global__ void k_ib_weights(IbArgs a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.nm) return;
+  double pos[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < a.dim; ++d) pos[d] = a.xm[d * a.nm + m];
+  int b[3];
+  marker_weights(a, m, pos, b);
 }
 
 __device__ inline void target_velocity(const IbArgs& a, int m, double* ux, double* uy, double* uz) {
@@ -598,19 +585,42 @@ struct MoveArgs {
   double px, py, pivot_x, cs, sn, omega, z;
 };
 
-__global__ void k_ib_move(MoveArgs a) {
+// One pass per step of a moving body: positions and target velocities
+// (k_ib_move), stencil cells and weights (k_ib_weights), and the cell bounds
+// of each kMarkerChunk-marker chunk (k_ib_chunk_bounds) from a 32-lane
+// min / max; blocks hold whole chunks.
+__global__ void __launch_bounds__(256) k_ib_move_weights(MoveArgs mv, IbArgs a) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
-  if (m >= a.nm) return;
   const int nm = a.nm;
-  const double xrr = a.xref[m] - a.pivot_x, yr = a.xref[nm + m];
-  const double x = a.px + a.cs * xrr - a.sn * yr;
-  const double y = a.py + a.sn * xrr + a.cs * yr;
-  a.xm[m] = x;
-  a.xm[nm + m] = y;
-  a.xm[2 * nm + m] = a.z;
-  a.ut[m] = -a.omega * (y - a.py);
-  a.ut[nm + m] = a.omega * (x - a.px);
-  a.ut[2 * nm + m] = 0.0;
+  const bool act = m < nm;
+  int b[3] = {0, 0, 0};
+  if (act) {
+    const double xrr = mv.xref[m] - mv.pivot_x, yr = mv.xref[nm + m];
+    const double x = mv.px + mv.cs * xrr - mv.sn * yr;
+    const double y = mv.py + mv.sn * xrr + mv.cs * yr;
+    mv.xm[m] = x;
+    mv.xm[nm + m] = y;
+    mv.xm[2 * nm + m] = mv.z;
+    mv.ut[m] = -mv.omega * (y - mv.py);
+    mv.ut[nm + m] = mv.omega * (x - mv.px);
+    mv.ut[2 * nm + m] = 0.0;
+    const double pos[3] = {x, y, mv.z};
+    marker_weights(a, m, pos, b);
+  }
+  const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
+  const int ch = m / kMarkerChunk;
+  for (int d = 0; d < 3; ++d) {
+    int lo = act ? b[d] : (1 << 30), hi = act ? b[d] : -(1 << 30);
+    for (int off = kMarkerChunk / 2; off > 0; off >>= 1) {
+      const int ol = __shfl_xor(lo, off, kMarkerChunk), oh = __shfl_xor(hi, off, kMarkerChunk);
+      lo = ol < lo ? ol : lo;
+      hi = oh > hi ? oh : hi;
+    }
+    if (act && (m % kMarkerChunk) == 0) {
+      a.chunk_box[(2 * d) * nch + ch] = lo;
+      a.chunk_box[(2 * d + 1) * nch + ch] = hi;
+    }
+  }
 }
 
 // Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
@@ -681,35 +691,101 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const doub
 // term of the coming collision on the same cell (collide_forced.H:22-50),
 // which the stream-collide kernel left out because F was not yet known.
 // Each block also sums its exported force for the step's force record.
-// Per band cell totals of a moving body's gather spreading: the markers
-// whose stencil covers the cell, in marker order (k_ib_spread_all's
-// arithmetic), visiting only the marker chunks whose cell bounds reach it.
-__device__ inline void gather_all_sum(const IbArgs& a, int64_t b, bool active, double tot[3]) {
+// Moving body: the band is covered by 16 x 16 tiles of (i, j) cells per k
+// plane, one block each.  The block stages the marker chunks whose stencil
+// bounds reach its tile into LDS (chunk order = marker order) and every
+// cell sums the markers whose 5-point stencil covers it, in marker order,
+// with k_ib_spread_all's arithmetic.
+constexpr int kGatherTile = 16;
+constexpr int kGatherStage = 256 / kMarkerChunk;   // chunks staged per pass
+
+struct GatherTile {
+  int bi, bj, bk;     // band-local cell of this thread
+  bool active;        // inside the band
+  int64_t b;          // band cell index (valid when active)
+  int i0, i1, j0, j1, k;   // the tile's cells in level-local indices (inclusive)
+};
+
+__device__ inline GatherTile gather_tile_of(const IbArgs& a, int blk) {
+  GatherTile t;
+  const int tx = (a.band_n[0] + kGatherTile - 1) / kGatherTile;
+  const int ty = (a.band_n[1] + kGatherTile - 1) / kGatherTile;
+  const int cx = blk % tx, cy = (blk / tx) % ty;
+  t.bk = blk / (tx * ty);
+  t.bi = cx * kGatherTile + (int)(threadIdx.x % kGatherTile);
+  t.bj = cy * kGatherTile + (int)(threadIdx.x / kGatherTile);
+  t.active = t.bi < a.band_n[0] && t.bj < a.band_n[1];
+  t.b = ((int64_t)t.bk * a.band_n[1] + t.bj) * a.band_n[0] + t.bi;
+  t.i0 = a.band_lo[0] + cx * kGatherTile;
+  t.i1 = t.i0 + kGatherTile - 1;
+  t.j0 = a.band_lo[1] + cy * kGatherTile;
+  t.j1 = t.j0 + kGatherTile - 1;
+  t.k = a.band_lo[2] + t.bk;
+  return t;
+}
+
+__device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, double tot[3]) {
+  __shared__ int s_list[kGatherStage];
+  __shared__ int s_n, s_next;
+  __shared__ int s_base[3][256];
+  __shared__ double s_w[15][256];
+  __shared__ double s_f[4][256];   // F * dv (3 comps), scale
   tot[0] = tot[1] = tot[2] = 0.0;
-  if (!active) return;
   const int nm = a.nm;
   const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
-  const int i = a.band_lo[0] + (int)(b % a.band_n[0]);
-  const int j = a.band_lo[1] + (int)((b / a.band_n[0]) % a.band_n[1]);
-  const int k = a.band_lo[2] + (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
   const bool d3 = (a.dim == 3);
-  for (int ch = 0; ch < nch; ++ch) {
-    if (i < a.chunk_box[ch] - 2 || i > a.chunk_box[nch + ch] + 2) continue;
-    if (j < a.chunk_box[2 * nch + ch] - 2 || j > a.chunk_box[3 * nch + ch] + 2) continue;
-    if (d3 && (k < a.chunk_box[4 * nch + ch] - 2 || k > a.chunk_box[5 * nch + ch] + 2)) continue;
-    const int m0 = ch * kMarkerChunk, m1 = (m0 + kMarkerChunk < nm) ? m0 + kMarkerChunk : nm;
-    for (int m = m0; m < m1; ++m) {
-      const int ii = i - a.base[m] + 2, jj = j - a.base[nm + m] + 2;
-      const int kk = d3 ? k - a.base[2 * nm + m] + 2 : 0;
-      if ((unsigned)ii > 4u || (unsigned)jj > 4u || (unsigned)kk > 4u) continue;
-      const double tU = a.wts[(0 * 5 + ii) * nm + m];
-      const double tV = a.wts[(1 * 5 + jj) * nm + m];
-      const double sc = a.scale[nm + m];
-      const double dval = d3 ? (tU * tV * a.wts[(2 * 5 + kk) * nm + m]) * sc : (tU * tV) * sc;
-      tot[0] += dval * (a.F[m] * a.dv);
-      tot[1] += dval * (a.F[nm + m] * a.dv);
-      if (d3) tot[2] += dval * (a.F[2 * nm + m] * a.dv);
+  const int i = a.band_lo[0] + t.bi, j = a.band_lo[1] + t.bj;
+  if (threadIdx.x == 0) s_next = 0;
+  __syncthreads();
+  while (true) {
+    if (threadIdx.x == 0) {
+      int n = 0, ch = s_next;
+      for (; ch < nch && n < kGatherStage; ++ch) {
+        if (t.i1 < a.chunk_box[ch] - 2 || t.i0 > a.chunk_box[nch + ch] + 2) continue;
+        if (t.j1 < a.chunk_box[2 * nch + ch] - 2 || t.j0 > a.chunk_box[3 * nch + ch] + 2) continue;
+        if (d3 && (t.k < a.chunk_box[4 * nch + ch] - 2 || t.k > a.chunk_box[5 * nch + ch] + 2)) continue;
+        s_list[n++] = ch;
+      }
+      s_n = n;
+      s_next = ch;
     }
+    __syncthreads();
+    const int n = s_n;
+    if (n == 0) break;
+    const int slot = threadIdx.x;
+    if (slot < n * kMarkerChunk) {
+      const int m = s_list[slot / kMarkerChunk] * kMarkerChunk + slot % kMarkerChunk;
+      if (m < nm) {
+        s_base[0][slot] = a.base[m];
+        s_base[1][slot] = a.base[nm + m];
+        s_base[2][slot] = d3 ? a.base[2 * nm + m] : 0;
+        for (int r = 0; r < (d3 ? 15 : 10); ++r) s_w[r][slot] = a.wts[r * nm + m];
+        s_f[0][slot] = a.F[m] * a.dv;
+        s_f[1][slot] = a.F[nm + m] * a.dv;
+        s_f[2][slot] = d3 ? a.F[2 * nm + m] * a.dv : 0.0;
+        s_f[3][slot] = a.scale[nm + m];
+      } else {
+        s_base[0][slot] = -(1 << 29);   // no cell is covered
+        s_base[1][slot] = 0;
+        s_base[2][slot] = 0;
+      }
+    }
+    __syncthreads();
+    if (t.active) {
+      for (int e = 0; e < n * kMarkerChunk; ++e) {
+        const int ii = i - s_base[0][e] + 2, jj = j - s_base[1][e] + 2;
+        const int kk = d3 ? t.k - s_base[2][e] + 2 : 0;
+        if ((unsigned)ii > 4u || (unsigned)jj > 4u || (unsigned)kk > 4u) continue;
+        const double tU = s_w[ii][e];
+        const double tV = s_w[5 + jj][e];
+        const double sc = s_f[3][e];
+        const double dval = d3 ? (tU * tV * s_w[10 + kk][e]) * sc : (tU * tV) * sc;
+        tot[0] += dval * s_f[0][e];
+        tot[1] += dval * s_f[1][e];
+        if (d3) tot[2] += dval * s_f[2][e];
+      }
+    }
+    __syncthreads();
   }
 }
 
@@ -723,11 +799,17 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   __shared__ double lds[3][4];
-  const int64_t b = (int64_t)blk * 256 + threadIdx.x;
+  int64_t b = (int64_t)blk * 256 + threadIdx.x;
+  bool active = b < a.band_cells;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double cell_tot[3] = {0.0, 0.0, 0.0};
-  if constexpr (MODE == 2) gather_all_sum(a, b, b < a.band_cells, cell_tot);
-  if (b < a.band_cells) {
+  if constexpr (MODE == 2) {
+    const GatherTile t = gather_tile_of(a, blk);
+    b = t.b;
+    active = t.active;
+    gather_tile_sum(a, t, cell_tot);
+  }
+  if (active) {
     const double rho = a.Mband[b];
     if constexpr (MODE == 0) {
       const double* tot = a.ev_from_macro ? a.EFI : a.EFT;

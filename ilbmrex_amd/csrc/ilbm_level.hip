@@ -621,7 +621,9 @@ struct Level {
     HIPCHK(hipMemsetAsync(U_d, 0, sizeof(double) * 3 * nm, stream));
     HIPCHK(hipMemsetAsync(Mm_d, 0, sizeof(double) * 3 * nm, stream));
     rowptr_d = dev_alloc<int>(band_cells + 1);
-    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + 2));
+    const int64_t tiles = (int64_t)((band_n[0] + kGatherTile - 1) / kGatherTile) *
+                          ((band_n[1] + kGatherTile - 1) / kGatherTile) * band_n[2];
+    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + tiles + 2));
     upload_markers();
     ncol = (c.dim == 3) ? 33 : 23;
   }
@@ -806,6 +808,9 @@ struct Level {
   }
   unsigned fix_blocks() const {
     if (fix_mode() == 3) return (unsigned)((n_ne + kSpreadWaves - 1) / kSpreadWaves);
+    if (fix_mode() == 2)
+      return (unsigned)(((band_n[0] + kGatherTile - 1) / kGatherTile) * ((band_n[1] + kGatherTile - 1) / kGatherTile) *
+                        band_n[2]);
     return (unsigned)((band_cells + 255) / 256);
   }
 
@@ -835,11 +840,8 @@ struct Level {
       mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
       mv.cs = std::cos(angle); mv.sn = std::sin(angle);
       mv.z = 0.0;
-      hipLaunchKernelGGL(k_ib_move, dim3((nm + 255) / 256), dim3(256), 0, stream, mv);
       const IbArgs aw = ib_args();
-      hipLaunchKernelGGL(k_ib_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, aw);
-      const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
-      hipLaunchKernelGGL(k_ib_chunk_bounds, dim3((nch + 63) / 64), dim3(64), 0, stream, aw);
+      hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, mv, aw);
     } else if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
@@ -878,8 +880,9 @@ struct Level {
     const unsigned gfix = fix_blocks();
     if (gfix) fixfn(mode, dim3(gfix), dim3(tb), stream, a, g, gbuf[cur], tau, r);
     if (nloop == 1) {
+      // the marker sums feed only the force record (a sub-step without a
+      // record is followed by one that has it)
       if (rec_enable) hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
-      else hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
     } else if (rec_enable) {
       hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gfix);
     }
@@ -1031,9 +1034,15 @@ struct Level {
   }
 
   void init(const double* macro_host) {
-    grow_histories(1, 1);
     double* m0 = dev_alloc<double>(4 * ncells);
     HIPCHK(hipMemcpyAsync(m0, macro_host, sizeof(double) * 4 * ncells, hipMemcpyHostToDevice, stream));
+    init_dev(m0);
+    dev_free(m0);
+  }
+
+  // f = feq of the device macros m0 [4][cells] (InitEquilibrium)
+  void init_dev(const double* m0) {
+    grow_histories(1, 1);
     const bool small = (int64_t)c.q * ncells <= (int64_t)1 << 27;
     if (small) ensure_exports();
     InitArgs ia{};
@@ -1043,7 +1052,6 @@ struct Level {
     initfn(grid, block, stream, ia);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
-    dev_free(m0);
     state_valid = small;
     t_new = 0.0;
     t_old = -1.0e100;
@@ -1417,5 +1425,5 @@ int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
 }  // extern "C"
 
 #include "ilbm_batch_impl.h"
-#include "ilbm_amr_impl.h"
 #include "ilbm_box_impl.h"
+#include "ilbm_amr_impl.h"

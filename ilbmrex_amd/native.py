@@ -62,7 +62,7 @@ EXPORTED = [
     "ilbm_batch_case_tile", "ilbm_batch_info", "ilbm_device_memory",
     "ilbm_amr_create", "ilbm_amr_destroy", "ilbm_amr_advance", "ilbm_amr_level_cells",
     "ilbm_amr_get_f", "ilbm_amr_get_macro", "ilbm_amr_mass_history", "ilbm_amr_force_history",
-    "ilbm_amr_num_markers", "ilbm_amr_info",
+    "ilbm_amr_num_markers", "ilbm_amr_info", "ilbm_amr_set_regrid", "ilbm_amr_boxes",
     "ilbm_box_equilibrium", "ilbm_box_fill_periodic", "ilbm_box_fill_physical", "ilbm_box_collide",
     "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
     "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore", "ilbm_level_get_derived",
@@ -134,6 +134,8 @@ def load():
     L.ilbm_amr_force_history.argtypes = [vp, P(C.c_double), C.c_int, P(C.c_int), P(C.c_int)]
     L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
     L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
+    L.ilbm_amr_set_regrid.argtypes = [vp, P(RegridParams)]
+    L.ilbm_amr_boxes.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int)]
     L.ilbm_level_get_derived.argtypes = [vp, P(C.c_double)]
     L.ilbm_level_checkpoint_size.argtypes = [vp, P(C.c_int64)]
     L.ilbm_level_checkpoint.argtypes = [vp, C.c_void_p, C.c_int64]
@@ -475,6 +477,13 @@ class Batch:
                 "lups": float(out[7])}
 
 
+class RegridParams(C.Structure):
+    """ilbm_regrid_params (include/ilbm.h)."""
+    _fields_ = [("threshold_ratio", C.c_double * 4), ("no_refine_xlo", C.c_double), ("no_refine_xhi", C.c_double),
+                ("refine_upstream", C.c_double), ("refine_downstream", C.c_double), ("refine_cross", C.c_double),
+                ("blocking_factor", C.c_int), ("n_error_buf", C.c_int)]
+
+
 class AmrHierarchy:
     """Levels 0..nlev-1 at refinement ratio 2 (ilbm_amr_* in include/ilbm.h):
     level 0 is the case's domain, each box in ``boxes`` (level-l cell
@@ -517,6 +526,27 @@ class AmrHierarchy:
 
     def advance(self, nsteps: int, want_state: bool = False):
         _check(load().ilbm_amr_advance(self._h, int(nsteps), int(want_state)), "amr_advance")
+
+    def set_regrid(self, threshold_ratio=(1.0,), blocking_factor: int = 8, n_error_buf: int = 1,
+                   no_refine_xlo: float = 0.0, no_refine_xhi: float = 0.0, refine=(-1.0, -1.0, -1.0)):
+        """Switch regridding on every regrid_int steps (amrvorth.thresholdRatio,
+        amr.blocking_factor, amr.n_error_buf, amrvorth.no_refine_xlo/hi,
+        ibm.refine_upstream/downstream/cross)."""
+        p = RegridParams()
+        tr = list(threshold_ratio) + [threshold_ratio[-1]] * (4 - len(threshold_ratio))
+        for i in range(4):
+            p.threshold_ratio[i] = float(tr[i])
+        p.no_refine_xlo, p.no_refine_xhi = float(no_refine_xlo), float(no_refine_xhi)
+        p.refine_upstream, p.refine_downstream, p.refine_cross = (float(x) for x in refine)
+        p.blocking_factor, p.n_error_buf = int(blocking_factor), int(n_error_buf)
+        _check(load().ilbm_amr_set_regrid(self._h, C.byref(p)), "amr_set_regrid")
+
+    def boxes(self):
+        """current boxes of levels 1.. as ((lo), (hi)) and the regrid count"""
+        n = max(self.nlev - 1, 1)
+        lo, hi, nr = (C.c_int * (3 * n))(), (C.c_int * (3 * n))(), C.c_int(0)
+        _check(load().ilbm_amr_boxes(self._h, lo, hi, C.byref(nr)), "amr_boxes")
+        return [(tuple(lo[3 * l:3 * l + 3]), tuple(hi[3 * l:3 * l + 3])) for l in range(self.nlev - 1)], nr.value
 
     def cells(self, lev: int) -> int:
         n = C.c_int64(0)

@@ -201,8 +201,9 @@ int ilbm_device_memory(int64_t* free_bytes, int64_t* total_bytes);
  * coarse-fine transfer of the reference (Source/Core/AmrCoreLBM.cpp:
  * FillPatchMesoscopic lev > 0 :1545-1627, FillPatchMacro :2166-2198,
  * GetDataMacro :2239-2269, AverageDownTo :1520-1526,
- * MakeNewLevelFromCoarse :751-830, UpdateDerivedMacroFields :150-168);
- * regridding is left out: the boxes are fixed.  box_lo / box_hi hold, for
+ * MakeNewLevelFromCoarse :751-830, UpdateDerivedMacroFields :150-168).
+ * The boxes stay fixed unless ilbm_amr_set_regrid switches regridding on
+ * (below).  box_lo / box_hi hold, for
  * level l >= 1, its box in level-l cell indices at [3*(l-1) + d]; a box
  * covers whole coarse cells and keeps 3 coarse cells from its coarse box.
  * The immersed body lives on the finest level; the initial state
@@ -226,6 +227,31 @@ int ilbm_amr_num_markers(ilbm_amr* A, int* out);
 /* [0] levels, [1] coarse steps, [2] time, [3..5] cells of levels 0..2,
  * [6] tau of the finest level, [7] tau of level 0 */
 int ilbm_amr_info(ilbm_amr* A, double out[8]);
+
+/* Regridding (timeStepWithSubcycling's regrid, AmrCoreLBM.cpp:1692-1720):
+ * every regrid_int steps of level l < finest, ErrorEst tags level l
+ * (AmrCoreLBM.cpp:986-1180: |vorticity| > thresholdRatio[l] * max
+ * vorticity, the level-set band of a cylinder / sphere, the body
+ * refinement region :56-104, cleared in the no-refine x bands), the tags
+ * are grown by n_error_buf and each finer level gets ONE box: the bounding
+ * box of its parent's tags (and of its own child's footprint grown by 3),
+ * aligned to blocking_factor and kept 3 cells inside its parent; AMReX's
+ * Berger-Rigoutsos clustering into several boxes is not reproduced.  A
+ * changed level is remade as RemakeLevel does (:835-914: macros from the
+ * old level where it was valid and coarse interpolation elsewhere,
+ * f = feq(macros), new marker backend on the finest level). */
+typedef struct {
+  double threshold_ratio[4];   /* amrvorth.thresholdRatio per level */
+  double no_refine_xlo, no_refine_xhi;                      /* amrvorth.no_refine_xlo / xhi */
+  double refine_upstream, refine_downstream, refine_cross;  /* ibm.refine_* (cylinder / sphere) */
+  int blocking_factor;         /* amr.blocking_factor, fine cells, even */
+  int n_error_buf;             /* amr.n_error_buf */
+} ilbm_regrid_params;
+/* before the first ilbm_amr_advance; ILBM_ERR_STATE afterwards */
+int ilbm_amr_set_regrid(ilbm_amr* A, const ilbm_regrid_params* params);
+/* current boxes of levels 1..nlev-1 (level-l indices, [3*(l-1) + d]) and the
+ * number of regrids that changed a box */
+int ilbm_amr_boxes(ilbm_amr* A, int* box_lo, int* box_hi, int* n_regrids);
 
 /* ------------------------------------------------------------------------
  * Per-box entry points on caller-owned device memory in the reference's

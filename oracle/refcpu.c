@@ -1240,7 +1240,14 @@ struct rc_amr {
   rc_level *L[RC_MAXLEV];
   int regrid_int;
   int istep;
+  rc_case c0;                      /* the case as given (refinement-region parameters) */
+  int regrid_on;
+  rc_regrid rg;
+  int last_regrid_step[RC_MAXLEV];
+  int nregrid;                     /* regrids that changed at least one box */
 };
+
+static void amr_regrid(struct rc_amr *A, int lbase, double time);
 
 static int floordiv2(int i) { return (i >= 0) ? i / 2 : -((1 - i) / 2); }
 
@@ -1441,8 +1448,14 @@ static void average_down_to(struct rc_amr *A, int lc) {
       }
 }
 
-/* timeStepWithSubcycling (AmrCoreLBM.cpp:1688-1754) without regrid. */
+/* timeStepWithSubcycling (AmrCoreLBM.cpp:1688-1754); the regrid check of
+ * :1692-1720 runs when regridding is switched on. */
 static void amr_substep(struct rc_amr *A, int l, double time) {
+  if (A->regrid_on && l < A->nlev - 1 && A->L[l]->istep > A->last_regrid_step[l] &&
+      A->L[l]->istep % A->regrid_int == 0) {
+    amr_regrid(A, l, time);
+    for (int k = l; k < A->nlev; ++k) A->last_regrid_step[k] = A->L[k]->istep;
+  }
   rc_level *L = A->L[l];
   L->t_old = L->t_new;
   L->t_new += L->dt;
@@ -1509,12 +1522,295 @@ static rc_level *amr_new_level(struct rc_amr *A, int l, const rc_case *c0, const
   return L;
 }
 
+
+/* ------------------------------------------------------------------ */
+/* Regrid: ErrorEst (AmrCoreLBM.cpp:986-1180) with vorticity_tagging /  */
+/* levelset_tagging (Source/Tagging.H:12-35) and the body refinement    */
+/* region (AmrCoreLBM.cpp:56-104, user geometry                          */
+/* Examples/Pitching_airfoil/IBMUserDefinedGeometry.H:25-57), then      */
+/* RemakeLevel (AmrCoreLBM.cpp:835-914).  AMReX's clustering of the tags */
+/* into boxes (Berger-Rigoutsos, grid_eff, max_grid_size) is replaced by */
+/* one box per level: the bounding box of the buffered tags, aligned to */
+/* the blocking factor and kept 3 coarse cells inside the coarser box    */
+/* (the nesting the coarse-fine interpolation needs).  A level whose     */
+/* tags are empty keeps its box.  The box choice is this build's own:   */
+/* parity of the boxes with AMReX is unpinned; everything after the box  */
+/* choice follows the reference.                                         */
+/* ------------------------------------------------------------------ */
+
+void rc_amr_set_regrid(rc_amr *A, const rc_regrid *p) {
+  A->rg = *p;
+  A->regrid_on = 1;
+}
+
+int rc_amr_regrids(const rc_amr *A) { return A->nregrid; }
+
+/* regrid(lbase) at the hierarchy's current time, outside the step loop */
+void rc_amr_regrid_now(rc_amr *A, int lbase) { amr_regrid(A, lbase, A->L[lbase]->t_new); }
+
+int rc_amr_boxes(const rc_amr *A, int *lo, int *hi) {
+  for (int l = 1; l < A->nlev; ++l)
+    for (int d = 0; d < 3; ++d) { lo[3 * (l - 1) + d] = A->L[l]->vlo[d]; hi[3 * (l - 1) + d] = A->L[l]->vhi[d]; }
+  return A->nlev - 1;
+}
+
+/* immersed_boundary_refinement_region (AmrCoreLBM.cpp:60-104) */
+static int ib_refine_region(const rc_case *c, const rc_regrid *rg, double dx, double *lo, double *hi) {
+  const double pad = fmax(6.0 * dx, 2.0);   /* ib_refine_padding :56-58 */
+  if (c->geometry_type == RC_GEOM_BOX) {
+    lo[0] = c->box_lo[0] - pad; hi[0] = c->box_hi[0] + pad;
+    lo[1] = (c->box_lid_only > 0) ? (c->box_hi[1] - pad) : (c->box_lo[1] - pad);
+    hi[1] = c->box_hi[1] + pad;
+    lo[2] = c->box_lo[2] - pad; hi[2] = c->box_hi[2] + pad;
+    return 1;
+  }
+  if (c->geometry_type == RC_GEOM_USER) {
+    /* the swept section: leading edge at (x0, y0); the vertical reach is
+     * that of the point farthest from the pivot (the trailing edge for the
+     * leading-edge pivot of the reference hook) */
+    if (c->dim != 2) return 0;
+    const double chord = fmax(c->airfoil_chord, 1.0e-12);
+    const double ht = 0.5 * c->airfoil_thickness * chord;
+    const double arm = fmax(c->pivot_x, chord - c->pivot_x);
+    const double yext = arm * sin(fabs(c->pitch_amplitude)) + ht;
+    lo[0] = c->x0 - pad; hi[0] = c->x0 + chord + pad;
+    lo[1] = c->y0 - yext - pad; hi[1] = c->y0 + yext + pad;
+    lo[2] = c->z0 - pad; hi[2] = c->z0 + pad;
+    return 1;
+  }
+  if (c->R > 0.0) {
+    const double x0 = c->x0, y0 = c->y0, z0 = c->z0, R = c->R;
+    if (rg->refine_upstream > 0.0 && rg->refine_downstream > 0.0 && rg->refine_cross > 0.0) {
+      lo[0] = x0 - rg->refine_upstream * R - pad; hi[0] = x0 + rg->refine_downstream * R + pad;
+      lo[1] = y0 - rg->refine_cross * R - pad; hi[1] = y0 + rg->refine_cross * R + pad;
+      lo[2] = z0 - rg->refine_cross * R - pad; hi[2] = z0 + rg->refine_cross * R + pad;
+    } else {
+      lo[0] = x0 - R - pad; hi[0] = x0 + R + pad;
+      lo[1] = y0 - R - pad; hi[1] = y0 + R + pad;
+      lo[2] = z0 - R - pad; hi[2] = z0 + R + pad;
+    }
+    return 1;
+  }
+  return 0;
+}
+
+/* ErrorEst on level l at `time`: bounding box (level-l indices) of the
+ * tagged cells grown by n_error_buf; returns the tag count. */
+static int64_t amr_tag_bbox(struct rc_amr *A, int l, double time, int blo[3], int bhi[3]) {
+  rc_level *L = A->L[l];
+  const rc_case *c = &L->c;
+  const rc_case *c0 = &A->c0;
+  const int dim = c->dim;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { lo[d] = L->vlo[d]; hi[d] = L->vhi[d]; }
+  for (int d = 0; d < dim; ++d) { lo[d] -= 1; hi[d] += 1; }
+  fab mp = fab_alloc(lo, hi, 4);      /* FillPatchMacro(lev, time, macro_tag, 0, 4), one ghost */
+  if (l > 0) {
+    amr_fill_macro(A, l, time, &mp, 4);
+  } else {
+    fab src = fab_alloc(L->m_new.lo, L->m_new.hi, 4);
+    for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+      for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+        for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
+          for (int n = 0; n < 4; ++n) AT(src, i, j, k, n) = macro_at_time(L, time, i, j, k, n);
+    fill_patch(L, &mp, &src, 4, 0);
+    fab_free(&src);
+  }
+  fab vort = fab_alloc(L->vlo, L->vhi, 1);
+  double vmax = -HUGE_VAL;            /* vort_for_tag.max(0) */
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
+        double v, P;
+        vis_para(c, &mp, i, j, k, &v, &P);
+        AT(vort, i, j, k, 0) = v;
+        vmax = fmax(vmax, v);
+      }
+  fab_free(&mp);
+  const double thr = A->rg.threshold_ratio[l] * vmax;
+  const double dxm = c->dx[0];
+  double rlo[3] = {0, 0, 0}, rhi[3] = {0, 0, 0};
+  const int region = c0->use_ib && ib_refine_region(c0, &A->rg, dxm, rlo, rhi);
+  /* levelset_tagging applies for the cylinder / sphere marker geometry
+   * (marker_levelset_ibm, AmrCoreLBM.cpp:1070-1073), phi = r - R */
+  const int lset = c0->use_ib && c0->geometry_type == RC_GEOM_CYLINDER;
+  const double band = (5.0 + 0.5) * dxm;
+  const int buf = A->rg.n_error_buf;
+  const double nrlo = A->rg.no_refine_xlo, nrhi = A->rg.no_refine_xhi;
+  const double xlo_block = c->plo[0] + nrlo;
+  const double xhi_block = c->plo[0] + c->dx[0] * (double)(c->n[0]) - nrhi;   /* probhi[0] - no_refine_xhi */
+  int64_t count = 0;
+  for (int d = 0; d < 3; ++d) { blo[d] = 1 << 30; bhi[d] = -(1 << 30); }
+  for (int k = L->vlo[2]; k <= L->vhi[2]; ++k)
+    for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
+      for (int i = L->vlo[0]; i <= L->vhi[0]; ++i) {
+        const double x = c->plo[0] + ((double)i + 0.5) * c->dx[0];
+        const double y = c->plo[1] + ((double)j + 0.5) * c->dx[1];
+        const double z = (dim == 3) ? c->plo[2] + ((double)k + 0.5) * c->dx[2] : 0.0;
+        if ((nrlo > 0.0 || nrhi > 0.0) && (x <= xlo_block || x >= xhi_block)) continue;
+        int tag = fabs(AT(vort, i, j, k, 0)) > thr;
+        if (lset) {
+          double r2 = (x - c0->x0) * (x - c0->x0) + (y - c0->y0) * (y - c0->y0);
+          if (dim == 3) r2 += (z - c0->z0) * (z - c0->z0);
+          if (sqrt(r2) - c0->R < band) tag = 1;
+        }
+        if (region && x >= rlo[0] && x <= rhi[0] && y >= rlo[1] && y <= rhi[1] && z >= rlo[2] && z <= rhi[2])
+          tag = 1;
+        if (!tag) continue;
+        ++count;
+        const int iv[3] = {i, j, k};
+        for (int d = 0; d < dim; ++d) {
+          if (iv[d] - buf < blo[d]) blo[d] = iv[d] - buf;
+          if (iv[d] + buf > bhi[d]) bhi[d] = iv[d] + buf;
+        }
+      }
+  for (int d = dim; d < 3; ++d) { blo[d] = 0; bhi[d] = 0; }
+  fab_free(&vort);
+  return count;
+}
+
+static int fdiv(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+/* One box for level l+1 (level-(l+1) indices) from a tag bounding box in
+ * level-l indices: clipped to the parent box shrunk by 3 cells, aligned
+ * outward to the blocking unit (blocking_factor / 2 level-l cells), then
+ * inward where the alignment leaves the clip region.  0 if empty. */
+static int box_from_tags(int dim, const int tlo[3], const int thi[3], const int plo[3], const int phi[3], int bu,
+                         int out_lo[3], int out_hi[3]) {
+  for (int d = 0; d < 3; ++d) { out_lo[d] = 0; out_hi[d] = 0; }
+  for (int d = 0; d < dim; ++d) {
+    const int nlo = plo[d] + 3, nhi = phi[d] - 3;
+    const int lo = (tlo[d] > nlo) ? tlo[d] : nlo, hi = (thi[d] < nhi) ? thi[d] : nhi;
+    if (lo > hi) return 0;
+    int alo = fdiv(lo, bu) * bu, ahi = (fdiv(hi, bu) + 1) * bu - 1;
+    if (alo < nlo) alo = -fdiv(-nlo, bu) * bu;
+    if (ahi > nhi) ahi = fdiv(nhi + 1, bu) * bu - 1;
+    if (alo > ahi) return 0;
+    out_lo[d] = 2 * alo;
+    out_hi[d] = 2 * ahi + 1;
+  }
+  return 1;
+}
+
+/* RemakeLevel (AmrCoreLBM.cpp:835-914): macros by FillPatchMacro (the old
+ * level where it was valid, coarse interpolation elsewhere, all nmac
+ * components, ghosts included), populations = feq of those macros,
+ * macro_old = macro_new; the finest level gets a new marker backend
+ * (:906-909), the force history carries on; then UpdateDerivedMacroFields. */
+static void amr_remake(struct rc_amr *A, int l, double time, const int blo[3], const int bhi[3]) {
+  rc_level *O = A->L[l];
+  rc_level *N = (rc_level *)calloc(1, sizeof(rc_level));
+  N->c = O->c;
+  N->ng = O->ng;
+  N->amr = A;
+  N->lev = l;
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) {
+    N->dom_lo[d] = O->dom_lo[d]; N->dom_hi[d] = O->dom_hi[d];
+    N->vlo[d] = blo[d]; N->vhi[d] = bhi[d];
+    lo[d] = blo[d]; hi[d] = bhi[d];
+  }
+  for (int d = 0; d < N->c.dim; ++d) { lo[d] -= N->ng; hi[d] += N->ng; }
+  N->f_new = fab_alloc(lo, hi, N->c.q);
+  N->f_old = fab_alloc(lo, hi, N->c.q);
+  N->m_new = fab_alloc(lo, hi, N->c.nmac);
+  N->m_old = fab_alloc(lo, hi, N->c.nmac);
+  N->forcing = fab_alloc(lo, hi, 3);
+  N->tau = O->tau; N->dt = O->dt;
+  N->t_new = O->t_new; N->t_old = O->t_old;
+  N->istep = O->istep;
+  N->have_prev = O->have_prev;
+  for (int d = 0; d < 3; ++d) N->prev_body[d] = O->prev_body[d];
+  amr_fill_macro(A, l, time, &N->m_new, N->c.nmac);   /* A->L[l] is still the old level */
+  const rc_case *c = &N->c;
+  for (int64_t s = 0; s < N->f_new.ns; ++s) {
+    const double r = fmax(N->m_new.p[s], 1.0e-12);
+    const double u0 = N->m_new.p[s + N->m_new.ns], v0 = N->m_new.p[s + 2 * N->m_new.ns];
+    const double w0 = N->m_new.p[s + 3 * N->m_new.ns];
+    const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+    for (int q = 0; q < c->q; ++q) {
+      const double cu = c->cx[q] * u0 + c->cy[q] * v0 + c->cz[q] * w0;
+      const double feq = c->w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+      N->f_new.p[s + q * N->f_new.ns] = feq;
+      N->f_old.p[s + q * N->f_old.ns] = feq;
+    }
+  }
+  memcpy(N->m_old.p, N->m_new.p, sizeof(double) * (size_t)(N->m_new.ns * N->m_new.nc));
+  A->L[l] = N;
+  if (l == A->nlev - 1 && N->c.use_ib) {
+    ib_build(N);
+    ib_update_markers(N, time);
+  }
+  update_derived(A, N, time);
+  rc_destroy(O);
+}
+
+/* regrid(lbase, time) with the one-box clustering above: the boxes of
+ * levels lbase+1 .. finest are made finest first, each from the tags of
+ * its parent level (old data) and, for a parent that is itself re-boxed,
+ * from the footprint of the new child box grown by 3 cells; changed
+ * levels are remade coarse to fine (AmrCore::regrid). */
+static void amr_regrid(struct rc_amr *A, int lbase, double time) {
+  const int dim = A->L[0]->c.dim;
+  const int bu = (A->rg.blocking_factor / 2 > 0) ? A->rg.blocking_factor / 2 : 1;
+  int tlo[RC_MAXLEV][3], thi[RC_MAXLEV][3];
+  int64_t tcnt[RC_MAXLEV];
+  for (int l = lbase; l < A->nlev - 1; ++l) tcnt[l] = amr_tag_bbox(A, l, time, tlo[l], thi[l]);
+  int nlo[RC_MAXLEV][3], nhi[RC_MAXLEV][3];
+  for (int l = 1; l < A->nlev; ++l)
+    for (int d = 0; d < 3; ++d) { nlo[l][d] = A->L[l]->vlo[d]; nhi[l][d] = A->L[l]->vhi[d]; }
+  for (int lf = A->nlev - 1; lf > lbase; --lf) {
+    const int lc = lf - 1;
+    int qlo[3], qhi[3];
+    int have = tcnt[lc] > 0;
+    for (int d = 0; d < 3; ++d) { qlo[d] = tlo[lc][d]; qhi[d] = thi[lc][d]; }
+    if (lf < A->nlev - 1) {   /* contain the (new) child footprint grown by 3 */
+      for (int d = 0; d < dim; ++d) {
+        const int flo = fdiv(nlo[lf + 1][d], 2) - 3, fhi = fdiv(nhi[lf + 1][d], 2) + 3;
+        const int clo = fdiv(flo, 2), chi = fdiv(fhi, 2);
+        if (!have || clo < qlo[d]) qlo[d] = clo;
+        if (!have || chi > qhi[d]) qhi[d] = chi;
+      }
+      if (!have) for (int d = 0; d < dim; ++d) {   /* keep the old box as well */
+        const int olo = fdiv(nlo[lf][d], 2), ohi = fdiv(nhi[lf][d], 2);
+        if (olo < qlo[d]) qlo[d] = olo;
+        if (ohi > qhi[d]) qhi[d] = ohi;
+      }
+      have = 1;
+    }
+    if (!have) continue;      /* no tags: the level keeps its box */
+    int blo[3], bhi[3];
+    const rc_level *P = A->L[lc];
+    const int *plo = (lc > lbase) ? nlo[lc] : P->vlo, *phi = (lc > lbase) ? nhi[lc] : P->vhi;
+    if (box_from_tags(dim, qlo, qhi, plo, phi, bu, blo, bhi))
+      for (int d = 0; d < 3; ++d) { nlo[lf][d] = blo[d]; nhi[lf][d] = bhi[d]; }
+  }
+  /* a parent box chosen after its child: re-clip the child into it */
+  for (int lf = lbase + 2; lf < A->nlev; ++lf) {
+    int clo[3], chi[3], blo[3], bhi[3];
+    for (int d = 0; d < 3; ++d) { clo[d] = fdiv(nlo[lf][d], 2); chi[d] = fdiv(nhi[lf][d], 2); }
+    if (box_from_tags(dim, clo, chi, nlo[lf - 1], nhi[lf - 1], bu, blo, bhi))
+      for (int d = 0; d < 3; ++d) { nlo[lf][d] = blo[d]; nhi[lf][d] = bhi[d]; }
+  }
+  int changed = 0;
+  for (int l = lbase + 1; l < A->nlev; ++l) {
+    int same = 1;
+    for (int d = 0; d < 3; ++d)
+      if (nlo[l][d] != A->L[l]->vlo[d] || nhi[l][d] != A->L[l]->vhi[d]) same = 0;
+    if (same) continue;
+    amr_remake(A, l, time, nlo[l], nhi[l]);
+    changed = 1;
+  }
+  A->nregrid += changed;
+}
+
 rc_amr *rc_amr_create(const rc_case *c0, int nlev, const int *box_lo, const int *box_hi, int regrid_int,
                       const double *init_macro) {
   if (nlev < 1 || nlev > RC_MAXLEV || regrid_int < 1 || c0->nmac > RC_MAXC) return NULL;
   rc_amr *A = (rc_amr *)calloc(1, sizeof(rc_amr));
   A->nlev = nlev;
   A->regrid_int = regrid_int;
+  A->c0 = *c0;
   rc_case cc = *c0;
   if (nlev > 1) cc.use_ib = 0;
   A->L[0] = rc_create(&cc, init_macro);

@@ -142,6 +142,20 @@ rc_amr *rc_amr_create(const rc_case *c, int nlev, const int *box_lo, const int *
 void rc_amr_destroy(rc_amr *A);
 void rc_amr_step(rc_amr *A);
 void rc_amr_advance(rc_amr *A, int nsteps);
+/* Regrid (ErrorEst / RemakeLevel with one box per level): per-level
+ * amrvorth.thresholdRatio, amrvorth.no_refine_xlo/hi, ibm.refine_*,
+ * amr.blocking_factor and amr.n_error_buf. */
+typedef struct {
+  double threshold_ratio[4];
+  double no_refine_xlo, no_refine_xhi;
+  double refine_upstream, refine_downstream, refine_cross;
+  int blocking_factor;
+  int n_error_buf;
+} rc_regrid;
+void rc_amr_set_regrid(rc_amr *A, const rc_regrid *p);
+int rc_amr_boxes(const rc_amr *A, int *lo, int *hi);
+int rc_amr_regrids(const rc_amr *A);
+void rc_amr_regrid_now(rc_amr *A, int lbase);
 rc_level *rc_amr_level(rc_amr *A, int l);
 int rc_get_macro_all(const rc_level *L, double *out);
 void rc_cons_interp_test(const double *crse, const int *clo, const int *chi, int dim, int ncomp,

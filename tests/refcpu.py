@@ -46,6 +46,12 @@ class RcCase(C.Structure):
     ]
 
 
+class RcRegrid(C.Structure):
+    _fields_ = [("threshold_ratio", C.c_double * 4), ("no_refine_xlo", C.c_double), ("no_refine_xhi", C.c_double),
+                ("refine_upstream", C.c_double), ("refine_downstream", C.c_double), ("refine_cross", C.c_double),
+                ("blocking_factor", C.c_int), ("n_error_buf", C.c_int)]
+
+
 def build() -> None:
     subprocess.run(["make", "-s", "-C", ORACLE_DIR], check=True)
 
@@ -98,6 +104,10 @@ def lib():
         L.rc_amr_advance.argtypes = [C.c_void_p, C.c_int]
         L.rc_amr_level.restype = C.c_void_p
         L.rc_amr_level.argtypes = [C.c_void_p, C.c_int]
+        L.rc_amr_set_regrid.argtypes = [C.c_void_p, P(RcRegrid)]
+        L.rc_amr_boxes.argtypes = [C.c_void_p, P(C.c_int), P(C.c_int)]
+        L.rc_amr_regrids.argtypes = [C.c_void_p]
+        L.rc_amr_regrid_now.argtypes = [C.c_void_p, C.c_int]
         L.rc_get_macro_all.argtypes = [C.c_void_p, P(C.c_double)]
         L.rc_cons_interp_test.argtypes = [P(C.c_double), P(C.c_int), P(C.c_int), C.c_int, C.c_int,
                                           P(C.c_int), P(C.c_double)]
@@ -244,6 +254,31 @@ class AmrOracle:
 
     def advance(self, n: int):
         lib().rc_amr_advance(self._h, int(n))
+        self.boxes = self.current_boxes()
+
+    def set_regrid(self, threshold_ratio=(1.0, 1.0, 1.0, 1.0), blocking_factor=8, n_error_buf=1,
+                   no_refine_xlo=0.0, no_refine_xhi=0.0, refine=(-1.0, -1.0, -1.0)):
+        r = RcRegrid()
+        tr = list(threshold_ratio) + [threshold_ratio[-1]] * (4 - len(threshold_ratio))
+        for i in range(4):
+            r.threshold_ratio[i] = float(tr[i])
+        r.no_refine_xlo, r.no_refine_xhi = float(no_refine_xlo), float(no_refine_xhi)
+        r.refine_upstream, r.refine_downstream, r.refine_cross = map(float, refine)
+        r.blocking_factor, r.n_error_buf = int(blocking_factor), int(n_error_buf)
+        lib().rc_amr_set_regrid(self._h, C.byref(r))
+
+    def current_boxes(self):
+        n = max(self.nlev - 1, 1)
+        lo, hi = (C.c_int * (3 * n))(), (C.c_int * (3 * n))()
+        lib().rc_amr_boxes(self._h, lo, hi)
+        return [(tuple(lo[3 * l:3 * l + 3]), tuple(hi[3 * l:3 * l + 3])) for l in range(self.nlev - 1)]
+
+    def regrid_now(self, lbase: int = 0):
+        lib().rc_amr_regrid_now(self._h, int(lbase))
+        self.boxes = self.current_boxes()
+
+    def regrids(self) -> int:
+        return lib().rc_amr_regrids(self._h)
 
     def ncells(self, l):
         if l == 0:

@@ -23,15 +23,18 @@ TWO_LEVEL = [((24, 28, 0), (71, 67, 0))]
 THREE_LEVEL = [((16, 16, 0), (79, 79, 0)), ((48, 48, 0), (127, 127, 0))]
 
 
-def run_pair(case, boxes, nsteps, regrid_int=2):
+def run_pair(case, boxes, nsteps, regrid_int=2, regrid=None):
     from ilbmrex_amd.native import AmrHierarchy
     print(f"[amr] {case.name} {case.lattice.name} levels={1 + len(boxes)} storage={case.storage} "
-          f"steps={nsteps}", flush=True)
+          f"steps={nsteps} regrid={regrid}", flush=True)
     u0 = case.bc_lo_val[0][1]
     state = (1.0, u0, 0.0, 0.0)
     gpu = AmrHierarchy(case, boxes, regrid_int, state)
-    gpu.advance(nsteps, want_state=True)
     orc = refcpu.AmrOracle(case, boxes, regrid_int, cf.uniform_init(case, u=(u0, 0.0, 0.0)))
+    if regrid is not None:
+        gpu.set_regrid(**regrid)
+        orc.set_regrid(**regrid)
+    gpu.advance(nsteps, want_state=True)
     orc.advance(nsteps)
     return gpu, orc
 
@@ -104,7 +107,7 @@ def test_two_level_pitching_section_bitexact():
 
 def test_pitching_naca_three_levels_runs():
     from ilbmrex_amd.native import AmrHierarchy
-    c, boxes = cf.pitching_naca_amr()
+    c, boxes = cf.pitching_naca_amr(Re=100.0)
     h = AmrHierarchy(c, boxes, 10, (1.0, c.U0, 0.0, 0.0))
     assert h.num_markers() == 256
     h.advance(20)
@@ -112,3 +115,54 @@ def test_pitching_naca_three_levels_runs():
     assert fh.shape == (20, 23) and np.isfinite(fh).all()
     info = h.info()
     assert info["cells"] == [512 * 256, 128 * 96, 160 * 128]
+
+
+# ---------------------------------------------------------------------------
+# Regrid: the GPU's tags, boxes and remade levels against the oracle's
+# ---------------------------------------------------------------------------
+
+def check_boxes(gpu, orc):
+    boxes, n = gpu.boxes()
+    assert boxes == orc.boxes, (boxes, orc.boxes)
+    assert n == orc.regrids()
+    return n
+
+
+def test_two_level_cylinder_regrid_bitexact():
+    c = cyl_case()
+    gpu, orc = run_pair(c, TWO_LEVEL, 12, regrid_int=2,
+                        regrid=dict(threshold_ratio=(0.5, 0.5), blocking_factor=4))
+    assert check_boxes(gpu, orc) >= 1
+    check_levels(gpu, orc)
+    check_record(gpu, orc)
+
+
+def test_three_level_cylinder_regrid_bitexact():
+    """regrid(0) re-boxes levels 1 and 2; regrid(1) between coarse steps
+    re-boxes level 2 alone (istep[1] % regrid_int == 0)."""
+    c = cyl_case()
+    gpu, orc = run_pair(c, THREE_LEVEL, 8, regrid_int=3,
+                        regrid=dict(threshold_ratio=(0.5, 0.6, 0.6), blocking_factor=4))
+    assert check_boxes(gpu, orc) >= 1
+    check_levels(gpu, orc)
+    check_record(gpu, orc)
+
+
+def test_two_level_pitching_section_regrid_bitexact():
+    c = cf.cylinder_2d(nx=64, ny=48, D=10.0, xc=20.0, yc=24.0, U=0.03, Re=100.0).replace(
+        geometry_type=cf.GEOM_USER, airfoil_chord=10.0, n_marker=96, pitch_amplitude=0.2,
+        pitch_frequency=0.01, pivot_x=2.5, force_eval_method=cf.FORCE_EVAL_MARKER, name="pitching_amr")
+    gpu, orc = run_pair(c, [((28, 28, 0), (75, 67, 0))], 10, regrid_int=2,
+                        regrid=dict(threshold_ratio=(0.7, 0.7), blocking_factor=4))
+    assert check_boxes(gpu, orc) >= 1
+    check_levels(gpu, orc)
+    check_record(gpu, orc)
+
+
+def test_two_level_sphere_regrid_bitexact():
+    c = cf.sphere_3d(nx=32, ny=24, nz=24, D=6.0, center=(10.0, 12.0, 12.0), U=0.04, Re=30.0)
+    gpu, orc = run_pair(c, [((8, 10, 10), (35, 37, 37))], 4, regrid_int=2,
+                        regrid=dict(threshold_ratio=(0.5, 0.5), blocking_factor=4))
+    assert check_boxes(gpu, orc) >= 1
+    check_levels(gpu, orc)
+    check_record(gpu, orc)

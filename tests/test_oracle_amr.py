@@ -157,3 +157,105 @@ def test_body_markers_use_finest_spacing():
     rec = A.force_record()
     assert len(rec) == 23 and rec[0] == 4 and rec[1] == 4.0
     assert np.isfinite(rec).all() and rec[4] > 0.0
+
+
+# ---------------------------------------------------------------------------
+# Regrid (ErrorEst + one box per level + RemakeLevel)
+# ---------------------------------------------------------------------------
+
+def _cyl():
+    return cf.cylinder_2d(nx=64, ny=48, D=8.0, xc=20.0, yc=24.0, U=0.05, Re=40.0)
+
+
+def _feq(case, m):
+    lat = case.lattice
+    r = np.maximum(m[0], 1.0e-12)
+    u, v, w = m[1], m[2], m[3]
+    u2 = u * u + v * v + w * w
+    out = []
+    for q in range(lat.q):
+        cu = float(lat.cx[q]) * u + float(lat.cy[q]) * v + float(lat.cz[q]) * w
+        out.append(lat.w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2))
+    return np.stack(out)
+
+
+def _expected_region_box(case, bf, buf=1):
+    """Level-1 box from the body refinement region alone: cells whose centre
+    lies in [c - R - pad, c + R + pad] (pad = max(6 dx, 2)), grown by the
+    error buffer, clipped 3 cells inside the domain, aligned outward to
+    blocking_factor / 2 coarse cells; refined by 2."""
+    pad = max(6.0, 2.0)
+    centre = (case.x0, case.y0)
+    lo, hi = [], []
+    bu = bf // 2
+    for d in range(2):
+        a = math.ceil(centre[d] - case.R - pad - 0.5) - buf
+        b = math.floor(centre[d] + case.R + pad - 0.5) + buf
+        a, b = max(a, 3), min(b, case.n[d] - 1 - 3)
+        a, b = (a // bu) * bu, (b // bu + 1) * bu - 1
+        lo.append(2 * a)
+        hi.append(2 * b + 1)
+    return (lo[0], lo[1], 0), (hi[0], hi[1], 0)
+
+
+def test_regrid_box_of_body_region():
+    c = _cyl()
+    o = refcpu.AmrOracle(c, [((24, 28, 0), (71, 67, 0))], 2, cf.uniform_init(c, u=(0.05, 0.0, 0.0)))
+    o.advance(4)
+    # a threshold far above the vorticity maximum leaves the body region
+    # (and the level-set band inside it) as the only tags
+    o.set_regrid(threshold_ratio=(1.0e30, 1.0e30), blocking_factor=8)
+    o.regrid_now(0)
+    assert o.boxes[0] == _expected_region_box(c, 8)
+    assert o.regrids() == 1
+    o.close()
+
+
+def test_remake_keeps_old_data_interpolates_new_cells_and_sets_feq():
+    c = _cyl()
+    box = ((24, 28, 0), (71, 67, 0))
+    o = refcpu.AmrOracle(c, [box], 2, cf.uniform_init(c, u=(0.05, 0.0, 0.0)))
+    o.advance(6)
+    old = o.macro(1, all_comps=True).reshape(c.nmac, box[1][1] - box[0][1] + 1, box[1][0] - box[0][0] + 1)
+    crse = o.macro(0, all_comps=True).reshape(c.nmac, c.n[1], c.n[0])
+    o.set_regrid(threshold_ratio=(0.5, 0.5), blocking_factor=4)
+    o.regrid_now(0)
+    (nlo, nhi), = o.boxes
+    assert (nlo, nhi) != box
+    ny, nx = nhi[1] - nlo[1] + 1, nhi[0] - nlo[0] + 1
+    new = o.macro(1).reshape(4, ny, nx)
+    f = o.f(1).reshape(c.lattice.q, ny, nx)
+    nkept = ninterp = 0
+    for j in range(nlo[1], nhi[1] + 1):
+        for i in range(nlo[0], nhi[0] + 1):
+            got = new[:, j - nlo[1], i - nlo[0]]
+            if box[0][0] <= i <= box[1][0] and box[0][1] <= j <= box[1][1]:
+                assert np.array_equal(got, old[:4, j - box[0][1], i - box[0][0]])
+                nkept += 1
+            else:
+                ic, jc = i // 2, j // 2
+                blk = crse[:, jc - 1:jc + 2, ic - 1:ic + 2][:, None]
+                want = refcpu.cons_interp(blk, (ic - 1, jc - 1, 0), 2, (i, j, 0))
+                assert np.array_equal(got, want[:4]), (i, j)
+                ninterp += 1
+    assert nkept > 0 and ninterp > 0
+    np.testing.assert_allclose(f, _feq(c, new), rtol=1e-15, atol=0.0)
+    o.close()
+
+
+def test_regrid_schedule_counts_and_nesting_three_levels():
+    c = cf.cylinder_2d(nx=64, ny=48, D=8.0, xc=20.0, yc=24.0, U=0.05, Re=40.0)
+    boxes = [((16, 16, 0), (79, 79, 0)), ((48, 48, 0), (127, 127, 0))]
+    o = refcpu.AmrOracle(c, boxes, 2, cf.uniform_init(c, u=(0.05, 0.0, 0.0)))
+    o.set_regrid(threshold_ratio=(0.5, 0.5, 0.5), blocking_factor=4)
+    o.advance(6)
+    assert o.regrids() >= 1
+    (b1lo, b1hi), (b2lo, b2hi) = o.boxes
+    for d in range(2):
+        # whole coarse cells, blocking-aligned, 3 cells of nesting margin
+        assert b1lo[d] % 4 == 0 and (b1hi[d] + 1) % 4 == 0
+        assert b2lo[d] % 4 == 0 and (b2hi[d] + 1) % 4 == 0
+        assert b1lo[d] // 2 >= 3 and b1hi[d] // 2 <= c.n[d] - 1 - 3
+        assert b2lo[d] // 2 - 3 >= b1lo[d] and b2hi[d] // 2 + 3 <= b1hi[d]
+    assert np.isfinite(o.force_record()).all()
+    o.close()
