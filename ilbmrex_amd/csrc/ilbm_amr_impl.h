@@ -794,6 +794,23 @@ int ilbm_amr_num_markers(ilbm_amr* h, int* out) {
   return ILBM_OK;
 }
 
+int ilbm_amr_get_derived(ilbm_amr* h, int lev, double* out) {
+  if (!h || !out || lev < 0 || lev >= h->A.nlev) return ILBM_ERR_INVALID;
+  ilbm::Amr& A = h->A;
+  ilbm::Level& V = *A.L[lev];
+  if (!V.state_valid || !V.mout) return ILBM_ERR_STATE;
+  return guarded([&] {
+    if (2 * V.ncells > A.scratch_cap) {
+      ilbm::dev_free(A.scratch);
+      A.scratch = ilbm::dev_alloc<double>(2 * V.ncells);
+      A.scratch_cap = 2 * V.ncells;
+    }
+    A.patch_derived(lev, V.t_new, A.scratch);
+    HIPCHK(hipMemcpyAsync(out, A.scratch, sizeof(double) * 2 * V.ncells, hipMemcpyDeviceToHost, A.stream));
+    HIPCHK(hipStreamSynchronize(A.stream));
+  });
+}
+
 int ilbm_amr_set_regrid(ilbm_amr* h, const ilbm_regrid_params* p) {
   if (!h || !p) return ILBM_ERR_INVALID;
   return guarded([&] { h->A.set_regrid(*p); });

@@ -62,7 +62,7 @@ EXPORTED = [
     "ilbm_batch_case_tile", "ilbm_batch_info", "ilbm_device_memory",
     "ilbm_amr_create", "ilbm_amr_destroy", "ilbm_amr_advance", "ilbm_amr_level_cells",
     "ilbm_amr_get_f", "ilbm_amr_get_macro", "ilbm_amr_mass_history", "ilbm_amr_force_history",
-    "ilbm_amr_num_markers", "ilbm_amr_info", "ilbm_amr_set_regrid", "ilbm_amr_boxes",
+    "ilbm_amr_num_markers", "ilbm_amr_info", "ilbm_amr_set_regrid", "ilbm_amr_boxes", "ilbm_amr_get_derived",
     "ilbm_box_equilibrium", "ilbm_box_fill_periodic", "ilbm_box_fill_physical", "ilbm_box_collide",
     "ilbm_box_bounce_back", "ilbm_box_stream_macro", "ilbm_box_derived", "ilbm_box_sum",
     "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore", "ilbm_level_get_derived",
@@ -135,6 +135,7 @@ def load():
     L.ilbm_amr_num_markers.argtypes = [vp, P(C.c_int)]
     L.ilbm_amr_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_amr_set_regrid.argtypes = [vp, P(RegridParams)]
+    L.ilbm_amr_get_derived.argtypes = [vp, C.c_int, P(C.c_double)]
     L.ilbm_amr_boxes.argtypes = [vp, P(C.c_int), P(C.c_int), P(C.c_int)]
     L.ilbm_level_get_derived.argtypes = [vp, P(C.c_double)]
     L.ilbm_level_checkpoint_size.argtypes = [vp, P(C.c_int64)]
@@ -562,6 +563,12 @@ class AmrHierarchy:
         out = np.empty(4 * self.cells(lev))
         _check(load().ilbm_amr_get_macro(self._h, int(lev), _dp(out)), "amr_get_macro")
         return out.reshape(4, -1)
+
+    def derived(self, lev: int) -> np.ndarray:
+        """vorticity and pressure of an exported level, [2][cells]"""
+        out = np.empty(2 * self.cells(lev))
+        _check(load().ilbm_amr_get_derived(self._h, int(lev), _dp(out)), "amr_get_derived")
+        return out.reshape(2, -1)
 
     def mass_history(self) -> np.ndarray:
         n = C.c_int(0)

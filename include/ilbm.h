@@ -220,6 +220,9 @@ int ilbm_amr_advance(ilbm_amr* A, int nsteps, int want_state);
 int ilbm_amr_level_cells(ilbm_amr* A, int lev, int64_t* cells);
 int ilbm_amr_get_f(ilbm_amr* A, int lev, double* host_out);      /* [q][box cells] */
 int ilbm_amr_get_macro(ilbm_amr* A, int lev, double* host_out);  /* [4][box cells] */
+/* UpdateDerivedMacroFields of an exported level (AmrCoreLBM.cpp:150-168):
+ * vorticity and pressure from FillPatchMacro with one ghost, [2][box cells] */
+int ilbm_amr_get_derived(ilbm_amr* A, int lev, double* host_out);
 /* Sum(Rho) of level 0 and the finest level's force.dat rows per coarse step */
 int ilbm_amr_mass_history(ilbm_amr* A, double* host_out, int max_rows, int* n_out);
 int ilbm_amr_force_history(ilbm_amr* A, double* host_out, int max_rows, int* n_rows, int* n_cols);
