@@ -27,11 +27,24 @@
 // registers.
 #pragma once
 
+#include <utility>
+
 #include "ilbm_lattice.h"
 
 namespace ilbm {
 
 constexpr int kBlock = 256;
+
+// f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>):
+// a loop over the directions whose index is a constant expression
+template <class F, int... I>
+__device__ __forceinline__ void static_for_impl(F& f, std::integer_sequence<int, I...>) {
+  (f(std::integral_constant<int, I>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  static_for_impl(f, std::make_integer_sequence<int, N>{});
+}
 
 struct GridGeom {
   int n[3];        // valid extents (n[2] == 1 in 2D)
@@ -273,8 +286,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     T f[LAT::Q];
 #pragma unroll
     for (int q = 0; q < LAT::Q; ++q) {
-      constexpr int dummy = 0;
-      (void)dummy;
+      if constexpr (WRAP) {
+        // all-periodic level with planes under 4 GiB: a 32-bit byte offset
+        // from the plane's (uniform) base
+        const uint32_t src = (uint32_t)(ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
+        const R* plane = gin + (int64_t)q * g.qs;
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
+        f[q] = ST::load(p, 0, T(LAT::w[q]));
+        continue;
+      }
       int64_t off = (int64_t)q * g.qs + g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
       if constexpr (!WRAP) {
         const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
@@ -287,15 +307,20 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       }
       f[q] = ST::load(gin, off, T(LAT::w[q]));
     }
-    // moments in the reference's order (calculateMacroForcing.H:21-27)
+    // moments in the reference's order (calculateMacroForcing.H:21-27); a
+    // term whose direction component is 0 is left out and -1 * f is
+    // subtracted, which leaves every finite sum bit-identical
     T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
-#pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) {
+    static_for<LAT::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
       r += f[q];
-      m1 += T(LAT::cx[q]) * f[q];
-      m2 += T(LAT::cy[q]) * f[q];
-      m3 += T(LAT::cz[q]) * f[q];
-    }
+      if constexpr (LAT::cx[q] > 0) m1 += f[q];
+      else if constexpr (LAT::cx[q] < 0) m1 -= f[q];
+      if constexpr (LAT::cy[q] > 0) m2 += f[q];
+      else if constexpr (LAT::cy[q] < 0) m2 -= f[q];
+      if constexpr (LAT::cz[q] > 0) m3 += f[q];
+      else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
+    });
     const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
     const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
                          (unsigned)bk < (unsigned)a.band_n[2];
@@ -356,12 +381,60 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
     const T fac = T(1.0) - T(0.5) / T(a.tau);
     const int64_t self = g.off0 + ox[1] + oy[1] + oz[1];
+    auto put = [&](int q, T gq) {
+      if constexpr (WRAP) {
+        R* plane = gout + (int64_t)q * g.qs;
+        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
+        ST::store_nt(p, 0, gq, T(LAT::w[q]));
+      } else {
+        ST::store_nt(gout, (int64_t)q * g.qs + self, gq, T(LAT::w[q]));
+      }
+    };
+    if constexpr (ST::exact) {
 #pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) {
-      const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
-      T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
-      if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
-      ST::store_nt(gout, (int64_t)q * g.qs + self, gq, T(LAT::w[q]));
+      for (int q = 0; q < LAT::Q; ++q) {
+        const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
+        T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
+        if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+        put(q, gq);
+      }
+    } else {
+      // reduced-precision collision, a direction and its opposite together:
+      // feq = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u), then
+      // g = f + (feq - f) / tau with fused multiply-adds
+      const T base = T(1.0) - T(1.5) * uv;
+      const T om = T(a.inv_tau);
+      static_for<LAT::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        constexpr int p = opposite<LAT>(q);
+        if constexpr (p == q) {
+          const T fe = T(LAT::w[q]) * r * base;
+          T gq = __builtin_fmaf(fe - f[q], om, f[q]);
+          if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+          put(q, gq);
+        } else if constexpr (p > q) {
+          T eu = T(0);
+          if constexpr (LAT::cx[q] > 0) eu += u;
+          else if constexpr (LAT::cx[q] < 0) eu -= u;
+          if constexpr (LAT::cy[q] > 0) eu += v;
+          else if constexpr (LAT::cy[q] < 0) eu -= v;
+          if constexpr (LAT::D == 3) {
+            if constexpr (LAT::cz[q] > 0) eu += w;
+            else if constexpr (LAT::cz[q] < 0) eu -= w;
+          }
+          const T wr = T(LAT::w[q]) * r;
+          const T A = wr * __builtin_fmaf(T(4.5) * eu, eu, base);
+          const T B = (T(3.0) * wr) * eu;
+          T gq = __builtin_fmaf((A + B) - f[q], om, f[q]);
+          T gp = __builtin_fmaf((A - B) - f[p], om, f[p]);
+          if (forced) {
+            gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+            gp += guo_q<LAT, T>(p, fac, u, v, w, cfx, cfy, cfz);
+          }
+          put(q, gq);
+          put(p, gp);
+        }
+      });
     }
   }
   const double s = block_sum(rho_acc, lds);

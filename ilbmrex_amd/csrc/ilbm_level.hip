@@ -467,6 +467,8 @@ struct Level {
     wrap = true;
     for (int d = 0; d < c.dim; ++d)
       if (!c.periodic[d] || cf) { wrap = false; nonper[d] = 1; }
+    // the all-periodic kernel addresses a plane with 32-bit byte offsets
+    if ((int64_t)c.n[0] * c.n[1] * c.n[2] * 8 >= ((int64_t)1 << 32)) wrap = false;
     g.dim = c.dim;
     for (int d = 0; d < 3; ++d) g.n[d] = c.n[d];
     g.pad = 0;
