@@ -435,14 +435,5 @@ __global__ void k_crop(const double* src, int sn0, int sn1, int64_t scells, doub
   for (int c = 0; c < nc; ++c) dst[c * dcells + t] = src[c * scells + s];
 }
 
-// Sum(Rho) of level 0 after the average-down: the stream-collide kernel's
-// block partials plus the covered cells' corrections, in fixed order.
-__global__ void __launch_bounds__(1024) k_mass_total(const double* p0, int64_t n0, const double* p1, int64_t n1,
-                                                     double* out) {
-  const double a = sum_partials_body(p0, n0);
-  __syncthreads();
-  const double b = (n1 > 0) ? sum_partials_body(p1, n1) : 0.0;
-  if (threadIdx.x == 0) *out = a + b;
-}
 
 }  // namespace ilbm

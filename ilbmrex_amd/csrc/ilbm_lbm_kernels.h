@@ -116,6 +116,10 @@ struct StepArgs {
   double* partial;       // per-block mass partials
   int tile_bx;           // tile width of the batched launch (block = tile_bx x kBlock/tile_bx)
   int band_force;        // the band carries the IB force (else it only exports macros)
+  // sub-box split of a level: 0 every cell; 1 the cells outside the box;
+  // 2 the cells of the box, the launch grid covering the box from sub_lo
+  int sub_mode;
+  int sub_lo[3], sub_n[3];
 };
 
 // Block of a batched launch: case index into the launch's argument table and
@@ -232,7 +236,13 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   const GridGeom& g = a.g;
-  const bool active = (i < g.n[0]) && (j < g.n[1]);
+  bool active = (i < g.n[0]) && (j < g.n[1]);
+  if (a.sub_mode != 0) {
+    const bool in_sub = (unsigned)(i - a.sub_lo[0]) < (unsigned)a.sub_n[0] &&
+                        (unsigned)(j - a.sub_lo[1]) < (unsigned)a.sub_n[1] &&
+                        (unsigned)(k - a.sub_lo[2]) < (unsigned)a.sub_n[2];
+    active = active && (a.sub_mode == 2 ? in_sub : !in_sub);
+  }
   double rho_acc = 0.0;
   if (active) {
     const R* __restrict__ gin = static_cast<const R*>(a.gin);
@@ -445,8 +455,10 @@ template <class LAT, class ST, bool WRAP>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double lds[kBlock / 64];
   const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  step_body<LAT, ST, WRAP>(a, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y * blockDim.y + threadIdx.y,
-                           blockIdx.z, blk, lds);
+  const int o0 = (a.sub_mode == 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode == 2) ? a.sub_lo[1] : 0;
+  const int o2 = (a.sub_mode == 2) ? a.sub_lo[2] : 0;
+  step_body<LAT, ST, WRAP>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x, o1 + blockIdx.y * blockDim.y + threadIdx.y,
+                           o2 + blockIdx.z, blk, lds);
 }
 
 // Tile (bx, by) of block b of a case's grid, x fastest, as k_step's 2D grid.
@@ -760,6 +772,16 @@ __device__ __forceinline__ double sum_partials_body(const double* p, int64_t n) 
   if (threadIdx.x == 0)
     for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) r += lds[wv];
   return r;
+}
+
+// Sum(Rho) from two partial arrays (a split step, or a level and its
+// average-down corrections), in fixed order
+__global__ void __launch_bounds__(1024) k_mass_total(const double* p0, int64_t n0, const double* p1, int64_t n1,
+                                                     double* out) {
+  const double a = sum_partials_body(p0, n0);
+  __syncthreads();
+  const double b = (n1 > 0) ? sum_partials_body(p1, n1) : 0.0;
+  if (threadIdx.x == 0) *out = a + b;
 }
 
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t n, double* out) {

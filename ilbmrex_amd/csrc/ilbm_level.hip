@@ -419,10 +419,41 @@ struct Level {
   std::vector<hipEvent_t> ev_start, ev_stop;
   double last_avg_ms = 0.0;
   int last_launches = 0;
+  // Overlap of an IB level's step: the cells farther than one cell from the
+  // IB band stream-collide on a second stream while the IB forcing runs on
+  // the first; the box of the band grown by one cell follows the forcing.
+  bool overlap = true;
+  int near_lo[3] = {0, 0, 0}, near_n[3] = {0, 0, 0};
+  dim3 grid_near;
+  int64_t nblocks_near = 0;
+  double* partial_near = nullptr;
+  hipStream_t stream_b = nullptr;
+  hipEvent_t ev_a = nullptr, ev_ghost = nullptr, ev_near = nullptr, ev_b = nullptr;
+  // During an overlapped advance the force record of a fixed body runs on a
+  // third stream: step s's marker forces and export block sums go to slot
+  // s % kRecRing, which the forcing reuses only after that record is done.
+  static constexpr int kRecRing = 4;
+  int ring_slot = -1;
+  double* F_ring[kRecRing] = {};
+  double* fp_ring[kRecRing] = {};
+  int64_t fpartial_n = 0;
+  hipStream_t stream_c = nullptr;
+  hipEvent_t ev_exp = nullptr, ev_c = nullptr, ev_rec[kRecRing] = {};
 
   ~Level() {
     for (auto e : ev_start) (void)hipEventDestroy(e);
     for (auto e : ev_stop) (void)hipEventDestroy(e);
+    if (stream_b) (void)hipStreamSynchronize(stream_b);
+    if (stream_c) (void)hipStreamSynchronize(stream_c);
+    for (hipEvent_t e : {ev_a, ev_ghost, ev_near, ev_b, ev_exp, ev_c})
+      if (e) (void)hipEventDestroy(e);
+    for (int k = 0; k < kRecRing; ++k) {
+      if (ev_rec[k]) (void)hipEventDestroy(ev_rec[k]);
+      if (k > 0) { dev_free(F_ring[k]); dev_free(fp_ring[k]); }
+    }
+    if (stream_b) (void)hipStreamDestroy(stream_b);
+    if (stream_c) (void)hipStreamDestroy(stream_c);
+    dev_free(partial_near);
     dev_free(gbuf[0]); dev_free(gbuf[1]);
     for (int f = 0; f < 6; ++f) dev_free(faces.p[f]);
     dev_free(partial); dev_free(mout); dev_free(fout); dev_free(mass_dev); dev_free(force_dev);
@@ -638,7 +669,8 @@ struct Level {
     rowptr_d = dev_alloc<int>(band_cells + 1);
     const int64_t tiles = (int64_t)((band_n[0] + kGatherTile - 1) / kGatherTile) *
                           ((band_n[1] + kGatherTile - 1) / kGatherTile) * band_n[2];
-    fpartial = dev_alloc<double>(3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + tiles + 2));
+    fpartial_n = 3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + tiles + 2);
+    fpartial = dev_alloc<double>(fpartial_n);
     upload_markers();
     ncol = (c.dim == 3) ? 33 : 23;
   }
@@ -793,7 +825,7 @@ struct Level {
     a.om[0] = c.omx; a.om[1] = c.omy; a.om[2] = c.omz;
     a.box_yhi = c.box_hi[1]; a.wall_tol = mg.wall_tol;
     for (int d = 0; d < 3; ++d) a.lid_u[d] = c.box_lid_u[d];
-    a.xm = xm_d; a.ut = ut_d; a.U = U_d; a.F = F_d; a.Mm = Mm_d;
+    a.xm = xm_d; a.ut = ut_d; a.U = U_d; a.F = (ring_slot >= 0) ? F_ring[ring_slot] : F_d; a.Mm = Mm_d;
     a.base = base_d; a.wts = wts_d; a.scale = scale_d;
     a.Mband = Mband; a.EV = EV; a.EFI = EFI; a.EFT = EFT; a.Fband = Fband;
     a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
@@ -843,6 +875,8 @@ struct Level {
   // with the export of F fused with the Guo term of the coming collision and
   // with the ComputeIBForce record of the step (row `row` of the history).
   void ib_update_forcing(double time, int row) {
+    // the slot's previous record must be done before its inputs are rewritten
+    if (ring_slot >= 0) HIPCHK(hipStreamWaitEvent(stream, ev_rec[ring_slot], 0));
     if (dev_kin) {
       // the pitching kinematics on the device; the angle terms as
       // marker_positions computes them
@@ -884,7 +918,7 @@ struct Level {
       if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
     }
     RecArgs r{};
-    r.partial = fpartial; r.sums = sums_d; r.state = frstate_d;
+    r.partial = (ring_slot >= 0) ? fp_ring[ring_slot] : fpartial; r.sums = sums_d; r.state = frstate_d;
     r.hist = force_dev; r.row = row; r.ncol = ncol; r.method = c.force_eval_method;
     r.time = time + dt; r.step = (double)(istep + 1);
     r.dV = (c.dim == 3) ? c.dx[0] * c.dx[1] * c.dx[2] : c.dx[0] * c.dx[1];
@@ -897,7 +931,15 @@ struct Level {
     if (nloop == 1) {
       // the marker sums feed only the force record (a sub-step without a
       // record is followed by one that has it)
-      if (rec_enable) hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
+      if (rec_enable && ring_slot >= 0) {
+        // off the step's critical path: the record reads only its slot
+        HIPCHK(hipEventRecord(ev_exp, stream));
+        HIPCHK(hipStreamWaitEvent(stream_c, ev_exp, 0));
+        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream_c, a, r, (int)gfix);
+        HIPCHK(hipEventRecord(ev_rec[ring_slot], stream_c));
+      } else if (rec_enable) {
+        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
+      }
     } else if (rec_enable) {
       hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gfix);
     }
@@ -1086,11 +1128,91 @@ struct Level {
     }
   }
 
+  // the near box (IB band grown by one cell; the whole extent of a periodic
+  // dimension it would wrap across) and the resources of the overlapped step
+  bool split_ready() {
+    if (!overlap || timing || !ib || cf || window || band_cells <= 0) return false;
+    if (!stream_b) {
+      for (int d = 0; d < 3; ++d) {
+        int lo = band_lo[d] - 1, hi = band_lo[d] + band_n[d];
+        if (d >= c.dim) { lo = 0; hi = 0; }
+        else if (c.periodic[d] && (lo < 0 || hi > c.n[d] - 1)) { lo = 0; hi = c.n[d] - 1; }
+        else { lo = std::max(lo, 0); hi = std::min(hi, c.n[d] - 1); }
+        near_lo[d] = lo;
+        near_n[d] = hi - lo + 1;
+      }
+      grid_near = dim3((near_n[0] + block.x - 1) / block.x, (near_n[1] + block.y - 1) / block.y, near_n[2]);
+      nblocks_near = (int64_t)grid_near.x * grid_near.y * grid_near.z;
+      partial_near = dev_alloc<double>(nblocks_near);
+      HIPCHK(hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking));
+      for (hipEvent_t* e : {&ev_a, &ev_ghost, &ev_near, &ev_b})
+        HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      if (async_record()) {
+        HIPCHK(hipStreamCreateWithFlags(&stream_c, hipStreamNonBlocking));
+        for (hipEvent_t* e : {&ev_exp, &ev_c})
+          HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        F_ring[0] = F_d;
+        fp_ring[0] = fpartial;
+        for (int k = 0; k < kRecRing; ++k) {
+          HIPCHK(hipEventCreateWithFlags(&ev_rec[k], hipEventDisableTiming));
+          if (k > 0) {
+            F_ring[k] = dev_alloc<double>(3 * nm);
+            fp_ring[k] = dev_alloc<double>(fpartial_n);
+          }
+        }
+      }
+    }
+    return true;
+  }
+
+  // the record can run apart from the forcing: one iteration, a fixed body
+  bool async_record() const {
+    const int nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
+    return nloop == 1 && !dev_kin && !moving;
+  }
+
+  // One step of an IB level on two streams.  Stream B: ghost slabs, the far
+  // cells, then Sum(Rho) of both parts once the near box is done.  Stream A:
+  // IB forcing (reads the band macros of the last near box, adds the Guo
+  // term to band cells, which no far cell reads), then the near box after
+  // the ghost slabs.  The far part of the next step starts after this near
+  // box (its sources) through the wait before Sum(Rho).
+  void split_step(int s, double time, bool want) {
+    const int tb = 256;
+    StepArgs sf = step_args(want);
+    sf.sub_mode = 1;
+    for (int d = 0; d < 3; ++d) { sf.sub_lo[d] = near_lo[d]; sf.sub_n[d] = near_n[d]; }
+    StepArgs sn = sf;
+    sn.sub_mode = 2;
+    sn.partial = partial_near;
+    if (!wrap) {
+      GhostArgs ga = ghost_args();
+      const int64_t npad = slabs.face_off[slabs.nface];
+      if (npad) ghostfn(dim3((unsigned)((npad + tb - 1) / tb)), dim3(tb), stream_b, ga);
+    }
+    HIPCHK(hipEventRecord(ev_ghost, stream_b));
+    stepfn(grid, block, stream_b, sf);
+    ring_slot = stream_c ? s % kRecRing : -1;
+    ib_update_forcing(time, s);
+    ring_slot = -1;
+    HIPCHK(hipStreamWaitEvent(stream, ev_ghost, 0));
+    stepfn(grid_near, block, stream, sn);
+    HIPCHK(hipEventRecord(ev_near, stream));
+    HIPCHK(hipStreamWaitEvent(stream_b, ev_near, 0));
+    hipLaunchKernelGGL(k_mass_total, dim3(1), dim3(1024), 0, stream_b, (const double*)partial, nblocks,
+                       (const double*)partial_near, nblocks_near, mass_dev + s);
+  }
+
   void advance(int nsteps, bool want_state) {
     if (nsteps < 0) throw StatusError{ILBM_ERR_INVALID, "nsteps < 0"};
     if (nsteps == 0) return;
     if (want_state) ensure_exports();
     grow_histories(nsteps, nsteps);
+    const bool split = split_ready();
+    if (split) {
+      HIPCHK(hipEventRecord(ev_a, stream));
+      HIPCHK(hipStreamWaitEvent(stream_b, ev_a, 0));
+    }
     if (timing) {
       while ((int)ev_start.size() < nsteps) {
         hipEvent_t a, b;
@@ -1106,6 +1228,13 @@ struct Level {
       const double time = t_new;  // timeStepWithSubcycling, AmrCoreLBM.cpp:1730-1733
       t_old = t_new;
       t_new += dt;
+      if (split) {
+        split_step(s, time, want_state && s == nsteps - 1);
+        cur ^= 1;
+        ++istep;
+        HIPCHK(hipGetLastError());
+        continue;
+      }
       if (ib) ib_update_forcing(time, s);
       if (!wrap) {
         GhostArgs ga = ghost_args();
@@ -1122,6 +1251,18 @@ struct Level {
       cur ^= 1;
       ++istep;
       HIPCHK(hipGetLastError());
+    }
+    if (split) {
+      HIPCHK(hipEventRecord(ev_b, stream_b));
+      HIPCHK(hipStreamWaitEvent(stream, ev_b, 0));
+      if (stream_c) {
+        HIPCHK(hipEventRecord(ev_c, stream_c));
+        HIPCHK(hipStreamWaitEvent(stream, ev_c, 0));
+        // the marker forces of the last step where the getters read them
+        const int last = (nsteps - 1) % kRecRing;
+        if (last != 0)
+          HIPCHK(hipMemcpyAsync(F_d, F_ring[last], sizeof(double) * 3 * nm, hipMemcpyDeviceToDevice, stream));
+      }
     }
     std::vector<double> mh(nsteps);
     HIPCHK(hipMemcpyAsync(mh.data(), mass_dev, sizeof(double) * nsteps, hipMemcpyDeviceToHost, stream));
@@ -1396,6 +1537,12 @@ int ilbm_level_kernel_time(ilbm_level* h, double* avg_ms, int* launches) {
 int ilbm_level_set_timing(ilbm_level* h, int enable) {
   if (!h) return ILBM_ERR_INVALID;
   h->L.timing = enable != 0;
+  return ILBM_OK;
+}
+
+int ilbm_level_set_overlap(ilbm_level* h, int enable) {
+  if (!h) return ILBM_ERR_INVALID;
+  h->L.overlap = enable != 0;
   return ILBM_OK;
 }
 

@@ -54,7 +54,7 @@ EXPORTED = [
     "ilbm_level_mass_history", "ilbm_level_force_history", "ilbm_level_num_markers",
     "ilbm_level_marker_dv", "ilbm_level_marker_state", "ilbm_level_marker_force_sum",
     "ilbm_level_marker_torque", "ilbm_level_info", "ilbm_level_kernel_time",
-    "ilbm_level_set_timing", "ilbm_copy_probe", "ilbm_status_string", "ilbm_last_error",
+    "ilbm_level_set_timing", "ilbm_level_set_overlap", "ilbm_copy_probe", "ilbm_status_string", "ilbm_last_error",
     "ilbm_abi_version", "ilbm_case_desc_size",
     "ilbm_batch_create", "ilbm_batch_destroy", "ilbm_batch_footprint", "ilbm_batch_add",
     "ilbm_batch_advance", "ilbm_batch_finished", "ilbm_batch_progress",
@@ -111,6 +111,7 @@ def load():
     L.ilbm_level_info.argtypes = [vp, P(C.c_double)]
     L.ilbm_level_kernel_time.argtypes = [vp, P(C.c_double), P(C.c_int)]
     L.ilbm_level_set_timing.argtypes = [vp, C.c_int]
+    L.ilbm_level_set_overlap.argtypes = [vp, C.c_int]
     L.ilbm_batch_create.argtypes = [C.c_int64, C.c_int, C.c_int, P(vp)]
     L.ilbm_batch_destroy.argtypes = [vp]
     L.ilbm_batch_footprint.argtypes = [P(CaseDesc), C.c_int, P(C.c_int64)]
@@ -330,6 +331,10 @@ class Level:
 
     def set_timing(self, enable: bool):
         _check(load().ilbm_level_set_timing(self._h, int(enable)), "set_timing")
+
+    def set_overlap(self, enable: bool):
+        """IB level: far cells stream-collide on a second stream during the IB forcing."""
+        _check(load().ilbm_level_set_overlap(self._h, int(enable)), "set_overlap")
 
     def checkpoint(self) -> bytes:
         """The level's dynamic state (ilbm_level_checkpoint)."""
