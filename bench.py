@@ -57,6 +57,9 @@ def periodic_case(storage, force):
                            storage=storage, name="periodic256")
 
 
+OVERLAP = True
+
+
 def run_level(case, init, warmup, steps, want_forces=False):
     """Wall-clock MLUPS of `steps` steps (an IB level overlaps its far cells
     with the IB forcing), then the stream-collide kernel's average duration
@@ -64,6 +67,7 @@ def run_level(case, init, warmup, steps, want_forces=False):
     overlap, so that the kernel covers the whole level."""
     from ilbmrex_amd.native import Level
     lev = Level(case, init)
+    lev.set_overlap(OVERLAP)
     lev.advance(warmup)
     lev.synchronize()
     t0 = time.perf_counter()
@@ -214,6 +218,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--no-overlap", action="store_true",
+                    help="IB levels step on one stream (profiler runs of the whole-level kernel)")
     ap.add_argument("--workload", choices=["sphere", "ensemble", "amr"], default="sphere",
                     help="ensemble / amr: time only that workload (profiler runs) and print its summary")
     ap.add_argument("--probe", action="store_true",
@@ -225,6 +231,8 @@ def main():
         sys.exit(2)
     from ilbmrex_amd import native
     native.load()
+    global OVERLAP
+    OVERLAP = not args.no_overlap
     if args.workload == "ensemble":
         from ilbmrex_amd import serving
         cases = serving.ensemble_cases(256, seed=20261202)

@@ -142,10 +142,11 @@ int ilbm_level_info(ilbm_level* L, double out[8]);
  * events on the level's stream, and the number of launches it covers. */
 int ilbm_level_kernel_time(ilbm_level* L, double* avg_ms, int* launches);
 int ilbm_level_set_timing(ilbm_level* L, int enable);
-/* IB level: stream-collide the cells farther than one cell from the IB band
- * on a second stream while the IB forcing runs (default on; off while
- * timing is on).  The results are the same either way; Sum(Rho) is summed in
- * another order.  No counterpart in the reference. */
+/* IB level of at least 2^20 cells: stream-collide the cells farther than
+ * one cell from the IB band on a second stream while the IB forcing runs
+ * (default on; off while timing is on).  The results are the same either
+ * way; Sum(Rho) is summed in another order.  No counterpart in the
+ * reference. */
 int ilbm_level_set_overlap(ilbm_level* L, int enable);
 
 /* Measurement helper (no counterpart in the reference): average time of a

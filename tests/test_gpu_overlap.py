@@ -1,6 +1,7 @@
 """The overlapped IB step (far cells on a second stream during the IB
-forcing) gives the same populations, macros and force records as the
-single-stream step; Sum(Rho) is summed in another order."""
+forcing; levels of at least 2^20 cells) gives the same populations, macros
+and force records as the single-stream step; Sum(Rho) is summed in another
+order."""
 import numpy as np
 import pytest
 
@@ -23,8 +24,8 @@ def _pair(case, init, nsteps):
 
 @pytest.mark.parametrize("storage", [cf.STORE_F64, cf.STORE_F32])
 def test_overlap_matches_single_stream_cylinder(storage):
-    c = cf.cylinder_2d(nx=128, ny=64, D=12.0, xc=40.0, yc=32.0, U=0.05, Re=100.0, storage=storage)
-    print(f"[overlap] cylinder 128x64 storage={storage}", flush=True)
+    c = cf.cylinder_2d(nx=1024, ny=1024, D=24.0, xc=200.0, yc=512.0, U=0.05, Re=100.0, storage=storage)
+    print(f"[overlap] cylinder 1024x1024 storage={storage}", flush=True)
     (f1, m1, h1, s1), (f2, m2, h2, s2) = _pair(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)), 40)
     assert np.array_equal(f1, f2) and np.array_equal(m1, m2) and np.array_equal(h1, h2)
     np.testing.assert_allclose(s1, s2, rtol=1e-13)
@@ -32,9 +33,9 @@ def test_overlap_matches_single_stream_cylinder(storage):
 
 def test_overlap_matches_single_stream_sphere_near_wall():
     # the band reaches the inflow face: the near box is clipped there
-    c = cf.sphere_3d(nx=48, ny=32, nz=32, D=8.0, center=(6.0, 16.0, 16.0), U=0.04, Re=40.0,
+    c = cf.sphere_3d(nx=128, ny=96, nz=96, D=8.0, center=(6.0, 48.0, 48.0), U=0.04, Re=40.0,
                      storage=cf.STORE_F32)
-    print("[overlap] sphere 48x32x32 FP32 near the inflow", flush=True)
+    print("[overlap] sphere 128x96x96 FP32 near the inflow", flush=True)
     (f1, m1, h1, s1), (f2, m2, h2, s2) = _pair(c, cf.uniform_init(c, u=(0.04, 0.0, 0.0)), 20)
     assert np.array_equal(f1, f2) and np.array_equal(m1, m2) and np.array_equal(h1, h2)
     np.testing.assert_allclose(s1, s2, rtol=1e-12)
@@ -42,7 +43,7 @@ def test_overlap_matches_single_stream_sphere_near_wall():
 
 def test_overlap_band_across_periodic_boundary():
     # body at the periodic y edge: the near box spans the whole y extent
-    c = cf.cylinder_2d(nx=96, ny=48, D=10.0, xc=30.0, yc=2.0, U=0.05, Re=60.0, storage=cf.STORE_F64)
+    c = cf.cylinder_2d(nx=1024, ny=1024, D=10.0, xc=30.0, yc=2.0, U=0.05, Re=60.0, storage=cf.STORE_F64)
     print("[overlap] cylinder across the periodic y boundary", flush=True)
     (f1, m1, h1, s1), (f2, m2, h2, s2) = _pair(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)), 20)
     assert np.array_equal(f1, f2) and np.array_equal(m1, m2) and np.array_equal(h1, h2)
