@@ -117,7 +117,7 @@ static int64_t case_footprint(const ilbm_case_desc& c0, int nsteps) {
     slab += 2 * f;
     faces += 2 * f;
   }
-  int64_t b = 2 * (int64_t)pb * Q * (qs + slab) + (int64_t)pb * Q * faces;
+  int64_t b = 2 * (int64_t)pb * Q * (qs + slab) + (int64_t)pb * face_stride(Q, pb) * faces;
   b += 8 * (cells / 64 + 64);                          // block partials
   if (c.use_ib) {
     // band: stencil box of the body, markers and their lists

@@ -64,6 +64,28 @@ struct FaceBuf {
   int dims[6][2];  // the two in-plane dims of each face (-1 if unused)
 };
 
+// A face cell's Q populations are one record of face_stride elements (the
+// record padded to 16 bytes): the boundary cell writes it with 16-byte
+// stores instead of Q scattered ones.
+__host__ __device__ constexpr int face_stride(int q, int pop_bytes) { return ((q * pop_bytes + 15) / 16 * 16) / pop_bytes; }
+
+template <class LAT, class ST>
+__device__ __forceinline__ void face_store(typename ST::raw* fb, int64_t fi, const typename ST::compute_t* f) {
+  using R = typename ST::raw;
+  using T = typename ST::compute_t;
+  constexpr int QP = face_stride(LAT::Q, (int)sizeof(R));
+  alignas(16) R rec[QP];
+#pragma unroll
+  for (int q = 0; q < QP; ++q) {
+    if (q < LAT::Q) ST::store(rec, q, f[q], T(LAT::w[q]));
+    else rec[q] = R(0);
+  }
+  uint4* dst = reinterpret_cast<uint4*>(fb + fi * QP);
+  const uint4* src = reinterpret_cast<const uint4*>(rec);
+#pragma unroll
+  for (int c = 0; c < QP * (int)sizeof(R) / 16; ++c) dst[c] = src[c];
+}
+
 __host__ __device__ inline int64_t face_index(const GridGeom& g, int d, int i, int j, int k) {
   const int c[3] = {i, j, k};
   if (d == 0) return (int64_t)c[2] * g.n[1] + c[1];
@@ -120,6 +142,7 @@ struct StepArgs {
   // 2 the cells of the box, the launch grid covering the box from sub_lo
   int sub_mode;
   int sub_lo[3], sub_n[3];
+  int off32;             // a plane's cells are addressable with 32-bit byte offsets
 };
 
 // Block of a batched launch: case index into the launch's argument table and
@@ -305,17 +328,36 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         f[q] = ST::load(p, 0, T(LAT::w[q]));
         continue;
       }
-      int64_t off = (int64_t)q * g.qs + g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
-      if constexpr (!WRAP) {
-        const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
-        // slab of the lowest non-periodic dimension the source leaves
-        const int64_t t0 = tb[0] - LAT::cy[q] * mu[0][1] - LAT::cz[q] * mu[0][2];
-        const int64_t t1 = tb[1] - LAT::cx[q] * mu[1][0] - LAT::cz[q] * mu[1][2];
-        const int64_t t2 = tb[2] - LAT::cx[q] * mu[2][0] - LAT::cy[q] * mu[2][1];
-        const int64_t t = o0 ? t0 : (o1 ? t1 : t2);
-        off = (o0 | o1 | o2) ? gofs + (int64_t)q * npad + t : off;
+      // the source cell inside the level (a ghost source points at the
+      // wrapped cell and is replaced below): a cell offset from the plane's
+      // uniform base
+      const uint32_t src = (uint32_t)(g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
+      const R* plane = gin + (int64_t)q * g.qs;
+      if (a.off32) {
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
+        f[q] = ST::load(p, 0, T(LAT::w[q]));
+      } else {
+        f[q] = ST::load(plane, (int64_t)src, T(LAT::w[q]));
       }
-      f[q] = ST::load(gin, off, T(LAT::w[q]));
+    }
+    if constexpr (!WRAP) {
+      // a cell on a non-periodic face: the sources outside the face come
+      // from the ghost slabs (only such lanes take this branch)
+      const bool edge = outx[0] | outx[2] | outy[0] | outy[2] | outz[0] | outz[2];
+      if (edge) {
+#pragma unroll
+        for (int q = 0; q < LAT::Q; ++q) {
+          const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
+          if (o0 | o1 | o2) {
+            // slab of the lowest non-periodic dimension the source leaves
+            const int64_t t0 = tb[0] - LAT::cy[q] * mu[0][1] - LAT::cz[q] * mu[0][2];
+            const int64_t t1 = tb[1] - LAT::cx[q] * mu[1][0] - LAT::cz[q] * mu[1][2];
+            const int64_t t2 = tb[2] - LAT::cx[q] * mu[2][0] - LAT::cy[q] * mu[2][1];
+            const int64_t t = o0 ? t0 : (o1 ? t1 : t2);
+            f[q] = ST::load(gin, gofs + (int64_t)q * npad + t, T(LAT::w[q]));
+          }
+        }
+      }
     }
     // moments in the reference's order (calculateMacroForcing.H:21-27); a
     // term whose direction component is 0 is left out and -1 * f is
@@ -372,17 +414,10 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       else if (cc[d] == g.n[d] - 1) side = 1;
       if (side < 0) continue;
       R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
-      int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
-                                                                 : (int64_t)g.n[0] * g.n[1];
       const int64_t fi = face_index(g, d, i, j, k);
-#pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) ST::store(fb, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
+      face_store<LAT, ST>(fb, fi, f);
       // a cell sitting on the lo and hi face of a 1-cell dimension writes both
-      if (g.n[d] == 1) {
-        R* fb2 = static_cast<R*>(a.faces.p[2 * d + 1]);
-#pragma unroll
-        for (int q = 0; q < LAT::Q; ++q) ST::store(fb2, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
-      }
+      if (g.n[d] == 1) face_store<LAT, ST>(static_cast<R*>(a.faces.p[2 * d + 1]), fi, f);
     }
     // BGK collision with the macros just computed (feq uses macro_old of the
     // next step, AdvanceLBMAtLevel.cpp:225-238)
@@ -392,7 +427,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const T fac = T(1.0) - T(0.5) / T(a.tau);
     const int64_t self = g.off0 + ox[1] + oy[1] + oz[1];
     auto put = [&](int q, T gq) {
-      if constexpr (WRAP) {
+      if (WRAP || a.off32) {
         R* plane = gout + (int64_t)q * g.qs;
         R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
         ST::store_nt(p, 0, gq, T(LAT::w[q]));
@@ -547,11 +582,7 @@ __global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
       for (int side = 0; side < 2; ++side) {
         if (cc[d] != (side ? g.n[d] - 1 : 0)) continue;
         R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
-        int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
-                                                                   : (int64_t)g.n[0] * g.n[1];
-        const int64_t fi = face_index(g, d, i, j, k);
-#pragma unroll
-        for (int q = 0; q < LAT::Q; ++q) ST::store(fb, (int64_t)q * fn + fi, f[q], T(LAT::w[q]));
+        face_store<LAT, ST>(fb, face_index(g, d, i, j, k), f);
       }
     }
     const T r = T(rho0), u = T(u0), v = T(v0), w = T(w0);
@@ -617,11 +648,10 @@ __device__ inline void load_face(const GhostArgs& a, const int c[3], typename ST
     else if (c[d] == g.n[d] - 1) side = 1;
     if (side < 0) continue;
     const R* fb = static_cast<const R*>(a.faces.p[2 * d + side]);
-    const int64_t fn = (d == 0) ? (int64_t)g.n[1] * g.n[2] : (d == 1) ? (int64_t)g.n[0] * g.n[2]
-                                                                     : (int64_t)g.n[0] * g.n[1];
+    constexpr int QP = face_stride(LAT::Q, (int)sizeof(R));
     const int64_t fi = face_index(g, d, c[0], c[1], c[2]);
 #pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) out[q] = ST::load(fb, (int64_t)q * fn + fi, T(LAT::w[q]));
+    for (int q = 0; q < LAT::Q; ++q) out[q] = ST::load(fb, fi * QP + q, T(LAT::w[q]));
     return;
   }
 #pragma unroll

@@ -546,7 +546,8 @@ struct Level {
       if (!nonper[d]) continue;
       const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
                                                                        : (int64_t)c.n[0] * c.n[1];
-      for (int side = 0; side < 2; ++side) faces.p[2 * d + side] = dev_alloc_bytes(popbytes * (size_t)fn * Q);
+      for (int side = 0; side < 2; ++side)
+        faces.p[2 * d + side] = dev_alloc_bytes(popbytes * (size_t)fn * face_stride(Q, (int)popbytes));
     }
     const int bx = std::min(256, (c.n[0] + 63) / 64 * 64);
     const int by = 256 / bx;
@@ -981,6 +982,7 @@ struct Level {
     a.partial = partial;
     a.slabs = slabs;
     a.tile_bx = (int)block.x;
+    a.off32 = ((int64_t)popbytes * g.qs < ((int64_t)1 << 32)) ? 1 : 0;
     return a;
   }
 
@@ -1008,7 +1010,7 @@ struct Level {
       if (!nonper[d]) continue;
       const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
                                                                      : (int64_t)c.n[0] * c.n[1];
-      b += 2 * (int64_t)popbytes * fn * c.q;
+      b += 2 * (int64_t)popbytes * fn * face_stride(c.q, (int)popbytes);
     }
     return b;
   }
@@ -1040,7 +1042,7 @@ struct Level {
       const int d = f / 2;
       const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
                                                                      : (int64_t)c.n[0] * c.n[1];
-      const int64_t b = (int64_t)popbytes * fn * c.q;
+      const int64_t b = (int64_t)popbytes * fn * face_stride(c.q, (int)popbytes);
       HIPCHK(hipMemcpy(p, faces.p[f], b, hipMemcpyDeviceToHost));
       p += b;
     }
@@ -1071,7 +1073,7 @@ struct Level {
       const int d = f / 2;
       const int64_t fn = (d == 0) ? (int64_t)c.n[1] * c.n[2] : (d == 1) ? (int64_t)c.n[0] * c.n[2]
                                                                      : (int64_t)c.n[0] * c.n[1];
-      const int64_t b = (int64_t)popbytes * fn * c.q;
+      const int64_t b = (int64_t)popbytes * fn * face_stride(c.q, (int)popbytes);
       HIPCHK(hipMemcpy(faces.p[f], p, b, hipMemcpyHostToDevice));
       p += b;
     }

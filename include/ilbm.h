@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define ILBM_ABI_VERSION 1
+#define ILBM_ABI_VERSION 2
 
 typedef enum {
   ILBM_OK = 0,

@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(native.EXPORTED) <= set(names)
-    assert lib.ilbm_abi_version() == 1
+    assert lib.ilbm_abi_version() == 2
 
 
 def test_case_desc_layout_matches_binding():
