@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
   const double r = fmax(m[0], 1.0e-12);
   const double u0 = m[1], v0 = m[2], w0 = m[3];
   const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
-  R* out = static_cast<R*>(a.gbuf) + (int64_t)LAT::Q * a.g.qs;
+  R* out = static_cast<R*>(a.gbuf) + a.g.pop_n;
 #pragma unroll
   for (int q = 0; q < LAT::Q; ++q) {
     const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;

@@ -51,7 +51,8 @@ struct GridGeom {
   int pad;         // ghost layers stored per dimension (0 or 1)
   int dim;
   int64_t js, ks;  // strides of the stored grid
-  int64_t qs;      // elements per population plane
+  int64_t qs;      // elements between population q and q+1 of a cell
+  int64_t pop_n;   // elements of all populations (the ghost slabs follow)
   int64_t off0;    // offset of valid cell (0,0,0)
   __host__ __device__ int64_t idx(int i, int j, int k) const {
     return off0 + (int64_t)k * ks + (int64_t)j * js + i;
@@ -314,7 +315,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         tb[d] = t;
       }
     }
-    const int64_t gofs = (int64_t)LAT::Q * g.qs;          // start of the slabs
+    const int64_t gofs = g.pop_n;                       // start of the slabs
     const int64_t npad = a.slabs.face_off[a.slabs.nface];
     T f[LAT::Q];
 #pragma unroll
@@ -331,13 +332,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       // the source cell inside the level (a ghost source points at the
       // wrapped cell and is replaced below): a cell offset from the plane's
       // uniform base
-      const uint32_t src = (uint32_t)(g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
+      const int64_t src = g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
       const R* plane = gin + (int64_t)q * g.qs;
       if (a.off32) {
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) +
+                                                (uint32_t)src * (uint32_t)sizeof(R));
         f[q] = ST::load(p, 0, T(LAT::w[q]));
       } else {
-        f[q] = ST::load(plane, (int64_t)src, T(LAT::w[q]));
+        f[q] = ST::load(plane, src, T(LAT::w[q]));
       }
     }
     if constexpr (!WRAP) {
@@ -755,7 +757,7 @@ __device__ __forceinline__ void ghost_body(const GhostArgs& a, int64_t t) {
   int iv[3];
   slab_cell(a, t, iv);
   const R* gb = static_cast<const R*>(a.gbuf);
-  R* out = static_cast<R*>(a.gbuf) + (int64_t)LAT::Q * g.qs;
+  R* out = static_cast<R*>(a.gbuf) + g.pop_n;
   bool all_user1 = true;
   for (int d = 0; d < LAT::D; ++d) {
     const bool lo = iv[d] < 0, hi = iv[d] > g.n[d] - 1;

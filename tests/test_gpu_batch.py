@@ -168,3 +168,31 @@ def test_ensemble_fp32_cd_within_half_percent_of_fp64():
     for a, b in zip(h32, h64):
         cd32, cd64 = a[-500:, 4].mean(), b[-500:, 4].mean()
         assert np.isfinite(cd32) and abs(cd32 - cd64) <= 5e-3 * abs(cd64), (cd32, cd64)
+
+
+def test_batch_large_3d_slice_layout_matches_level():
+    """A 3D case of 2^20 cells or more (z-slice population layout) next to a
+    2D case in one batch: histories identical to the single level's, the
+    record within 1e-12 of the oracle."""
+    from ilbmrex_amd.native import Batch, Level
+    big = cf.sphere_3d(nx=128, ny=96, nz=96, D=12.0, center=(40.0, 48.3, 47.8), U=0.04, Re=50.0)
+    small = mixed_cases(cf.STORE_F64)[0]
+    nsteps = 4
+    print("[batch] 128x96x96 sphere + 96x64 cylinder, FP64", flush=True)
+    with Batch(BUDGET, graph_steps=2) as B:
+        ids = [B.add(c, _init(c), nsteps) for c in (big, small)]
+        done = 0
+        while done < nsteps:
+            done += B.advance(nsteps - done)
+        for c, cid in zip((big, small), ids):
+            fh_b = B.force_history(cid)
+            lev = Level(c, _init(c))
+            lev.advance(nsteps)
+            assert np.array_equal(fh_b, lev.force_history()), c.name
+            lev.close()
+            if c is big:
+                fh_big = fh_b
+    orc = refcpu.Oracle(big, _init(big))
+    orc.advance(nsteps)
+    rec = orc.force_record()
+    assert np.all(np.abs(fh_big[-1] - rec) <= 1e-12 * np.abs(rec) + 1e-10 * np.abs(rec).max())

@@ -217,3 +217,17 @@ def test_reduced_storage_against_fp64(storage, tol):
     err = np.linalg.norm(u - uo) / np.linalg.norm(uo)
     print(f"storage {storage}: rel L2(u) = {err:.3e}")
     assert err < tol
+
+
+def test_large_3d_slice_layout_bitexact():
+    """3D levels of 2^20 cells or more store z-slices of all populations
+    (and overlap the IB step): a 128x96x96 sphere with walls and a periodic
+    128x96x96 box, FP64, against the oracle."""
+    c = cf.sphere_3d(nx=128, ny=96, nz=96, D=12.0, center=(40.0, 48.3, 47.8), U=0.04, Re=50.0)
+    assert c.ncells >= 1 << 20
+    lev, orc = run_pair(c, cf.uniform_init(c, u=(0.04, 0.0, 0.0)), 3)
+    _ib_checks(lev, orc)
+    p = cf.periodic_box(cf.D3Q19, (128, 96, 96), nu=0.05, force=(1e-5, 0.0, 0.0))
+    lev, orc = run_pair(p, cf.random_init(p, seed=3), 3)
+    assert_bitexact(lev.f(), orc.f(), "f")
+    assert_bitexact(lev.macro(), orc.macro(), "macro")
