@@ -107,7 +107,8 @@ static int64_t case_footprint(const ilbm_case_desc& c0, int nsteps) {
   const int Q = c.q;
   const int64_t cells = (int64_t)c.n[0] * c.n[1] * c.n[2];
   const int pb = (c.storage == 0) ? 8 : (c.storage == 1) ? 4 : 2;
-  const int64_t qs = slice_layout(c.dim, cells) ? ((int64_t)c.n[0] * c.n[1] + 127) / 128 * 128 * c.n[2]
+  const int64_t sl = (int64_t)c.n[0] * c.n[1];
+  const int64_t qs = slice_layout(c.dim, cells) ? ((sl * pb + 16383) / 16384 * 16384 + 8192) / pb * c.n[2]
                                                  : plane_stride(cells, pb);
   int64_t slab = 0, faces = 0;
   for (int d = 0; d < c.dim; ++d) {
