@@ -11,7 +11,9 @@ CSRC = os.path.join(PKG_DIR, "csrc")
 OUT = os.path.join(PKG_DIR, "_lib", "libilbm_hip.so")
 
 SOURCES = ["ilbm_level.hip"]
-HEADERS = ["ilbm_lattice.h", "ilbm_lbm_kernels.h", "ilbm_ib_kernels.h"]
+# headers the one translation unit includes
+HEADERS = ["ilbm_lattice.h", "ilbm_lbm_kernels.h", "ilbm_ib_kernels.h", "ilbm_amr_kernels.h",
+           "ilbm_batch_impl.h", "ilbm_box_impl.h", "ilbm_amr_impl.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -25,9 +25,9 @@ using StepBFn = void (*)(dim3, hipStream_t, const StepArgs*, const BlockRef*);
 using GhostBFn = void (*)(dim3, hipStream_t, const GhostArgs*, const BlockRef*);
 using FixBFn = void (*)(dim3, hipStream_t, const IbCase*, const BlockRef*);
 
-template <class LAT, class ST, bool W>
+template <class LAT, class ST, int AM>
 static void launch_step_b(dim3 g, hipStream_t s, const StepArgs* a, const BlockRef* m) {
-  hipLaunchKernelGGL((k_step_b<LAT, ST, W>), g, dim3(kBlock), 0, s, a, m);
+  hipLaunchKernelGGL((k_step_b<LAT, ST, AM>), g, dim3(kBlock), 0, s, a, m);
 }
 template <class LAT, class ST>
 static void launch_ghost_b(dim3 g, hipStream_t s, const GhostArgs* a, const BlockRef* m) {
@@ -44,19 +44,20 @@ static void launch_fixw_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef
 }
 
 template <class LAT, class ST>
-static void pick_b(bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
-  sf = wrap ? launch_step_b<LAT, ST, true> : launch_step_b<LAT, ST, false>;
+static void pick_b(int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
+  sf = (am == kAddrWrap) ? launch_step_b<LAT, ST, kAddrWrap>
+     : (am == kAddrWall32) ? launch_step_b<LAT, ST, kAddrWall32> : launch_step_b<LAT, ST, kAddrWall64>;
   gf = launch_ghost_b<LAT, ST>;
   ff = launch_fix_b<LAT, ST>;
   fw = launch_fixw_b<LAT, ST>;
 }
 
-static void pick_class(int lat, int st, bool wrap, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
+static void pick_class(int lat, int st, int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
   auto pk = [&](auto latt) {
     using L = decltype(latt);
-    if (st == 0) pick_b<L, StF64>(wrap, sf, gf, ff, fw);
-    else if (st == 1) pick_b<L, StF32>(wrap, sf, gf, ff, fw);
-    else pick_b<L, StF16S>(wrap, sf, gf, ff, fw);
+    if (st == 0) pick_b<L, StF64>(am, sf, gf, ff, fw);
+    else if (st == 1) pick_b<L, StF32>(am, sf, gf, ff, fw);
+    else pick_b<L, StF16S>(am, sf, gf, ff, fw);
   };
   if (lat == 0) pk(D2Q9{});
   else if (lat == 1) pk(D3Q19{});
@@ -75,7 +76,7 @@ struct BatchCase {
 // Device tables of one (lattice, storage, layout) class.
 struct BatchGroup {
   int lat = 0, st = 0;
-  bool wrap = true;
+  int am = kAddrWrap;  // stream-collide addressing of the class's cases
   StepBFn step = nullptr;
   GhostBFn ghost = nullptr;
   FixBFn fix = nullptr;
@@ -196,7 +197,7 @@ struct Batch {
   // level of the case's own shape (no IB) and the fastest is kept for the
   // process, so every later case of the bucket uses the same tiling.
   int tile_for(const Level& ref) {
-    const int64_t key = ((int64_t)ref.lat * 4 + ref.st) * 2 * 16 + (ref.wrap ? 16 : 0) + bucket(ref.c.n[0]);
+    const int64_t key = (((int64_t)ref.lat * 4 + ref.st) * 4 + ref.am) * 16 + bucket(ref.c.n[0]);
     auto& tiles = tile_table();
     auto it = tiles.find(key);
     if (it != tiles.end()) return it->second;
@@ -222,7 +223,7 @@ struct Batch {
       for (int bx : cand) maxblk = std::max(maxblk, T.tile_blocks(bx));
       T.ensure_partial(maxblk);
       StepBFn sf; GhostBFn gf; FixBFn ff, fw;
-      pick_class(T.lat, T.st, T.wrap, sf, gf, ff, fw);
+      pick_class(T.lat, T.st, T.am, sf, gf, ff, fw);
       StepArgs* da = dev_alloc<StepArgs>(1);
       BlockRef* dm = dev_alloc<BlockRef>(maxblk);
       hipEvent_t e0, e1;
@@ -351,11 +352,11 @@ struct Batch {
       bc.base_cur = L.cur;
       int gi = -1;
       for (size_t k = 0; k < groups.size(); ++k)
-        if (groups[k].lat == L.lat && groups[k].st == L.st && groups[k].wrap == L.wrap) gi = (int)k;
+        if (groups[k].lat == L.lat && groups[k].st == L.st && groups[k].am == L.am) gi = (int)k;
       if (gi < 0) {
         BatchGroup g;
-        g.lat = L.lat; g.st = L.st; g.wrap = L.wrap;
-        pick_class(g.lat, g.st, g.wrap, g.step, g.ghost, g.fix, g.fixw);
+        g.lat = L.lat; g.st = L.st; g.am = L.am;
+        pick_class(g.lat, g.st, g.am, g.step, g.ghost, g.fix, g.fixw);
         groups.push_back(g);
         gi = (int)groups.size() - 1;
       }

@@ -27,6 +27,7 @@
 // registers.
 #pragma once
 
+#include <type_traits>
 #include <utility>
 
 #include "ilbm_lattice.h"
@@ -34,6 +35,13 @@
 namespace ilbm {
 
 constexpr int kBlock = 256;
+
+// Population addressing of the stream-collide kernel:
+//   kAddrWall64 a level with non-periodic faces, 64-bit element offsets
+//   kAddrWrap   an all-periodic level, 32-bit byte offsets from a plane base
+//   kAddrWall32 a level with non-periodic faces whose populations and ghost
+//               slabs span under 4 GiB: 32-bit byte offsets from the buffer
+enum : int { kAddrWall64 = 0, kAddrWrap = 1, kAddrWall32 = 2 };
 
 // f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>):
 // a loop over the directions whose index is a constant expression
@@ -255,10 +263,15 @@ __device__ inline void finish_mass(const StepArgs& a, double s, int64_t blk) {
 // ---------------------------------------------------------------------------
 // k_step: pull-stream + forced macros + BGK collision, one pass.
 // ---------------------------------------------------------------------------
-template <class LAT, class ST, bool WRAP>
+template <class LAT, class ST, int AM>
 __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k, int64_t blk, double* lds) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
+  constexpr bool WRAP = AM == kAddrWrap;
+  // offsets in 32-bit unsigned arithmetic: every intermediate is exact
+  // modulo 2^32 and every final offset lies below 2^32
+  constexpr bool O32 = AM != kAddrWall64;
+  using I = std::conditional_t<O32, uint32_t, int64_t>;
   const GridGeom& g = a.g;
   bool active = (i < g.n[0]) && (j < g.n[1]);
   if (a.sub_mode != 0) {
@@ -273,15 +286,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     R* __restrict__ gout = static_cast<R*>(a.gout);
     // source offsets for c = -1, 0, +1 (source coordinate = x - c); periodic
     // dimensions wrap
-    int64_t ox[3], oy[3], oz[3];
+    I ox[3], oy[3], oz[3];
     {
       const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
       const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
-      ox[0] = xp; ox[1] = i; ox[2] = xm;
-      oy[0] = (int64_t)yp * g.js; oy[1] = (int64_t)j * g.js; oy[2] = (int64_t)ym * g.js;
+      ox[0] = (I)xp; ox[1] = (I)i; ox[2] = (I)xm;
+      oy[0] = (I)yp * (I)g.js; oy[1] = (I)j * (I)g.js; oy[2] = (I)ym * (I)g.js;
       if constexpr (LAT::D == 3) {
         const int zm = (k == 0) ? g.n[2] - 1 : k - 1, zp = (k == g.n[2] - 1) ? 0 : k + 1;
-        oz[0] = (int64_t)zp * g.ks; oz[1] = (int64_t)k * g.ks; oz[2] = (int64_t)zm * g.ks;
+        oz[0] = (I)zp * (I)g.ks; oz[1] = (I)k * (I)g.ks; oz[2] = (I)zm * (I)g.ks;
       } else {
         oz[0] = oz[1] = oz[2] = 0;
       }
@@ -293,7 +306,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     // that is also outside a periodic dimension is not wrapped.  All loads
     // are issued unconditionally; only their offsets are selected.
     bool outx[3] = {false, false, false}, outy[3] = {false, false, false}, outz[3] = {false, false, false};
-    int64_t tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    I tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     if constexpr (!WRAP) {
       const int cc[3] = {i, j, k};
       bool* outs[3] = {outx, outy, outz};
@@ -304,19 +317,19 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         outs[d][2] = (cc[d] == 0);
         const int sd = (cc[d] == 0) ? 0 : 1;
         const int f = a.slabs.slab_of[d][sd];
-        int64_t t = a.slabs.face_off[f], mul = 1;
+        I t = (I)a.slabs.face_off[f], mul = 1;
 #pragma unroll
         for (int e = 0; e < 3; ++e) {
           if (e == d) continue;
-          t += (int64_t)(cc[e] - a.slabs.face_lo[f][e]) * mul;
+          t += (I)(cc[e] - a.slabs.face_lo[f][e]) * mul;
           mu[d][e] = mul;
-          mul *= a.slabs.face_n[f][e];
+          mul *= (I)a.slabs.face_n[f][e];
         }
         tb[d] = t;
       }
     }
-    const int64_t gofs = g.pop_n;                       // start of the slabs
-    const int64_t npad = a.slabs.face_off[a.slabs.nface];
+    const I gofs = (I)g.pop_n;                          // start of the slabs
+    const I npad = (I)a.slabs.face_off[a.slabs.nface];
     T f[LAT::Q];
 #pragma unroll
     for (int q = 0; q < LAT::Q; ++q) {
@@ -329,36 +342,21 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         f[q] = ST::load(p, 0, T(LAT::w[q]));
         continue;
       }
-      // the source cell inside the level (a ghost source points at the
-      // wrapped cell and is replaced below): a cell offset from the plane's
-      // uniform base
-      const int64_t src = g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
-      const R* plane = gin + (int64_t)q * g.qs;
-      if (a.off32) {
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) +
-                                                (uint32_t)src * (uint32_t)sizeof(R));
+      // the source cell inside the level, or for a source beyond a
+      // non-periodic face the slab of the lowest such dimension it leaves
+      // (a per-lane select: one load per direction on every lane)
+      I off = (I)q * (I)g.qs + (I)g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+      const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
+      const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
+      const I t1 = tb[1] - (I)LAT::cx[q] * mu[1][0] - (I)LAT::cz[q] * mu[1][2];
+      const I t2 = tb[2] - (I)LAT::cx[q] * mu[2][0] - (I)LAT::cy[q] * mu[2][1];
+      const I t = o0 ? t0 : (o1 ? t1 : t2);
+      off = (o0 | o1 | o2) ? gofs + (I)q * npad + t : off;
+      if constexpr (O32) {
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * (uint32_t)sizeof(R));
         f[q] = ST::load(p, 0, T(LAT::w[q]));
       } else {
-        f[q] = ST::load(plane, src, T(LAT::w[q]));
-      }
-    }
-    if constexpr (!WRAP) {
-      // a cell on a non-periodic face: the sources outside the face come
-      // from the ghost slabs (only such lanes take this branch)
-      const bool edge = outx[0] | outx[2] | outy[0] | outy[2] | outz[0] | outz[2];
-      if (edge) {
-#pragma unroll
-        for (int q = 0; q < LAT::Q; ++q) {
-          const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
-          if (o0 | o1 | o2) {
-            // slab of the lowest non-periodic dimension the source leaves
-            const int64_t t0 = tb[0] - LAT::cy[q] * mu[0][1] - LAT::cz[q] * mu[0][2];
-            const int64_t t1 = tb[1] - LAT::cx[q] * mu[1][0] - LAT::cz[q] * mu[1][2];
-            const int64_t t2 = tb[2] - LAT::cx[q] * mu[2][0] - LAT::cy[q] * mu[2][1];
-            const int64_t t = o0 ? t0 : (o1 ? t1 : t2);
-            f[q] = ST::load(gin, gofs + (int64_t)q * npad + t, T(LAT::w[q]));
-          }
-        }
+        f[q] = ST::load(gin, off, T(LAT::w[q]));
       }
     }
     // moments in the reference's order (calculateMacroForcing.H:21-27); a
@@ -427,14 +425,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const T cfx = T(a.Fcol[0]), cfy = T(a.Fcol[1]), cfz = T(a.Fcol[2]);
     const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
     const T fac = T(1.0) - T(0.5) / T(a.tau);
-    const int64_t self = g.off0 + ox[1] + oy[1] + oz[1];
+    const I self = (I)g.off0 + ox[1] + oy[1] + oz[1];
     auto put = [&](int q, T gq) {
-      if (WRAP || a.off32) {
+      if (O32 || a.off32) {
         R* plane = gout + (int64_t)q * g.qs;
         R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
         ST::store_nt(p, 0, gq, T(LAT::w[q]));
       } else {
-        ST::store_nt(gout, (int64_t)q * g.qs + self, gq, T(LAT::w[q]));
+        ST::store_nt(gout, (int64_t)q * g.qs + (int64_t)self, gq, T(LAT::w[q]));
       }
     };
     if constexpr (ST::exact) {
@@ -488,13 +486,13 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
   finish_mass(a, s, blk);
 }
 
-template <class LAT, class ST, bool WRAP>
+template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double lds[kBlock / 64];
   const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   const int o0 = (a.sub_mode == 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode == 2) ? a.sub_lo[1] : 0;
   const int o2 = (a.sub_mode == 2) ? a.sub_lo[2] : 0;
-  step_body<LAT, ST, WRAP>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x, o1 + blockIdx.y * blockDim.y + threadIdx.y,
+  step_body<LAT, ST, AM>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x, o1 + blockIdx.y * blockDim.y + threadIdx.y,
                            o2 + blockIdx.z, blk, lds);
 }
 
@@ -511,14 +509,14 @@ __device__ inline void batch_tile(const GridGeom& g, int bx, int b, int* i, int*
 }
 
 // Many cases of one lattice / storage class in one launch.
-template <class LAT, class ST, bool WRAP>
+template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map) {
   __shared__ double lds[kBlock / 64];
   const BlockRef r = map[blockIdx.x];
   const StepArgs& a = A[r.c];
   int i, j, k;
   batch_tile(a.g, a.tile_bx, r.b, &i, &j, &k);
-  step_body<LAT, ST, WRAP>(a, i, j, k, r.b, lds);
+  step_body<LAT, ST, AM>(a, i, j, k, r.b, lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -530,7 +528,7 @@ struct InitArgs {
   const double* macro0; // [4][cells] device
 };
 
-template <class LAT, class ST, bool WRAP>
+template <class LAT, class ST>
 __global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;

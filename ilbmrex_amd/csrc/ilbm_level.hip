@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -284,13 +285,13 @@ using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
 using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 
-template <class LAT, class ST, bool W>
+template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a) {
-  hipLaunchKernelGGL((k_step<LAT, ST, W>), g, b, 0, s, a);
+  hipLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, a);
 }
-template <class LAT, class ST, bool W>
+template <class LAT, class ST>
 static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
-  hipLaunchKernelGGL((k_init<LAT, ST, W>), g, b, 0, s, a);
+  hipLaunchKernelGGL((k_init<LAT, ST>), g, b, 0, s, a);
 }
 // mode 0: export of EulerForceIter / Total; 1: fused CSR spreading (thread
 // per band cell); 2: fused gather spreading of a moving body; 3: fused CSR
@@ -309,9 +310,10 @@ static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
 }
 
 template <class LAT, class ST>
-static void pick(bool wrap, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf) {
-  sf = wrap ? launch_step<LAT, ST, true> : launch_step<LAT, ST, false>;
-  inf = wrap ? launch_init<LAT, ST, true> : launch_init<LAT, ST, false>;
+static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf) {
+  sf = (am == kAddrWrap) ? launch_step<LAT, ST, kAddrWrap>
+     : (am == kAddrWall32) ? launch_step<LAT, ST, kAddrWall32> : launch_step<LAT, ST, kAddrWall64>;
+  inf = launch_init<LAT, ST>;
   ff = launch_fix<LAT, ST>;
   gf = launch_ghost<LAT, ST>;
 }
@@ -347,6 +349,7 @@ struct Level {
   ilbm_case_desc c{};
   int lat = -1, st = -1;
   bool wrap = true;
+  int am = kAddrWall64;  // stream-collide addressing (kAddr*)
   GridGeom g{};
   int64_t ncells = 0;
   size_t popbytes = 8;
@@ -555,6 +558,13 @@ struct Level {
     popbytes = (st == 0) ? 8 : (st == 1) ? 4 : 2;
     const int Q = c.q;
     const int64_t npad = slabs.face_off[slabs.nface];
+    // walled levels under 4 GiB take 32-bit offsets (a VALU-bound FP16S
+    // sphere kernel: 3.78 -> 4.43 TB/s); ILBM_FORCE_ADDR64=1 keeps 64-bit
+    // offsets on every walled level (the path of levels of 4 GiB or more)
+    const char* f64 = std::getenv("ILBM_FORCE_ADDR64");
+    const bool force64 = f64 && f64[0] == '1';
+    am = wrap ? kAddrWrap
+         : (!force64 && (int64_t)popbytes * (g.pop_n + npad * Q) < ((int64_t)1 << 32)) ? kAddrWall32 : kAddrWall64;
     for (int b = 0; b < 2; ++b) {
       // Q population planes followed by the Q ghost slabs
       const size_t bytes = popbytes * ((size_t)g.pop_n + (size_t)npad * Q);
@@ -579,9 +589,9 @@ struct Level {
 
     auto pk = [&](auto latt) {
       using L = decltype(latt);
-      if (st == 0) pick<L, StF64>(wrap, stepfn, initfn, fixfn, ghostfn);
-      else if (st == 1) pick<L, StF32>(wrap, stepfn, initfn, fixfn, ghostfn);
-      else pick<L, StF16S>(wrap, stepfn, initfn, fixfn, ghostfn);
+      if (st == 0) pick<L, StF64>(am, stepfn, initfn, fixfn, ghostfn);
+      else if (st == 1) pick<L, StF32>(am, stepfn, initfn, fixfn, ghostfn);
+      else pick<L, StF16S>(am, stepfn, initfn, fixfn, ghostfn);
     };
     if (lat == 0) pk(D2Q9{});
     else if (lat == 1) pk(D3Q19{});
@@ -1547,7 +1557,8 @@ int ilbm_level_info(ilbm_level* h, double out[8]) {
   if (!h || !out) return ILBM_ERR_INVALID;
   const ilbm::Level& L = h->L;
   out[0] = L.tau; out[1] = L.dt; out[2] = L.t_new; out[3] = L.istep;
-  out[4] = (double)L.band_cells; out[5] = (double)L.popbytes; out[6] = L.wrap ? 0.0 : 1.0;
+  out[4] = (double)L.band_cells; out[5] = (double)L.popbytes;
+  out[6] = (L.am == ilbm::kAddrWrap) ? 0.0 : (L.am == ilbm::kAddrWall64) ? 1.0 : 2.0;
   out[7] = (double)L.nblocks;
   return ILBM_OK;
 }

@@ -231,3 +231,31 @@ def test_large_3d_slice_layout_bitexact():
     lev, orc = run_pair(p, cf.random_init(p, seed=3), 3)
     assert_bitexact(lev.f(), orc.f(), "f")
     assert_bitexact(lev.macro(), orc.macro(), "macro")
+
+
+@pytest.mark.parametrize("force64", [False, True])
+def test_walled_addressing_modes_bitexact(force64, monkeypatch):
+    """Walled levels under 4 GiB address the populations and ghost slabs with
+    32-bit offsets; ILBM_FORCE_ADDR64=1 takes the 64-bit path of larger
+    levels.  Both against the oracle: the mixed-wall D3Q19 box (every face
+    kind, edges and corners), FP64, and a D2Q9 channel."""
+    if force64:
+        monkeypatch.setenv("ILBM_FORCE_ADDR64", "1")
+    else:
+        monkeypatch.delenv("ILBM_FORCE_ADDR64", raising=False)
+    want = "wall64" if force64 else "wall32"
+    w1 = (1.0, 0.01, 0.0, 0.0)
+    w2 = (1.0, 0.0, 0.02, 0.0)
+    c = cf.Case(lattice=cf.D3Q19, n=(11, 9, 7), nu=0.04, periodic=(0, 0, 0),
+                bc_lo=(cf.EXT_DIR, cf.USER_1, cf.USER_2), bc_hi=(cf.FOEXTRAP, cf.USER_1, cf.USER_2),
+                bc_lo_val=((1.0, 0.03, 0.0, 0.0), w1, w2), bc_hi_val=((0.0,) * 4, w1, w2),
+                name="mixed3d_addr")
+    lev, orc = run_pair(c, cf.random_init(c, seed=16, du=0.02), 9)
+    assert lev.info()["addressing"] == want
+    assert_bitexact(lev.f(), orc.f(), "f")
+    assert_bitexact(lev.macro(), orc.macro(), "macro")
+    ch = cf.cylinder_2d(nx=40, ny=24).replace(use_ib=False, name="channel_addr")
+    lev, orc = run_pair(ch, cf.random_init(ch, seed=17), 12)
+    assert lev.info()["addressing"] == want
+    assert_bitexact(lev.f(), orc.f(), "f")
+    assert_bitexact(lev.macro(), orc.macro(), "macro")
