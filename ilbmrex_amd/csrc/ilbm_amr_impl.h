@@ -647,7 +647,7 @@ struct Amr {
     if (!V.cf && !V.wrap) {
       GhostArgs ga = V.ghost_args();
       const int64_t npad = V.slabs.face_off[V.slabs.nface];
-      if (npad) V.ghostfn(dim3((unsigned)((npad + 255) / 256)), dim3(256), V.stream, ga);
+      if (npad) V.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), V.stream, ga);
     }
     StepArgs sa = V.step_args(export_state);
     V.stepfn(V.grid, V.block, stream, sa);

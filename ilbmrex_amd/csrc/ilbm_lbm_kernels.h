@@ -35,6 +35,9 @@
 namespace ilbm {
 
 constexpr int kBlock = 256;
+// ghost-slab fill of a single level: one thread per ghost cell, small blocks
+// so that the few thousand ghost cells of a face spread over every CU
+constexpr int kGhostBlock = 64;
 
 // Population addressing of the stream-collide kernel:
 //   kAddrWall64 a level with non-periodic faces, 64-bit element offsets
@@ -93,6 +96,20 @@ __device__ __forceinline__ void face_store(typename ST::raw* fb, int64_t fi, con
   const uint4* src = reinterpret_cast<const uint4*>(rec);
 #pragma unroll
   for (int c = 0; c < QP * (int)sizeof(R) / 16; ++c) dst[c] = src[c];
+}
+
+template <class LAT, class ST>
+__device__ __forceinline__ void face_load(const typename ST::raw* fb, int64_t fi, typename ST::compute_t* f) {
+  using R = typename ST::raw;
+  using T = typename ST::compute_t;
+  constexpr int QP = face_stride(LAT::Q, (int)sizeof(R));
+  alignas(16) R rec[QP];
+  const uint4* src = reinterpret_cast<const uint4*>(fb + fi * QP);
+  uint4* dst = reinterpret_cast<uint4*>(rec);
+#pragma unroll
+  for (int c = 0; c < QP * (int)sizeof(R) / 16; ++c) dst[c] = src[c];
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) f[q] = ST::load(rec, q, T(LAT::w[q]));
 }
 
 __host__ __device__ inline int64_t face_index(const GridGeom& g, int d, int i, int j, int k) {
@@ -623,16 +640,18 @@ struct GhostArgs {
   Slabs slabs;
 };
 
+// a slab holds under 2^31 cells (checked at set-up): 32-bit divisions
 __device__ inline void slab_cell(const GhostArgs& a, int64_t t, int c[3]) {
   const Slabs& S = a.slabs;
   int f = 0;
   while (f + 1 < S.nface && t >= S.face_off[f + 1]) ++f;
-  int64_t r = t - S.face_off[f];
+  uint32_t r = (uint32_t)(t - S.face_off[f]);
   for (int e = 0; e < 3; ++e) {
     if (e == S.face_d[f]) { c[e] = S.face_side[f] ? a.g.n[e] : -1; continue; }
-    const int n = S.face_n[f][e];
-    c[e] = S.face_lo[f][e] + (int)(r % n);
-    r /= n;
+    const uint32_t n = (uint32_t)S.face_n[f][e];
+    const uint32_t qn = r / n;
+    c[e] = S.face_lo[f][e] + (int)(r - qn * n);
+    r = qn;
   }
 }
 
@@ -647,11 +666,7 @@ __device__ inline void load_face(const GhostArgs& a, const int c[3], typename ST
     if (c[d] == 0) side = 0;
     else if (c[d] == g.n[d] - 1) side = 1;
     if (side < 0) continue;
-    const R* fb = static_cast<const R*>(a.faces.p[2 * d + side]);
-    constexpr int QP = face_stride(LAT::Q, (int)sizeof(R));
-    const int64_t fi = face_index(g, d, c[0], c[1], c[2]);
-#pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) out[q] = ST::load(fb, fi * QP + q, T(LAT::w[q]));
+    face_load<LAT, ST>(static_cast<const R*>(a.faces.p[2 * d + side]), face_index(g, d, c[0], c[1], c[2]), out);
     return;
   }
 #pragma unroll
@@ -817,6 +832,18 @@ __global__ void __launch_bounds__(1024) k_mass_total(const double* p0, int64_t n
 __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t n, double* out) {
   const double r = sum_partials_body(p, n);
   if (threadIdx.x == 0) *out = r;
+}
+
+// Sum(Rho) of consecutive steps whose block partials fill consecutive ring
+// slots: block b reduces slot b as k_mass_total does (the second array may
+// be empty) and writes out[b]
+__global__ void __launch_bounds__(1024) k_mass_ring(const double* p0, int64_t stride0, int64_t n0,
+                                                    const double* p1, int64_t stride1, int64_t n1, double* out) {
+  const int64_t b = blockIdx.x;
+  const double a = sum_partials_body(p0 + b * stride0, n0);
+  __syncthreads();
+  const double c = (n1 > 0) ? sum_partials_body(p1 + b * stride1, n1) : 0.0;
+  if (threadIdx.x == 0) out[b] = a + c;
 }
 
 // End of a batched step, one block per case: Sum(Rho) into the case's mass

@@ -432,9 +432,26 @@ struct Level {
   int near_lo[3] = {0, 0, 0}, near_n[3] = {0, 0, 0};
   dim3 grid_near;
   int64_t nblocks_near = 0;
-  double* partial_near = nullptr;
+  // Sum(Rho) block partials of consecutive steps fill the slots of a ring
+  // (near-box partials in a second ring); one k_mass_ring launch reduces the
+  // slots filled so far when the ring is full or the advance ends
+  double* mring = nullptr;
+  double* nring = nullptr;
+  int mring_slots = 0;
+  int mring_used = 0;   // next slot
+  int mring_base = 0;   // first slot not yet reduced
   hipStream_t stream_b = nullptr;
-  hipEvent_t ev_a = nullptr, ev_ghost = nullptr, ev_near = nullptr, ev_b = nullptr;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;  // start / end of an overlapped advance
+  // Per-step dependencies between the streams: one event per kind of
+  // signal, recorded by the producer (xsignal) and waited for by the
+  // consumer (xwait), which always waits for the latest record.  (Stream
+  // memory operations, hipStreamWrite/WaitValue64, cost less host time but
+  // run as runtime kernels of 3-5 us each on the GPU, which lengthens the
+  // far stream's chain; the host has slack, the GPU has not.)
+  enum { X_GHOST = 0, X_NEAR = 1, X_EXP = 2, X_REC0 = 3, X_N = X_REC0 + 4 };
+  hipEvent_t xev[X_N] = {};
+  uint64_t xseq[X_N] = {};
+  uint64_t rec_seq[4] = {};
   // During an overlapped advance the force record of a fixed body runs on a
   // third stream: step s's marker forces and export block sums go to slot
   // s % kRecRing, which the forcing reuses only after that record is done.
@@ -444,22 +461,26 @@ struct Level {
   double* fp_ring[kRecRing] = {};
   int64_t fpartial_n = 0;
   hipStream_t stream_c = nullptr;
-  hipEvent_t ev_exp = nullptr, ev_c = nullptr, ev_rec[kRecRing] = {};
+  hipEvent_t ev_c = nullptr;
+  // the near box keeps off the non-periodic faces: the ghost slabs of the
+  // next step depend on the far cells only and are filled right after them
+  bool near_off_walls = false;
+  bool ghost_ahead = false;               // this step's slabs are already queued
+  uint64_t ahead_signal = 0;              // and signalled with this value
 
   ~Level() {
     for (auto e : ev_start) (void)hipEventDestroy(e);
     for (auto e : ev_stop) (void)hipEventDestroy(e);
     if (stream_b) (void)hipStreamSynchronize(stream_b);
     if (stream_c) (void)hipStreamSynchronize(stream_c);
-    for (hipEvent_t e : {ev_a, ev_ghost, ev_near, ev_b, ev_exp, ev_c})
+    for (hipEvent_t e : {ev_a, ev_b, ev_c})
       if (e) (void)hipEventDestroy(e);
-    for (int k = 0; k < kRecRing; ++k) {
-      if (ev_rec[k]) (void)hipEventDestroy(ev_rec[k]);
-      if (k > 0) { dev_free(F_ring[k]); dev_free(fp_ring[k]); }
-    }
+    for (int k = 1; k < kRecRing; ++k) { dev_free(F_ring[k]); dev_free(fp_ring[k]); }
+    for (hipEvent_t e : xev)
+      if (e) (void)hipEventDestroy(e);
     if (stream_b) (void)hipStreamDestroy(stream_b);
     if (stream_c) (void)hipStreamDestroy(stream_c);
-    dev_free(partial_near);
+    dev_free(mring); dev_free(nring);
     dev_free(gbuf[0]); dev_free(gbuf[1]);
     for (int f = 0; f < 6; ++f) dev_free(faces.p[f]);
     dev_free(partial); dev_free(mout); dev_free(fout); dev_free(mass_dev); dev_free(force_dev);
@@ -547,6 +568,7 @@ struct Level {
           else { slabs.face_lo[nf][e] = -1; slabs.face_n[nf][e] = c.n[e] + 2; }
           cnt *= slabs.face_n[nf][e];
         }
+        if (cnt >= ((int64_t)1 << 31)) throw StatusError{ILBM_ERR_INVALID, "face slab of 2^31 cells or more"};
         slabs.face_off[nf] = off;
         off += cnt;
         ++nf;
@@ -907,7 +929,7 @@ struct Level {
   // with the ComputeIBForce record of the step (row `row` of the history).
   void ib_update_forcing(double time, int row) {
     // the slot's previous record must be done before its inputs are rewritten
-    if (ring_slot >= 0) HIPCHK(hipStreamWaitEvent(stream, ev_rec[ring_slot], 0));
+    if (ring_slot >= 0 && rec_seq[ring_slot]) xwait(stream, X_REC0 + ring_slot, rec_seq[ring_slot]);
     if (dev_kin) {
       // the pitching kinematics on the device; the angle terms as
       // marker_positions computes them
@@ -964,10 +986,9 @@ struct Level {
       // record is followed by one that has it)
       if (rec_enable && ring_slot >= 0) {
         // off the step's critical path: the record reads only its slot
-        HIPCHK(hipEventRecord(ev_exp, stream));
-        HIPCHK(hipStreamWaitEvent(stream_c, ev_exp, 0));
+        xwait(stream_c, X_EXP, xsignal(stream, X_EXP));
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream_c, a, r, (int)gfix);
-        HIPCHK(hipEventRecord(ev_rec[ring_slot], stream_c));
+        rec_seq[ring_slot] = xsignal(stream_c, X_REC0 + ring_slot);
       } else if (rec_enable) {
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
       }
@@ -1177,19 +1198,20 @@ struct Level {
         near_n[d] = hi - lo + 1;
       }
       grid_near = dim3((near_n[0] + block.x - 1) / block.x, (near_n[1] + block.y - 1) / block.y, near_n[2]);
+      near_off_walls = true;
+      for (int d = 0; d < c.dim; ++d)
+        if (nonper[d] && (near_lo[d] <= 0 || near_lo[d] + near_n[d] >= c.n[d])) near_off_walls = false;
       nblocks_near = (int64_t)grid_near.x * grid_near.y * grid_near.z;
-      partial_near = dev_alloc<double>(nblocks_near);
       HIPCHK(hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking));
-      for (hipEvent_t* e : {&ev_a, &ev_ghost, &ev_near, &ev_b})
+      for (hipEvent_t* e : {&ev_a, &ev_b})
         HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      for (hipEvent_t& e : xev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       if (async_record()) {
         HIPCHK(hipStreamCreateWithFlags(&stream_c, hipStreamNonBlocking));
-        for (hipEvent_t* e : {&ev_exp, &ev_c})
-          HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+        HIPCHK(hipEventCreateWithFlags(&ev_c, hipEventDisableTiming));
         F_ring[0] = F_d;
         fp_ring[0] = fpartial;
         for (int k = 0; k < kRecRing; ++k) {
-          HIPCHK(hipEventCreateWithFlags(&ev_rec[k], hipEventDisableTiming));
           if (k > 0) {
             F_ring[k] = dev_alloc<double>(3 * nm);
             fp_ring[k] = dev_alloc<double>(fpartial_n);
@@ -1206,36 +1228,96 @@ struct Level {
     return nloop == 1 && !dev_kin && !moving;
   }
 
-  // One step of an IB level on two streams.  Stream B: ghost slabs, the far
-  // cells, then Sum(Rho) of both parts once the near box is done.  Stream A:
-  // IB forcing (reads the band macros of the last near box, adds the Guo
-  // term to band cells, which no far cell reads), then the near box after
-  // the ghost slabs.  The far part of the next step starts after this near
-  // box (its sources) through the wait before Sum(Rho).
-  void split_step(int s, double time, bool want) {
-    const int tb = 256;
+  // One step of an IB level on two streams (a third takes the force record
+  // of a fixed body).  Stream B: ghost slabs, the far cells, then a wait for
+  // the near box (the sources of the next far part) and, when the ring of
+  // mass partials is full, Sum(Rho) of its steps.  Stream A: IB forcing
+  // (reads the band macros of the last near box, adds the Guo term to band
+  // cells, which no far cell reads), then the near box after the ghost
+  // slabs.  When the near box keeps off the non-periodic faces, stream B
+  // fills the next step's ghost slabs right after the far cells (they read
+  // only far cells and their boundary layer), before waiting for the near
+  // box.
+  void split_step(int s, double time, bool want, bool more) {
     StepArgs sf = step_args(want);
     sf.sub_mode = 1;
     for (int d = 0; d < 3; ++d) { sf.sub_lo[d] = near_lo[d]; sf.sub_n[d] = near_n[d]; }
+    sf.partial = mring + (int64_t)mring_used * nblocks;
     StepArgs sn = sf;
     sn.sub_mode = 2;
-    sn.partial = partial_near;
-    if (!wrap) {
-      GhostArgs ga = ghost_args();
-      const int64_t npad = slabs.face_off[slabs.nface];
-      if (npad) ghostfn(dim3((unsigned)((npad + tb - 1) / tb)), dim3(tb), stream_b, ga);
+    sn.partial = nring + (int64_t)mring_used * nblocks_near;
+    const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
+    const dim3 ggrid((unsigned)((npad + kGhostBlock - 1) / kGhostBlock));
+    // The near box waits for this signal: the ghost slabs it may read and
+    // the far cells of the last step, which read what it overwrites.
+    uint64_t ghost_done = ahead_signal;
+    if (!ghost_ahead) {
+      if (npad) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+      ghost_done = xsignal(stream_b, X_GHOST);
     }
-    HIPCHK(hipEventRecord(ev_ghost, stream_b));
+    ghost_ahead = false;
     stepfn(grid, block, stream_b, sf);
-    ring_slot = stream_c ? s % kRecRing : -1;
+    ring_slot = async_record() ? s % kRecRing : -1;
     ib_update_forcing(time, s);
     ring_slot = -1;
-    HIPCHK(hipStreamWaitEvent(stream, ev_ghost, 0));
+    xwait(stream, X_GHOST, ghost_done);
     stepfn(grid_near, block, stream, sn);
-    HIPCHK(hipEventRecord(ev_near, stream));
-    HIPCHK(hipStreamWaitEvent(stream_b, ev_near, 0));
-    hipLaunchKernelGGL(k_mass_total, dim3(1), dim3(1024), 0, stream_b, (const double*)partial, nblocks,
-                       (const double*)partial_near, nblocks_near, mass_dev + s);
+    const uint64_t near_done = xsignal(stream, X_NEAR);
+    if (more && (near_off_walls || !npad)) {
+      // the next step's slabs (from the boundary layer of the far cells
+      // just written) and its signal to the near box, ahead of the wait
+      if (npad) {
+        cur ^= 1;
+        const GhostArgs ga = ghost_args();
+        cur ^= 1;
+        ghostfn(ggrid, dim3(kGhostBlock), stream_b, ga);
+      }
+      ahead_signal = xsignal(stream_b, X_GHOST);
+      ghost_ahead = true;
+    }
+    xwait(stream_b, X_NEAR, near_done);
+    ring_step(stream_b, s, more, true);
+  }
+
+  // After step s wrote ring slot mring_used: when half of the ring is
+  // filled (or the advance ends) its steps' Sum(Rho) are reduced on `st`.
+  // The halves alternate, so a near box queued ahead of the reduction
+  // writes into the other half.
+  void ring_step(hipStream_t st, int s, bool more, bool with_near) {
+    ++mring_used;
+    const int n = mring_used - mring_base;
+    if (n == mring_slots / 2 || !more) {
+      hipLaunchKernelGGL(k_mass_ring, dim3(n), dim3(1024), 0, st, (const double*)(mring + (int64_t)mring_base * nblocks),
+                         nblocks, nblocks, (const double*)(with_near ? nring + (int64_t)mring_base * nblocks_near : nullptr),
+                         with_near ? nblocks_near : (int64_t)0, with_near ? nblocks_near : (int64_t)0,
+                         mass_dev + s + 1 - n);
+      mring_base = mring_used;
+      if (mring_used == mring_slots || !more) mring_used = mring_base = 0;
+    }
+  }
+
+  // the record numbers only document which record a wait expects: every
+  // wait below expects the latest one of its kind
+  uint64_t xsignal(hipStream_t st, int k) {
+    HIPCHK(hipEventRecord(xev[k], st));
+    return ++xseq[k];
+  }
+  void xwait(hipStream_t st, int k, uint64_t v) {
+    if (v != xseq[k]) throw StatusError{ILBM_ERR_STATE, "stream signal out of order"};
+    HIPCHK(hipStreamWaitEvent(st, xev[k], 0));
+  }
+
+  // ring slots for the mass partials of this level (and its near box)
+  void ensure_mring(bool split) {
+    const int64_t per = nblocks + (split ? nblocks_near : 0);
+    if (mring && (!split || nring)) return;
+    dev_free(mring); dev_free(nring);
+    mring = nring = nullptr;
+    // two halves of at most 32 steps each, at most 64 MiB
+    mring_slots = 2 * (int)std::max<int64_t>(1, std::min<int64_t>(32, ((int64_t)32 << 20) / (8 * per)));
+    mring = dev_alloc<double>((size_t)nblocks * mring_slots);
+    if (split) nring = dev_alloc<double>((size_t)nblocks_near * mring_slots);
+    mring_used = mring_base = 0;
   }
 
   void advance(int nsteps, bool want_state) {
@@ -1244,6 +1326,7 @@ struct Level {
     if (want_state) ensure_exports();
     grow_histories(nsteps, nsteps);
     const bool split = split_ready();
+    ensure_mring(split);
     if (split) {
       HIPCHK(hipEventRecord(ev_a, stream));
       HIPCHK(hipStreamWaitEvent(stream_b, ev_a, 0));
@@ -1258,13 +1341,12 @@ struct Level {
       }
     }
     state_valid = false;
-    const int tb = 256;
     for (int s = 0; s < nsteps; ++s) {
       const double time = t_new;  // timeStepWithSubcycling, AmrCoreLBM.cpp:1730-1733
       t_old = t_new;
       t_new += dt;
       if (split) {
-        split_step(s, time, want_state && s == nsteps - 1);
+        split_step(s, time, want_state && s == nsteps - 1, s + 1 < nsteps);
         cur ^= 1;
         ++istep;
         HIPCHK(hipGetLastError());
@@ -1274,15 +1356,15 @@ struct Level {
       if (!wrap) {
         GhostArgs ga = ghost_args();
         const int64_t npad = slabs.face_off[slabs.nface];
-        if (npad) ghostfn(dim3((unsigned)((npad + tb - 1) / tb)), dim3(tb), stream, ga);
+        if (npad) ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, ga);
       }
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
+      sa.partial = mring + (int64_t)mring_used * nblocks;
       if (timing) HIPCHK(hipEventRecord(ev_start[s], stream));
       stepfn(grid, block, stream, sa);
       if (timing) HIPCHK(hipEventRecord(ev_stop[s], stream));
-      hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, stream, (const double*)partial, nblocks,
-                         mass_dev + s);
+      ring_step(stream, s, !last, false);
       cur ^= 1;
       ++istep;
       HIPCHK(hipGetLastError());
