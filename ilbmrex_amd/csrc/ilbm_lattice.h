@@ -74,6 +74,7 @@ struct StF64 {
   static constexpr int ID = 0;
   static constexpr bool exact = true;
   __device__ static inline double load(const raw* p, int64_t i, double) { return p[i]; }
+  __device__ static inline double load_nt(const raw* p, int64_t i, double) { return __builtin_nontemporal_load(p + i); }
   __device__ static inline void store(raw* p, int64_t i, double v, double) { p[i] = v; }
   __device__ static inline void store_nt(raw* p, int64_t i, double v, double) {
     __builtin_nontemporal_store(v, p + i);
@@ -86,6 +87,7 @@ struct StF32 {
   static constexpr int ID = 1;
   static constexpr bool exact = false;
   __device__ static inline float load(const raw* p, int64_t i, float) { return p[i]; }
+  __device__ static inline float load_nt(const raw* p, int64_t i, float) { return __builtin_nontemporal_load(p + i); }
   __device__ static inline void store(raw* p, int64_t i, float v, float) { p[i] = v; }
   __device__ static inline void store_nt(raw* p, int64_t i, float v, float) {
     __builtin_nontemporal_store(v, p + i);
@@ -99,6 +101,10 @@ struct StF16S {
   static constexpr bool exact = false;
   __device__ static inline float load(const raw* p, int64_t i, float wq) {
     return __half2float(p[i]) + wq;
+  }
+  __device__ static inline float load_nt(const raw* p, int64_t i, float wq) {
+    const unsigned short bits = __builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + i);
+    return __half2float(__ushort_as_half(bits)) + wq;
   }
   __device__ static inline void store(raw* p, int64_t i, float v, float wq) {
     p[i] = __float2half(v - wq);

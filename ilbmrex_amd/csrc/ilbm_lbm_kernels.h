@@ -356,7 +356,10 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         const uint32_t src = (uint32_t)(ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
         const R* plane = gin + (int64_t)q * g.qs;
         const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
-        f[q] = ST::load(p, 0, T(LAT::w[q]));
+        // 3D: non-temporal loads (256^3 FP64 5.66 -> 5.94 TB/s, FP32 128^3
+        // +1 %); on walled and 2D levels they cost 2-9 %
+        if constexpr (LAT::D == 3) f[q] = ST::load_nt(p, 0, T(LAT::w[q]));
+        else f[q] = ST::load(p, 0, T(LAT::w[q]));
         continue;
       }
       // the source cell inside the level, or for a source beyond a
