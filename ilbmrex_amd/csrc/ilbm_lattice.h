@@ -67,12 +67,16 @@ __host__ __device__ constexpr int opposite(int q) {
 //   StF64  : FP64 storage, FP64 arithmetic (the parity path)
 //   StF32  : FP32 storage, FP32 arithmetic
 //   StF16S : FP16 storage of the shifted population f_q - w_q, FP32 arithmetic
+// load / store take and give f_q.  The stream-collide kernel works in the
+// stored variable when `shifted` is set (h_q = f_q - w_q for StF16S):
+// load_s / load_s_nt / store_s_nt move it without the +-w_q.
 // ---------------------------------------------------------------------------
 struct StF64 {
   using raw = double;
   using compute_t = double;
   static constexpr int ID = 0;
   static constexpr bool exact = true;
+  static constexpr bool shifted = false;
   __device__ static inline double load(const raw* p, int64_t i, double) { return p[i]; }
   __device__ static inline double load_nt(const raw* p, int64_t i, double) { return __builtin_nontemporal_load(p + i); }
   __device__ static inline void store(raw* p, int64_t i, double v, double) { p[i] = v; }
@@ -86,6 +90,7 @@ struct StF32 {
   using compute_t = float;
   static constexpr int ID = 1;
   static constexpr bool exact = false;
+  static constexpr bool shifted = false;
   __device__ static inline float load(const raw* p, int64_t i, float) { return p[i]; }
   __device__ static inline float load_nt(const raw* p, int64_t i, float) { return __builtin_nontemporal_load(p + i); }
   __device__ static inline void store(raw* p, int64_t i, float v, float) { p[i] = v; }
@@ -99,6 +104,15 @@ struct StF16S {
   using compute_t = float;
   static constexpr int ID = 2;
   static constexpr bool exact = false;
+  static constexpr bool shifted = true;
+  __device__ static inline float load_s(const raw* p, int64_t i) { return __half2float(p[i]); }
+  __device__ static inline float load_s_nt(const raw* p, int64_t i) {
+    return __half2float(__ushort_as_half(__builtin_nontemporal_load(reinterpret_cast<const unsigned short*>(p) + i)));
+  }
+  __device__ static inline void store_s(raw* p, int64_t i, float h) { p[i] = __float2half(h); }
+  __device__ static inline void store_s_nt(raw* p, int64_t i, float h) {
+    __builtin_nontemporal_store(__half_as_ushort(__float2half(h)), reinterpret_cast<unsigned short*>(p) + i);
+  }
   __device__ static inline float load(const raw* p, int64_t i, float wq) {
     return __half2float(p[i]) + wq;
   }
@@ -118,5 +132,23 @@ struct StF16S {
 
 template <class ST>
 constexpr size_t raw_size() { return sizeof(typename ST::raw); }
+
+// The stream-collide kernel's working variable: f_q, or h_q = f_q - w_q for
+// shifted storage.
+template <class ST>
+__device__ inline typename ST::compute_t pop_in(const typename ST::raw* p, int64_t i, typename ST::compute_t wq) {
+  if constexpr (ST::shifted) return ST::load_s(p, i);
+  else return ST::load(p, i, wq);
+}
+template <class ST>
+__device__ inline typename ST::compute_t pop_in_nt(const typename ST::raw* p, int64_t i, typename ST::compute_t wq) {
+  if constexpr (ST::shifted) return ST::load_s_nt(p, i);
+  else return ST::load_nt(p, i, wq);
+}
+template <class ST>
+__device__ inline void pop_out_nt(typename ST::raw* p, int64_t i, typename ST::compute_t v, typename ST::compute_t wq) {
+  if constexpr (ST::shifted) ST::store_s_nt(p, i, v);
+  else ST::store_nt(p, i, v, wq);
+}
 
 }  // namespace ilbm

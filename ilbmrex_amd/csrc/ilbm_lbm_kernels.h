@@ -81,7 +81,9 @@ struct FaceBuf {
 // stores instead of Q scattered ones.
 __host__ __device__ constexpr int face_stride(int q, int pop_bytes) { return ((q * pop_bytes + 15) / 16 * 16) / pop_bytes; }
 
-template <class LAT, class ST>
+// f holds the populations, or for SH_IN the kernel's shifted working
+// variable of shifted storage (the record's own format)
+template <class LAT, class ST, bool SH_IN = false>
 __device__ __forceinline__ void face_store(typename ST::raw* fb, int64_t fi, const typename ST::compute_t* f) {
   using R = typename ST::raw;
   using T = typename ST::compute_t;
@@ -89,8 +91,12 @@ __device__ __forceinline__ void face_store(typename ST::raw* fb, int64_t fi, con
   alignas(16) R rec[QP];
 #pragma unroll
   for (int q = 0; q < QP; ++q) {
-    if (q < LAT::Q) ST::store(rec, q, f[q], T(LAT::w[q]));
-    else rec[q] = R(0);
+    if (q < LAT::Q) {
+      if constexpr (SH_IN && ST::shifted) ST::store_s(rec, q, f[q]);
+      else ST::store(rec, q, f[q], T(LAT::w[q]));
+    } else {
+      rec[q] = R(0);
+    }
   }
   uint4* dst = reinterpret_cast<uint4*>(fb + fi * QP);
   const uint4* src = reinterpret_cast<const uint4*>(rec);
@@ -358,8 +364,8 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
         // 3D: non-temporal loads (256^3 FP64 5.66 -> 5.94 TB/s, FP32 128^3
         // +1 %); on walled and 2D levels they cost 2-9 %
-        if constexpr (LAT::D == 3) f[q] = ST::load_nt(p, 0, T(LAT::w[q]));
-        else f[q] = ST::load(p, 0, T(LAT::w[q]));
+        if constexpr (LAT::D == 3) f[q] = pop_in_nt<ST>(p, 0, T(LAT::w[q]));
+        else f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
         continue;
       }
       // the source cell inside the level, or for a source beyond a
@@ -374,11 +380,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       off = (o0 | o1 | o2) ? gofs + (I)q * npad + t : off;
       if constexpr (O32) {
         const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * (uint32_t)sizeof(R));
-        f[q] = ST::load(p, 0, T(LAT::w[q]));
+        f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
       } else {
-        f[q] = ST::load(gin, off, T(LAT::w[q]));
+        f[q] = pop_in<ST>(gin, off, T(LAT::w[q]));
       }
     }
+    // f holds f_q, or h_q = f_q - w_q for shifted storage (then the first
+    // moments are the same sums, since sum_q w_q c_q = 0, and rho = 1 +
+    // sum_q h_q)
+    constexpr bool SH = ST::shifted;
     // moments in the reference's order (calculateMacroForcing.H:21-27); a
     // term whose direction component is 0 is left out and -1 * f is
     // subtracted, which leaves every finite sum bit-identical
@@ -393,6 +403,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       if constexpr (LAT::cz[q] > 0) m3 += f[q];
       else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
     });
+    if constexpr (SH) r += T(1);
     const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
     const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
                          (unsigned)bk < (unsigned)a.band_n[2];
@@ -422,7 +433,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     }
     if (a.fout) {
 #pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q];
+      for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q] + (SH ? LAT::w[q] : 0.0);
     }
     // pre-collision boundary layer for the next ghost fill
     const int cc[3] = {i, j, k};
@@ -435,9 +446,9 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       if (side < 0) continue;
       R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
       const int64_t fi = face_index(g, d, i, j, k);
-      face_store<LAT, ST>(fb, fi, f);
+      face_store<LAT, ST, true>(fb, fi, f);
       // a cell sitting on the lo and hi face of a 1-cell dimension writes both
-      if (g.n[d] == 1) face_store<LAT, ST>(static_cast<R*>(a.faces.p[2 * d + 1]), fi, f);
+      if (g.n[d] == 1) face_store<LAT, ST, true>(static_cast<R*>(a.faces.p[2 * d + 1]), fi, f);
     }
     // BGK collision with the macros just computed (feq uses macro_old of the
     // next step, AdvanceLBMAtLevel.cpp:225-238)
@@ -450,9 +461,9 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       if (O32 || a.off32) {
         R* plane = gout + (int64_t)q * g.qs;
         R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
-        ST::store_nt(p, 0, gq, T(LAT::w[q]));
+        pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
       } else {
-        ST::store_nt(gout, (int64_t)q * g.qs + (int64_t)self, gq, T(LAT::w[q]));
+        pop_out_nt<ST>(gout, (int64_t)q * g.qs + (int64_t)self, gq, T(LAT::w[q]));
       }
     };
     if constexpr (ST::exact) {
@@ -466,14 +477,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     } else {
       // reduced-precision collision, a direction and its opposite together:
       // feq = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u), then
-      // g = f + (feq - f) / tau with fused multiply-adds
+      // g = f + (feq - f) / tau with fused multiply-adds.  Shifted storage
+      // relaxes h = f - w towards feq - w (the same update, shifted).
       const T base = T(1.0) - T(1.5) * uv;
       const T om = T(a.inv_tau);
       static_for<LAT::Q>([&](auto qc) {
         constexpr int q = decltype(qc)::value;
         constexpr int p = opposite<LAT>(q);
         if constexpr (p == q) {
-          const T fe = T(LAT::w[q]) * r * base;
+          const T fe = SH ? __builtin_fmaf(T(LAT::w[q]) * r, base, -T(LAT::w[q])) : T(LAT::w[q]) * r * base;
           T gq = __builtin_fmaf(fe - f[q], om, f[q]);
           if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
           put(q, gq);
@@ -488,7 +500,8 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
             else if constexpr (LAT::cz[q] < 0) eu -= w;
           }
           const T wr = T(LAT::w[q]) * r;
-          const T A = wr * __builtin_fmaf(T(4.5) * eu, eu, base);
+          const T A = SH ? __builtin_fmaf(wr, __builtin_fmaf(T(4.5) * eu, eu, base), -T(LAT::w[q]))
+                         : wr * __builtin_fmaf(T(4.5) * eu, eu, base);
           const T B = (T(3.0) * wr) * eu;
           T gq = __builtin_fmaf((A + B) - f[q], om, f[q]);
           T gp = __builtin_fmaf((A - B) - f[p], om, f[p]);
