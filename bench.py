@@ -13,10 +13,15 @@ cells * steps / wall seconds of the timed region, state resident in HBM.
 algorithmic bytes per launch = cells * 2 * Q * sizeof(population), divided
 by its average duration measured with HIP events on the level's stream.
 
-Also reported: the same case with FP64 and FP16-shifted storage, the
-periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6), a plain copy
-kernel's bandwidth, the Cd error of reduced storage against FP64, and the
-CPU oracle (FP64, OpenMP) timed on a bounded sample.
+Also reported:
+- the same case with FP64 and FP16-shifted storage;
+- ``accuracy``: 10000 steps of the sphere in each storage from the same state,
+  with the FP32 and FP16-shifted velocity field (relative L2), mean Cd and Cl
+  against the FP64 run;
+- the periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6; FP32, FP64,
+  FP16-shifted; 200 warm-up and 1000 timed steps);
+- a plain copy kernel's bandwidth;
+- the CPU oracle (FP64, OpenMP), timed on a bounded sample.
 
 ``batched``: the 256-case D2Q9 cylinder ensemble (FP32, 5000 steps, one
 launch per kernel for all cases, hipGraph replay) against the plain per-case
@@ -89,6 +94,36 @@ def run_level(case, init, warmup, steps, want_forces=False):
     return {"mlups": mlups, "ms_per_step": 1e3 * wall / steps, "kernel_ms": kms,
             "kernel_launches": nl, "bytes_per_lup": bpl, "kernel_gbs": kgbs,
             "kernel_frac": kgbs / HBM_PEAK_GBS, "bytes_per_launch": cells * bpl}, fh
+
+
+def accuracy(nsteps):
+    """The sphere in FP64, FP32 and FP16-shifted storage for nsteps steps
+    from the same state: relative L2 error of the velocity field at the end,
+    relative error of mean Cd and mean |Cl - Cl_64| over the last 1000
+    steps, against the FP64 run."""
+    from ilbmrex_amd.native import Level
+    out = {"steps": nsteps}
+    res = {}
+    for key, st in (("f64", cf.STORE_F64), ("f32", cf.STORE_F32), ("f16s", cf.STORE_F16S)):
+        case = sphere_case(st)
+        log(f"[bench] accuracy: sphere {key}, {nsteps} steps")
+        lev = Level(case, cf.uniform_init(case, u=(0.04, 0.0, 0.0)))
+        lev.set_overlap(OVERLAP)
+        lev.advance(nsteps, want_state=True)
+        res[key] = (lev.macro()[1:4].copy(), lev.force_history())
+        lev.close()
+    u64, fh64 = res["f64"]
+    tail = slice(max(0, nsteps - 1000), nsteps)
+    cd64 = fh64[tail, 4].mean()
+    out["cd_f64_mean"] = float(cd64)
+    out["cl_f64_rms"] = float(np.sqrt(np.mean(fh64[tail, 5] ** 2)))
+    for key in ("f32", "f16s"):
+        u, fh = res[key]
+        out[f"vel_rel_l2_{key}_vs_f64"] = float(np.linalg.norm(u - u64) / np.linalg.norm(u64))
+        out[f"cd_rel_err_{key}_vs_f64"] = float(abs(fh[tail, 4].mean() - cd64) / abs(cd64))
+        out[f"cl_mean_abs_diff_{key}_vs_f64"] = float(np.mean(np.abs(fh[tail, 5] - fh64[tail, 5])))
+        out[f"finite_{key}"] = bool(np.all(np.isfinite(u)) and np.all(np.isfinite(fh)))
+    return out
 
 
 def cpu_baseline(steps):
@@ -208,7 +243,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=200)
-    ap.add_argument("--variant-steps", type=int, default=400)
+    ap.add_argument("--variant-steps", type=int, default=1000, help="timed steps of the periodic 256^3 cases")
+    ap.add_argument("--roofline-warmup", type=int, default=200)
+    ap.add_argument("--accuracy-steps", type=int, default=10000,
+                    help="steps of the sphere in FP64 / FP32 / FP16-shifted storage for the error report")
     ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
@@ -248,43 +286,27 @@ def main():
     c32 = sphere_case(cf.STORE_F32)
     init = cf.uniform_init(c32, u=(0.04, 0.0, 0.0))
     log(f"[bench] sphere 256x128x128 FP32: warmup {args.warmup}, timed {args.steps}")
-    head, fh32 = run_level(c32, init, args.warmup, args.steps, want_forces=True)
+    head, _ = run_level(c32, init, args.warmup, args.steps)
     log(f"[bench] sphere FP32: {head['mlups']:.1f} MLUPS, k_step {head['kernel_ms']*1e3:.1f} us "
         f"({head['kernel_gbs']:.0f} GB/s)")
 
     variants = {}
     if not args.no_variants:
-        vs = args.variant_steps
         for name, case in (("sphere_f64", sphere_case(cf.STORE_F64)),
                            ("sphere_f16s", sphere_case(cf.STORE_F16S))):
-            log(f"[bench] {name}: {vs} steps")
-            r, fh = run_level(case, init, args.warmup, args.steps if name == "sphere_f64" else vs,
-                              want_forces=True)
+            log(f"[bench] {name}: warmup {args.warmup}, timed {args.steps}")
+            r, _ = run_level(case, init, args.warmup, args.steps)
             variants[name] = r
-            if name == "sphere_f64":
-                fh64 = fh
-            else:
-                fh16 = fh
-        # Cd of reduced storage against FP64 over the same steps
-        n = min(len(fh32), len(fh64))
-        cd32, cd64 = fh32[:n, 4], fh64[:n, 4]
-        tail = slice(max(0, n - 200), n)
-        variants["cd_err_f32_vs_f64"] = float(abs(cd32[tail].mean() - cd64[tail].mean()) /
-                                               abs(cd64[tail].mean()))
-        variants["cd_f64_last"] = float(cd64[-1])
-        variants["cd_f32_last"] = float(cd32[-1])
-        n16 = min(len(fh16), len(fh64))
-        t16 = slice(max(0, n16 - 200), n16)
-        variants["cd_err_f16s_vs_f64"] = float(abs(fh16[t16, 4].mean() - fh64[t16, 4].mean()) /
-                                                abs(fh64[t16, 4].mean()))
+        variants["accuracy"] = accuracy(args.accuracy_steps)
+        vw, vs = args.roofline_warmup, args.variant_steps
         for name, st, force in (("periodic256_f32_F0", cf.STORE_F32, None),
                                 ("periodic256_f32_F1e-6", cf.STORE_F32, (1e-6, 0.0, 0.0)),
                                 ("periodic256_f64_F0", cf.STORE_F64, None),
                                 ("periodic256_f16s_F0", cf.STORE_F16S, None)):
             case = periodic_case(st, force)
             pinit = cf.random_init(case, seed=20261202)
-            log(f"[bench] {name}: {vs} steps")
-            r, _ = run_level(case, pinit, 50, vs)
+            log(f"[bench] {name}: warmup {vw}, timed {vs}")
+            r, _ = run_level(case, pinit, vw, vs)
             variants[name] = r
         # the reference's CPU-runnable case: D2Q9 cylinder 512x256, FP64
         for name, coup in (("cylinder512_f64_explicit", cf.COUPLING_EXPLICIT),
