@@ -193,15 +193,20 @@ struct Batch {
   }
 
   // Tile width of the stream-collide launch for a case of this class and
-  // x extent: the candidate widths 64 / 128 / 256 are timed on a scratch
-  // level of the case's own shape (no IB) and the fastest is kept for the
-  // process, so every later case of the bucket uses the same tiling.
+  // x extent: linear tiles (0) and the widths 64 / 128 / 256 are timed on a
+  // scratch level of the case's own shape (no IB) and the fastest is kept
+  // for the process, so every later case of the bucket uses the same tiling.
   int tile_for(const Level& ref) {
+    // ILBM_BATCH_TILE=0|64|128|256 fixes the tiling (tests of each path)
+    if (const char* fix = std::getenv("ILBM_BATCH_TILE")) {
+      const int t = std::atoi(fix);
+      if (t == 0 || t == 64 || t == 128 || t == 256) return t;
+    }
     const int64_t key = (((int64_t)ref.lat * 4 + ref.st) * 4 + ref.am) * 16 + bucket(ref.c.n[0]);
     auto& tiles = tile_table();
     auto it = tiles.find(key);
     if (it != tiles.end()) return it->second;
-    std::vector<int> cand;
+    std::vector<int> cand{0};
     for (int bx = 64; bx <= 256; bx *= 2) {
       cand.push_back(bx);
       if (bx >= ref.c.n[0]) break;

@@ -529,8 +529,20 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
                            o2 + blockIdx.z, blk, lds);
 }
 
-// Tile (bx, by) of block b of a case's grid, x fastest, as k_step's 2D grid.
+// Tile (bx, by) of block b of a case's grid, x fastest, as k_step's 2D grid;
+// bx = 0: block b holds cells [256 b, 256 b + 256) of the case in x-fastest
+// order (no row padding; a cell past the end gets j = n[1], inactive).
 __device__ inline void batch_tile(const GridGeom& g, int bx, int b, int* i, int* j, int* k) {
+  if (bx == 0) {
+    const uint32_t lin = (uint32_t)b * kBlock + threadIdx.x;
+    const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1];
+    const uint32_t row = lin / nx;
+    *i = (int)(lin - row * nx);
+    *j = (int)(row % ny);
+    *k = (int)(row / ny);
+    if (*k >= g.n[2]) *j = g.n[1];
+    return;
+  }
   const int by = kBlock / bx;
   const int gx = (g.n[0] + bx - 1) / bx, gy = (g.n[1] + by - 1) / by;
   const int bxi = b % gx, rest = b / gx;

@@ -624,6 +624,7 @@ struct Level {
 
   // number of blocks of a tile of width bx (height kBlock / bx)
   int64_t tile_blocks(int bx) const {
+    if (bx == 0) return (ncells + kBlock - 1) / kBlock;  // linear tiles
     const int by = kBlock / bx;
     return (int64_t)((c.n[0] + bx - 1) / bx) * ((c.n[1] + by - 1) / by) * c.n[2];
   }

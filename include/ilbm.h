@@ -193,7 +193,9 @@ int ilbm_batch_checkpoint_size(ilbm_batch* B, int case_id, int64_t* bytes);
 int ilbm_batch_checkpoint(ilbm_batch* B, int case_id, void* host_buf, int64_t bytes);
 int ilbm_batch_add_from_checkpoint(ilbm_batch* B, const ilbm_case_desc* desc, const void* host_buf,
                                    int64_t bytes, int nsteps, int* case_id);
-/* Tile width (cells along x per workgroup) the autotuner chose for a case. */
+/* Tile width (cells along x per workgroup) the autotuner chose for a case;
+ * 0: linear tiles of 256 consecutive cells in x-fastest order (no padding
+ * of rows that are not a multiple of the width). */
 int ilbm_batch_case_tile(ilbm_batch* B, int case_id, int* tile_bx);
 /* [0] resident cases, [1] unfinished cases, [2] bytes charged, [3] budget,
  * [4] graph launches, [5] graph captures, [6] steps launched without a
