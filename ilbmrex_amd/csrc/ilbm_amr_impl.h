@@ -650,7 +650,7 @@ struct Amr {
       if (npad) V.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), V.stream, ga);
     }
     StepArgs sa = V.step_args(export_state);
-    V.stepfn(V.grid, V.block, stream, sa);
+    V.stepfn(V.grid, V.block, stream, sa, nullptr, nullptr);
     exported[l] = export_state;
     V.cur ^= 1;
     const int step_lev = V.istep;

@@ -9,6 +9,7 @@
 //   [wall] k_fill_ghosts      ghost slabs (FillPatch / bounce-back)
 //          k_step             pull + macros + BGK (AdvanceLBMAtLevel.cpp:223-386)
 //                             and Sum(Rho) (AmrCoreLBM.cpp:378)
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -280,14 +281,16 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
 // ---------------------------------------------------------------------------
 // Level
 // ---------------------------------------------------------------------------
-using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&);
+// e0 / e1: timing events recorded by the dispatch itself (kernel begin / end), or null
+using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&, hipEvent_t, hipEvent_t);
 using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
 using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 
 template <class LAT, class ST, int AM>
-static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a) {
-  hipLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, a);
+static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
+  if (e0) hipExtLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, e0, e1, 0, a);
+  else hipLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, a);
 }
 template <class LAT, class ST>
 static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
@@ -1257,12 +1260,12 @@ struct Level {
       ghost_done = xsignal(stream_b, X_GHOST);
     }
     ghost_ahead = false;
-    stepfn(grid, block, stream_b, sf);
+    stepfn(grid, block, stream_b, sf, nullptr, nullptr);
     ring_slot = async_record() ? s % kRecRing : -1;
     ib_update_forcing(time, s);
     ring_slot = -1;
     xwait(stream, X_GHOST, ghost_done);
-    stepfn(grid_near, block, stream, sn);
+    stepfn(grid_near, block, stream, sn, nullptr, nullptr);
     const uint64_t near_done = xsignal(stream, X_NEAR);
     if (more && (near_off_walls || !npad)) {
       // the next step's slabs (from the boundary layer of the far cells
@@ -1362,9 +1365,8 @@ struct Level {
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
       sa.partial = mring + (int64_t)mring_used * nblocks;
-      if (timing) HIPCHK(hipEventRecord(ev_start[s], stream));
-      stepfn(grid, block, stream, sa);
-      if (timing) HIPCHK(hipEventRecord(ev_stop[s], stream));
+      // timing: the dispatch records the kernel's own begin / end
+      stepfn(grid, block, stream, sa, timing ? ev_start[s] : nullptr, timing ? ev_stop[s] : nullptr);
       ring_step(stream, s, !last, false);
       cur ^= 1;
       ++istep;
