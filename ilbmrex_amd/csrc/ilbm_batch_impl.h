@@ -77,6 +77,7 @@ struct BatchCase {
 struct BatchGroup {
   int lat = 0, st = 0;
   int am = kAddrWrap;  // stream-collide addressing of the class's cases
+  int lane = 0;        // stream the group's step chain runs on (0: stream, 1: stream2)
   StepBFn step = nullptr;
   GhostBFn ghost = nullptr;
   FixBFn fix = nullptr;
@@ -149,6 +150,25 @@ static std::map<int64_t, int>& tile_table() {
 
 struct Batch {
   hipStream_t stream = nullptr;
+  // A class of two or more cases is split into two lanes of balanced cell
+  // counts; each lane's whole step chain (IB kernels, slab fill,
+  // stream-collide, tail) runs on its own stream, so one lane's IB kernels
+  // overlap the other lane's stream-collide.  The lanes share nothing within
+  // a step (cases are independent); advance forks stream2 from stream and
+  // joins it back, so every call returns with both ordered behind stream.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // after a fork, lane 1 starts once lane 0's first IB kernels are done: the
+  // lanes then run half a step apart (each lane's IB beside the other's
+  // stream-collide) instead of in phase
+  hipEvent_t ev_phase = nullptr;
+  bool phase_pending = false;
+  // two lanes when the active cases hold at least 2^22 cells: a step is then
+  // long enough for eager launches, and a graph replay would run the two
+  // lanes' branches one after the other on one queue (measured), so two-lane
+  // batches advance eagerly and one-lane batches replay graphs
+  static constexpr int64_t kTwoLaneCells = (int64_t)1 << 22;
+  int nlanes = 1;
   int64_t budget = 0, used = 0;
   int max_cases = 0;
   int graph_steps = 8;
@@ -165,11 +185,27 @@ struct Batch {
   double lups = 0.0;
 
   ~Batch() {
+    if (stream2) (void)hipStreamSynchronize(stream2);
     if (stream) (void)hipStreamSynchronize(stream);
     drop_graph();
     for (auto& g : groups) g.release();
     cases.clear();
+    if (ev_fork) (void)hipEventDestroy(ev_fork);
+    if (ev_join) (void)hipEventDestroy(ev_join);
+    if (ev_phase) (void)hipEventDestroy(ev_phase);
+    if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
+  }
+
+  // stream2 continues after stream's work so far / stream after stream2's
+  void fork() {
+    HIPCHK(hipEventRecord(ev_fork, stream));
+    HIPCHK(hipStreamWaitEvent(stream2, ev_fork, 0));
+    phase_pending = true;
+  }
+  void join() {
+    HIPCHK(hipEventRecord(ev_join, stream2));
+    HIPCHK(hipStreamWaitEvent(stream, ev_join, 0));
   }
 
   void drop_graph() {
@@ -350,6 +386,32 @@ struct Batch {
     for (auto& g : groups) g.release();
     groups.clear();
     phase = 0;
+    // lanes: within a class, cases by decreasing cell count go to the lane
+    // with fewer cells so far (a class of one case stays on lane 0)
+    std::vector<int> lane_of(cases.size(), 0);
+    int64_t active_cells = 0;
+    for (const BatchCase& bc : cases)
+      if (bc.done < bc.target) active_cells += bc.L->ncells;
+    nlanes = (active_cells >= kTwoLaneCells) ? 2 : 1;
+    if (nlanes == 2) {
+      std::map<int64_t, std::vector<int>> cls;
+      for (size_t i = 0; i < cases.size(); ++i) {
+        const BatchCase& bc = cases[i];
+        if (bc.done >= bc.target) continue;
+        const Level& L = *bc.L;
+        cls[((int64_t)L.lat * 4 + L.st) * 4 + L.am].push_back((int)i);
+      }
+      for (auto& kv : cls) {
+        std::vector<int>& v = kv.second;
+        std::stable_sort(v.begin(), v.end(), [&](int a, int b) { return cases[a].L->ncells > cases[b].L->ncells; });
+        int64_t load[2] = {0, 0};
+        for (int i : v) {
+          const int ln = (v.size() > 1 && load[1] < load[0]) ? 1 : 0;
+          lane_of[i] = ln;
+          load[ln] += cases[i].L->ncells;
+        }
+      }
+    }
     for (size_t i = 0; i < cases.size(); ++i) {
       BatchCase& bc = cases[i];
       if (bc.done >= bc.target) continue;  // finished cases stay resident until removed
@@ -357,10 +419,11 @@ struct Batch {
       bc.base_cur = L.cur;
       int gi = -1;
       for (size_t k = 0; k < groups.size(); ++k)
-        if (groups[k].lat == L.lat && groups[k].st == L.st && groups[k].am == L.am) gi = (int)k;
+        if (groups[k].lat == L.lat && groups[k].st == L.st && groups[k].am == L.am && groups[k].lane == lane_of[i])
+          gi = (int)k;
       if (gi < 0) {
         BatchGroup g;
-        g.lat = L.lat; g.st = L.st; g.am = L.am;
+        g.lat = L.lat; g.st = L.st; g.am = L.am; g.lane = lane_of[i];
         pick_class(g.lat, g.st, g.am, g.step, g.ghost, g.fix, g.fixw);
         groups.push_back(g);
         gi = (int)groups.size() - 1;
@@ -462,50 +525,73 @@ struct Batch {
   // ---------------- one batched step ----------------
   // The single Level's order: IB forcing with the export / Guo fix-up and
   // the force record, ghost slabs, stream-collide, Sum(Rho) and the clock.
-  void enqueue_step(int p) {
-    for (auto& G : groups) {
-      const BlockRef* m = G.dmap;
-      auto grid = [&](int k) { return dim3((unsigned)G.mcnt[k]); };
-      if (G.mcnt[BatchGroup::M_IBCASE] > 0) {
-        const IbCase* A = G.dib[p];
-        if (G.any_iter)
-          hipLaunchKernelGGL(k_ib_begin_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
-                             m + G.moff[BatchGroup::M_BAND]);
-        for (int it = 0; it < G.max_loop; ++it) {
-          hipLaunchKernelGGL(k_ib_interp_b, grid(BatchGroup::M_INTERP), dim3(64 * kInterpWaves), 0, stream, A,
-                             m + G.moff[BatchGroup::M_INTERP], it);
-          if (G.any_ivc) {
-            hipLaunchKernelGGL(k_ib_ivc_rhs_b, grid(BatchGroup::M_RHS), dim3(256), 0, stream, A,
-                               m + G.moff[BatchGroup::M_RHS], it);
-            hipLaunchKernelGGL(k_ib_ivc_matvec_b, grid(BatchGroup::M_MATVEC), dim3(256), 0, stream, A,
-                               m + G.moff[BatchGroup::M_MATVEC], it);
-          }
-          if (G.mcnt[BatchGroup::M_SPREAD] > 0)
-            hipLaunchKernelGGL(k_ib_spread_t_b, grid(BatchGroup::M_SPREAD), dim3(256), 0, stream, A,
-                               m + G.moff[BatchGroup::M_SPREAD], it);
-          hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
-                             m + G.moff[BatchGroup::M_IBCASE], it);
-          if (G.any_iter)
-            hipLaunchKernelGGL(k_ib_correct_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
-                               m + G.moff[BatchGroup::M_BAND], it);
+  // One step of a group on its lane's stream; lane0_ib: set once lane 0's
+  // first IB kernels after a fork are queued (and ev_phase recorded)
+  void enqueue_group(BatchGroup& G, int p, hipStream_t stream, bool& lane0_ib) {
+    const BlockRef* m = G.dmap;
+    auto grid = [&](int k) { return dim3((unsigned)G.mcnt[k]); };
+    if (G.mcnt[BatchGroup::M_IBCASE] > 0) {
+      const IbCase* A = G.dib[p];
+      if (G.any_iter)
+        hipLaunchKernelGGL(k_ib_begin_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
+                           m + G.moff[BatchGroup::M_BAND]);
+      for (int it = 0; it < G.max_loop; ++it) {
+        hipLaunchKernelGGL(k_ib_interp_b, grid(BatchGroup::M_INTERP), dim3(64 * kInterpWaves), 0, stream, A,
+                           m + G.moff[BatchGroup::M_INTERP], it);
+        if (G.any_ivc) {
+          hipLaunchKernelGGL(k_ib_ivc_rhs_b, grid(BatchGroup::M_RHS), dim3(256), 0, stream, A,
+                             m + G.moff[BatchGroup::M_RHS], it);
+          hipLaunchKernelGGL(k_ib_ivc_matvec_b, grid(BatchGroup::M_MATVEC), dim3(256), 0, stream, A,
+                             m + G.moff[BatchGroup::M_MATVEC], it);
         }
-        if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
-        if (G.mcnt[BatchGroup::M_FIXW] > 0)
-          G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
-        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
-                           m + G.moff[BatchGroup::M_IBCASE]);
+        if (G.mcnt[BatchGroup::M_SPREAD] > 0)
+          hipLaunchKernelGGL(k_ib_spread_t_b, grid(BatchGroup::M_SPREAD), dim3(256), 0, stream, A,
+                             m + G.moff[BatchGroup::M_SPREAD], it);
+        hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
+                           m + G.moff[BatchGroup::M_IBCASE], it);
+        if (G.any_iter)
+          hipLaunchKernelGGL(k_ib_correct_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
+                             m + G.moff[BatchGroup::M_BAND], it);
       }
-      if (G.mcnt[BatchGroup::M_GHOST] > 0)
-        G.ghost(grid(BatchGroup::M_GHOST), stream, G.dghost[p], m + G.moff[BatchGroup::M_GHOST]);
-      G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
-      hipLaunchKernelGGL(k_case_tail_b, grid(BatchGroup::M_TAIL), dim3(1024), 0, stream, G.dtail);
+      if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
+      if (G.mcnt[BatchGroup::M_FIXW] > 0)
+        G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
+      hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
+                         m + G.moff[BatchGroup::M_IBCASE]);
+      if (G.lane == 0 && phase_pending && !lane0_ib) {
+        HIPCHK(hipEventRecord(ev_phase, stream));
+        lane0_ib = true;
+      }
+    }
+    if (G.mcnt[BatchGroup::M_GHOST] > 0)
+      G.ghost(grid(BatchGroup::M_GHOST), stream, G.dghost[p], m + G.moff[BatchGroup::M_GHOST]);
+    G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
+    hipLaunchKernelGGL(k_case_tail_b, grid(BatchGroup::M_TAIL), dim3(1024), 0, stream, G.dtail);
+  }
+
+  void enqueue_step(int p) {
+    bool lane0_ib = false;
+    for (int lane = 0; lane < 2; ++lane) {
+      for (auto& G : groups) {
+        if (G.lane != lane) continue;
+        if (phase_pending && lane == 1) {
+          // the first lane-1 work after a fork waits for lane 0's IB kernels
+          // (or for lane 0's whole step if lane 0 has none)
+          if (!lane0_ib) HIPCHK(hipEventRecord(ev_phase, stream));
+          HIPCHK(hipStreamWaitEvent(stream2, ev_phase, 0));
+          phase_pending = false;
+        }
+        enqueue_group(G, p, lane ? stream2 : stream, lane0_ib);
+      }
     }
   }
 
   void capture() {
     drop_graph();
     HIPCHK(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+    fork();
     for (int s = 0; s < graph_steps; ++s) enqueue_step(s & 1);
+    join();
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(stream, &gr);
     if (e != hipSuccess || !gr) {
@@ -538,8 +624,10 @@ struct Batch {
     const int k = std::min(max_steps, remaining_min());
     if (k <= 0) return 0;
     int left = k;
+    bool forked = false;  // eager steps since the last fork run on both streams
     while (left > 0) {
-      if (use_graph && phase == 0 && left >= graph_steps && graph_steps >= 2 && !graph_failed) {
+      if (use_graph && nlanes == 1 && phase == 0 && left >= graph_steps && graph_steps >= 2 && !graph_failed) {
+        if (forked) { join(); forked = false; }
         if (!gexec) capture();
         if (gexec) {
           HIPCHK(hipGraphLaunch(gexec, stream));
@@ -548,12 +636,14 @@ struct Batch {
           continue;
         }
       }
+      if (!forked) { fork(); forked = true; }
       enqueue_step(phase);
       HIPCHK(hipGetLastError());
       phase ^= 1;
       ++eager_steps;
       --left;
     }
+    if (forked) join();
     HIPCHK(hipStreamSynchronize(stream));
     // host bookkeeping and the streamed-back histories of the k steps
     for (auto& bc : cases) {
@@ -609,6 +699,10 @@ int ilbm_batch_create(int64_t hbm_budget_bytes, int max_cases, int graph_steps, 
     h->B.max_cases = max_cases;
     h->B.graph_steps = (graph_steps / 2) * 2;
     HIPCHK(hipStreamCreateWithFlags(&h->B.stream, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&h->B.stream2, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&h->B.ev_fork, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->B.ev_join, hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&h->B.ev_phase, hipEventDisableTiming));
   });
   if (st != ILBM_OK) {
     delete h;
