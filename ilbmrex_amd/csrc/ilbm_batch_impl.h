@@ -547,8 +547,10 @@ struct Batch {
         if (G.mcnt[BatchGroup::M_SPREAD] > 0)
           hipLaunchKernelGGL(k_ib_spread_t_b, grid(BatchGroup::M_SPREAD), dim3(256), 0, stream, A,
                              m + G.moff[BatchGroup::M_SPREAD], it);
-        hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
-                           m + G.moff[BatchGroup::M_IBCASE], it);
+        // (a no-op for cases of one iteration, whose sums come with the record)
+        if (G.max_loop > 1)
+          hipLaunchKernelGGL(k_ib_sums_b, grid(BatchGroup::M_IBCASE), dim3(kSumsBlock), 0, stream, A,
+                             m + G.moff[BatchGroup::M_IBCASE], it);
         if (G.any_iter)
           hipLaunchKernelGGL(k_ib_correct_b, grid(BatchGroup::M_BAND), dim3(256), 0, stream, A,
                              m + G.moff[BatchGroup::M_BAND], it);
@@ -556,7 +558,7 @@ struct Batch {
       if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
       if (G.mcnt[BatchGroup::M_FIXW] > 0)
         G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
-      hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(1024), 0, stream, A,
+      hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(kSumsBlock), 0, stream, A,
                          m + G.moff[BatchGroup::M_IBCASE]);
       if (G.lane == 0 && phase_pending && !lane0_ib) {
         HIPCHK(hipEventRecord(ev_phase, stream));

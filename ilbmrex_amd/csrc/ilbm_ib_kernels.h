@@ -623,6 +623,13 @@ __global__ void __launch_bounds__(256) k_ib_move_weights(MoveArgs mv, IbArgs a) 
   }
 }
 
+// Block size of the one-block reductions (marker sums, force record): every
+// launch of a level, a batch or a hierarchy uses it, so the trees are the
+// same everywhere.  256 threads rather than 1024: a 1024-thread block waits
+// long for a CU beside a bandwidth-bound stream-collide (a no-op one took
+// 128 us in the two-lane batch).
+constexpr int kSumsBlock = 256;
+
 // Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
 // iterations) and the moments r x (F dv) (:1500-1503), with their sum as
 // the body torque.  One block, fixed reduction tree.
@@ -654,7 +661,7 @@ __device__ __forceinline__ void sums_body(const IbArgs& a, int first_iter) {
   }
 }
 
-__global__ void __launch_bounds__(1024) k_ib_sums(IbArgs a) { sums_body(a, a.first_iter); }
+__global__ void __launch_bounds__(kSumsBlock) k_ib_sums(IbArgs a) { sums_body(a, a.first_iter); }
 
 __device__ __forceinline__ void correct_body(const IbArgs& a, int64_t b) {
   if (b >= a.band_cells) return;
@@ -993,8 +1000,8 @@ __device__ __forceinline__ void force_record_body(const IbArgs& a, const RecArgs
 
 __global__ void __launch_bounds__(256) k_force_record(IbArgs a, RecArgs rec, int nblk) { force_record_body(a, rec, nblk); }
 
-// Single-iteration forcing: the marker sums (k_ib_sums, same 1024-thread
-// tree) and the record in one launch.
+// Single-iteration forcing: the marker sums (k_ib_sums, same tree) and the
+// record in one launch.
 __device__ __forceinline__ void sums_record_body(const IbArgs& a, const RecArgs& rec, int nblk) {
   __shared__ double lds[3][16];
   sums_body(a, 1);
@@ -1015,7 +1022,9 @@ __device__ __forceinline__ void sums_record_body(const IbArgs& a, const RecArgs&
   }
 }
 
-__global__ void __launch_bounds__(1024) k_sums_record(IbArgs a, RecArgs rec, int nblk) { sums_record_body(a, rec, nblk); }
+__global__ void __launch_bounds__(kSumsBlock) k_sums_record(IbArgs a, RecArgs rec, int nblk) {
+  sums_record_body(a, rec, nblk);
+}
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of
 // the exported force and the marker sums, appended to the history.
@@ -1126,7 +1135,7 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_b(const IbCase*
   spread_body(a, r.b);
 }
 
-__global__ void __launch_bounds__(1024) k_ib_sums_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map,
+__global__ void __launch_bounds__(kSumsBlock) k_ib_sums_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map,
                                                     int it) {
   const IbArgs& a = A[map[blockIdx.x].c].ib;
   if (it >= a.nloop || a.nloop == 1) return;
@@ -1157,7 +1166,8 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_export_w_b(cons
   spread_export_wave_body<LAT, ST>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
 }
 
-__global__ void __launch_bounds__(1024) k_force_record_b(const IbCase* __restrict__ A, const BlockRef* __restrict__ map) {
+__global__ void __launch_bounds__(kSumsBlock) k_force_record_b(const IbCase* __restrict__ A,
+                                                         const BlockRef* __restrict__ map) {
   const IbCase& c = A[map[blockIdx.x].c];
   if (c.ib.nloop == 1) sums_record_body(c.ib, c.rec, c.nblk_fix);
   else force_record_body(c.ib, c.rec, c.nblk_fix);

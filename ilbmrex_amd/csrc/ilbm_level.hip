@@ -971,7 +971,7 @@ struct Level {
         hipLaunchKernelGGL(k_ib_spread_all, dim3(gb), dim3(tb), 0, stream, a);
       else if (n_ne)
         hipLaunchKernelGGL(k_ib_spread_t, dim3((n_ne + 255) / 256), dim3(256), 0, stream, a);
-      hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(1024), 0, stream, a);
+      hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(kSumsBlock), 0, stream, a);
       if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
     }
     RecArgs r{};
@@ -991,10 +991,10 @@ struct Level {
       if (rec_enable && ring_slot >= 0) {
         // off the step's critical path: the record reads only its slot
         xwait(stream_c, X_EXP, xsignal(stream, X_EXP));
-        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream_c, a, r, (int)gfix);
+        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream_c, a, r, (int)gfix);
         rec_seq[ring_slot] = xsignal(stream_c, X_REC0 + ring_slot);
       } else if (rec_enable) {
-        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(1024), 0, stream, a, r, (int)gfix);
+        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream, a, r, (int)gfix);
       }
     } else if (rec_enable) {
       hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gfix);
