@@ -286,6 +286,26 @@ __device__ inline void finish_mass(const StepArgs& a, double s, int64_t blk) {
 // ---------------------------------------------------------------------------
 // k_step: pull-stream + forced macros + BGK collision, one pass.
 // ---------------------------------------------------------------------------
+// Index of the first nonzero entry of a lattice component table.
+__host__ __device__ constexpr int first_nonzero(const int* c, int n) {
+  for (int q = 0; q < n; ++q)
+    if (c[q] != 0) return q;
+  return -1;
+}
+
+// acc (+/-)= x for a term of sign S (0: no term); term K, the first nonzero
+// one (FIRST), assigns instead of adding.
+template <int FIRST, int K, int S, class T>
+__device__ __forceinline__ void accumulate_signed(T& acc, T x) {
+  if constexpr (S > 0) {
+    if constexpr (K == FIRST) acc = x;
+    else acc += x;
+  } else if constexpr (S < 0) {
+    if constexpr (K == FIRST) acc = -x;
+    else acc -= x;
+  }
+}
+
 template <class LAT, class ST, int AM>
 __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k, int64_t blk, double* lds) {
   using T = typename ST::compute_t;
@@ -393,16 +413,29 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     // term whose direction component is 0 is left out and -1 * f is
     // subtracted, which leaves every finite sum bit-identical
     T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
-    static_for<LAT::Q>([&](auto qc) {
-      constexpr int q = decltype(qc)::value;
-      r += f[q];
-      if constexpr (LAT::cx[q] > 0) m1 += f[q];
-      else if constexpr (LAT::cx[q] < 0) m1 -= f[q];
-      if constexpr (LAT::cy[q] > 0) m2 += f[q];
-      else if constexpr (LAT::cy[q] < 0) m2 -= f[q];
-      if constexpr (LAT::cz[q] > 0) m3 += f[q];
-      else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
-    });
+    if constexpr (ST::exact) {
+      static_for<LAT::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        r += f[q];
+        if constexpr (LAT::cx[q] > 0) m1 += f[q];
+        else if constexpr (LAT::cx[q] < 0) m1 -= f[q];
+        if constexpr (LAT::cy[q] > 0) m2 += f[q];
+        else if constexpr (LAT::cy[q] < 0) m2 -= f[q];
+        if constexpr (LAT::cz[q] > 0) m3 += f[q];
+        else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
+      });
+    } else {
+      // reduced precision: each sum starts at its first term (0 + x cannot
+      // be folded away in IEEE arithmetic, -0 + 0 = +0)
+      static_for<LAT::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (q == 0) r = f[q];
+        else r += f[q];
+        accumulate_signed<first_nonzero(LAT::cx, LAT::Q), q, LAT::cx[q]>(m1, f[q]);
+        accumulate_signed<first_nonzero(LAT::cy, LAT::Q), q, LAT::cy[q]>(m2, f[q]);
+        accumulate_signed<first_nonzero(LAT::cz, LAT::Q), q, LAT::cz[q]>(m3, f[q]);
+      });
+    }
     if constexpr (SH) r += T(1);
     const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
     const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
@@ -458,7 +491,11 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const T fac = T(1.0) - T(0.5) / T(a.tau);
     const I self = (I)g.off0 + ox[1] + oy[1] + oz[1];
     auto put = [&](int q, T gq) {
-      if (O32 || a.off32) {
+      if constexpr (O32) {
+        // a 32-bit byte offset from the buffer base, as the loads
+        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(gout) + ((I)q * (I)g.qs + self) * (uint32_t)sizeof(R));
+        pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
+      } else if (a.off32) {
         R* plane = gout + (int64_t)q * g.qs;
         R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
         pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
@@ -490,15 +527,13 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
           if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
           put(q, gq);
         } else if constexpr (p > q) {
+          // c.u from its nonzero components, starting at the first
+          constexpr int c0 = LAT::cx[q], c1 = LAT::cy[q], c2 = (LAT::D == 3) ? LAT::cz[q] : 0;
+          constexpr int k0 = c0 ? 0 : (c1 ? 1 : 2);
           T eu = T(0);
-          if constexpr (LAT::cx[q] > 0) eu += u;
-          else if constexpr (LAT::cx[q] < 0) eu -= u;
-          if constexpr (LAT::cy[q] > 0) eu += v;
-          else if constexpr (LAT::cy[q] < 0) eu -= v;
-          if constexpr (LAT::D == 3) {
-            if constexpr (LAT::cz[q] > 0) eu += w;
-            else if constexpr (LAT::cz[q] < 0) eu -= w;
-          }
+          accumulate_signed<k0, 0, c0>(eu, u);
+          accumulate_signed<k0, 1, c1>(eu, v);
+          accumulate_signed<k0, 2, c2>(eu, w);
           const T wr = T(LAT::w[q]) * r;
           const T A = SH ? __builtin_fmaf(wr, __builtin_fmaf(T(4.5) * eu, eu, base), -T(LAT::w[q]))
                          : wr * __builtin_fmaf(T(4.5) * eu, eu, base);
