@@ -46,6 +46,7 @@ static void launch_fixw_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef
 template <class LAT, class ST>
 static void pick_b(int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
   sf = (am == kAddrWrap) ? launch_step_b<LAT, ST, kAddrWrap>
+     : (am == kAddrWallX32) ? launch_step_b<LAT, ST, kAddrWallX32>
      : (am == kAddrWall32) ? launch_step_b<LAT, ST, kAddrWall32> : launch_step_b<LAT, ST, kAddrWall64>;
   gf = launch_ghost_b<LAT, ST>;
   ff = launch_fix_b<LAT, ST>;

@@ -44,7 +44,10 @@ constexpr int kGhostBlock = 64;
 //   kAddrWrap   an all-periodic level, 32-bit byte offsets from a plane base
 //   kAddrWall32 a level with non-periodic faces whose populations and ghost
 //               slabs span under 4 GiB: 32-bit byte offsets from the buffer
-enum : int { kAddrWall64 = 0, kAddrWrap = 1, kAddrWall32 = 2 };
+//   kAddrWallX32 as kAddrWall32 with the non-periodic faces in x only (the
+//               channel layout of the cylinder and sphere cases): a ghost
+//               select only for directions with an x component
+enum : int { kAddrWall64 = 0, kAddrWrap = 1, kAddrWall32 = 2, kAddrWallX32 = 3 };
 
 // f(std::integral_constant<int, 0>) ... f(std::integral_constant<int, N - 1>):
 // a loop over the directions whose index is a constant expression
@@ -311,6 +314,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   constexpr bool WRAP = AM == kAddrWrap;
+  constexpr bool XO = AM == kAddrWallX32;
   // offsets in 32-bit unsigned arithmetic: every intermediate is exact
   // modulo 2^32 and every final offset lies below 2^32
   constexpr bool O32 = AM != kAddrWall64;
@@ -350,7 +354,17 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     // are issued unconditionally; only their offsets are selected.
     bool outx[3] = {false, false, false}, outy[3] = {false, false, false}, outz[3] = {false, false, false};
     I tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if constexpr (!WRAP) {
+    if constexpr (XO) {
+      // both x faces span the same (y, z) range, so their strides are
+      // uniform; only the face's offset differs between the two sides
+      outx[0] = (i == g.n[0] - 1);
+      outx[2] = (i == 0);
+      const int f0 = a.slabs.slab_of[0][0], f1 = a.slabs.slab_of[0][1];
+      mu[0][1] = 1;
+      mu[0][2] = (I)a.slabs.face_n[f0][1];
+      const I base = (i == 0) ? (I)a.slabs.face_off[f0] : (I)a.slabs.face_off[f1];
+      tb[0] = base + (I)(j - a.slabs.face_lo[f0][1]) + (I)(k - a.slabs.face_lo[f0][2]) * mu[0][2];
+    } else if constexpr (!WRAP) {
       const int cc[3] = {i, j, k};
       bool* outs[3] = {outx, outy, outz};
 #pragma unroll
@@ -392,6 +406,15 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       // non-periodic face the slab of the lowest such dimension it leaves
       // (a per-lane select: one load per direction on every lane)
       I off = (I)q * (I)g.qs + (I)g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+      if constexpr (XO) {
+        if (LAT::cx[q] != 0) {
+          const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
+          off = outx[LAT::cx[q] + 1] ? gofs + (I)q * npad + t0 : off;
+        }
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * (uint32_t)sizeof(R));
+        f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
+        continue;
+      }
       const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
       const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
       const I t1 = tb[1] - (I)LAT::cx[q] * mu[1][0] - (I)LAT::cz[q] * mu[1][2];

@@ -315,6 +315,7 @@ static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
 template <class LAT, class ST>
 static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf) {
   sf = (am == kAddrWrap) ? launch_step<LAT, ST, kAddrWrap>
+     : (am == kAddrWallX32) ? launch_step<LAT, ST, kAddrWallX32>
      : (am == kAddrWall32) ? launch_step<LAT, ST, kAddrWall32> : launch_step<LAT, ST, kAddrWall64>;
   inf = launch_init<LAT, ST>;
   ff = launch_fix<LAT, ST>;
@@ -590,6 +591,10 @@ struct Level {
     const bool force64 = f64 && f64[0] == '1';
     am = wrap ? kAddrWrap
          : (!force64 && (int64_t)popbytes * (g.pop_n + npad * Q) < ((int64_t)1 << 32)) ? kAddrWall32 : kAddrWall64;
+    // non-periodic faces in x only (ILBM_ADDR_GENERIC=1 keeps the select of
+    // every dimension)
+    const char* gen = std::getenv("ILBM_ADDR_GENERIC");
+    if (am == kAddrWall32 && nonper[0] && !nonper[1] && !nonper[2] && !(gen && gen[0] == '1')) am = kAddrWallX32;
     for (int b = 0; b < 2; ++b) {
       // Q population planes followed by the Q ghost slabs
       const size_t bytes = popbytes * ((size_t)g.pop_n + (size_t)npad * Q);
@@ -1643,7 +1648,7 @@ int ilbm_level_info(ilbm_level* h, double out[8]) {
   const ilbm::Level& L = h->L;
   out[0] = L.tau; out[1] = L.dt; out[2] = L.t_new; out[3] = L.istep;
   out[4] = (double)L.band_cells; out[5] = (double)L.popbytes;
-  out[6] = (L.am == ilbm::kAddrWrap) ? 0.0 : (L.am == ilbm::kAddrWall64) ? 1.0 : 2.0;
+  out[6] = (L.am == ilbm::kAddrWrap) ? 0.0 : (L.am == ilbm::kAddrWall64) ? 1.0 : (L.am == ilbm::kAddrWall32) ? 2.0 : 3.0;
   out[7] = (double)L.nblocks;
   return ILBM_OK;
 }

@@ -233,17 +233,23 @@ def test_large_3d_slice_layout_bitexact():
     assert_bitexact(lev.macro(), orc.macro(), "macro")
 
 
-@pytest.mark.parametrize("force64", [False, True])
-def test_walled_addressing_modes_bitexact(force64, monkeypatch):
+@pytest.mark.parametrize("mode", ["default", "generic", "force64"])
+def test_walled_addressing_modes_bitexact(mode, monkeypatch):
     """Walled levels under 4 GiB address the populations and ghost slabs with
-    32-bit offsets; ILBM_FORCE_ADDR64=1 takes the 64-bit path of larger
-    levels.  Both against the oracle: the mixed-wall D3Q19 box (every face
-    kind, edges and corners), FP64, and a D2Q9 channel."""
-    if force64:
+    32-bit offsets, and levels walled in x only select a ghost source only
+    for directions with an x component; ILBM_ADDR_GENERIC=1 keeps the
+    select of every dimension and ILBM_FORCE_ADDR64=1 takes the 64-bit path
+    of larger levels.  Each against the oracle: the mixed-wall D3Q19 box
+    (every face kind, edges and corners), FP64, a D2Q9 channel and a D3Q19
+    channel (x walls, y and z periodic)."""
+    monkeypatch.delenv("ILBM_FORCE_ADDR64", raising=False)
+    monkeypatch.delenv("ILBM_ADDR_GENERIC", raising=False)
+    if mode == "force64":
         monkeypatch.setenv("ILBM_FORCE_ADDR64", "1")
-    else:
-        monkeypatch.delenv("ILBM_FORCE_ADDR64", raising=False)
-    want = "wall64" if force64 else "wall32"
+    if mode == "generic":
+        monkeypatch.setenv("ILBM_ADDR_GENERIC", "1")
+    want_box = "wall64" if mode == "force64" else "wall32"
+    want_ch = {"default": "wall32x", "generic": "wall32", "force64": "wall64"}[mode]
     w1 = (1.0, 0.01, 0.0, 0.0)
     w2 = (1.0, 0.0, 0.02, 0.0)
     c = cf.Case(lattice=cf.D3Q19, n=(11, 9, 7), nu=0.04, periodic=(0, 0, 0),
@@ -251,11 +257,18 @@ def test_walled_addressing_modes_bitexact(force64, monkeypatch):
                 bc_lo_val=((1.0, 0.03, 0.0, 0.0), w1, w2), bc_hi_val=((0.0,) * 4, w1, w2),
                 name="mixed3d_addr")
     lev, orc = run_pair(c, cf.random_init(c, seed=16, du=0.02), 9)
-    assert lev.info()["addressing"] == want
+    assert lev.info()["addressing"] == want_box
     assert_bitexact(lev.f(), orc.f(), "f")
     assert_bitexact(lev.macro(), orc.macro(), "macro")
     ch = cf.cylinder_2d(nx=40, ny=24).replace(use_ib=False, name="channel_addr")
     lev, orc = run_pair(ch, cf.random_init(ch, seed=17), 12)
-    assert lev.info()["addressing"] == want
+    assert lev.info()["addressing"] == want_ch
+    assert_bitexact(lev.f(), orc.f(), "f")
+    assert_bitexact(lev.macro(), orc.macro(), "macro")
+    ch3 = cf.sphere_3d(nx=14, ny=10, nz=9, D=4.0, center=(5.0, 5.0, 4.5), U=0.04, Re=20.0).replace(
+        use_ib=False, name="channel3d_addr")
+    assert tuple(ch3.periodic) == (0, 1, 1)
+    lev, orc = run_pair(ch3, cf.random_init(ch3, seed=18, du=0.02), 7)
+    assert lev.info()["addressing"] == want_ch
     assert_bitexact(lev.f(), orc.f(), "f")
     assert_bitexact(lev.macro(), orc.macro(), "macro")
