@@ -332,16 +332,23 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const R* __restrict__ gin = static_cast<const R*>(a.gin);
     R* __restrict__ gout = static_cast<R*>(a.gout);
     // source offsets for c = -1, 0, +1 (source coordinate = x - c); periodic
-    // dimensions wrap
+    // dimensions wrap.  Shifted storage (bound by vector ALU issue) holds
+    // byte offsets in the 32-bit modes (SB bytes per element), so that no
+    // direction scales its offset again (FP16-shifted 2D cylinder +3 %);
+    // the other formats scale each offset by EB where it is used.
+    constexpr uint32_t SB = (O32 && ST::shifted) ? (uint32_t)sizeof(R) : 1u;
+    constexpr uint32_t EB = O32 ? (uint32_t)sizeof(R) / SB : 1u;
     I ox[3], oy[3], oz[3];
     {
       const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
       const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
-      ox[0] = (I)xp; ox[1] = (I)i; ox[2] = (I)xm;
-      oy[0] = (I)yp * (I)g.js; oy[1] = (I)j * (I)g.js; oy[2] = (I)ym * (I)g.js;
+      const I jsb = (I)g.js * (I)SB;
+      ox[0] = (I)xp * (I)SB; ox[1] = (I)i * (I)SB; ox[2] = (I)xm * (I)SB;
+      oy[0] = (I)yp * jsb; oy[1] = (I)j * jsb; oy[2] = (I)ym * jsb;
       if constexpr (LAT::D == 3) {
+        const I ksb = (I)g.ks * (I)SB;
         const int zm = (k == 0) ? g.n[2] - 1 : k - 1, zp = (k == g.n[2] - 1) ? 0 : k + 1;
-        oz[0] = (I)zp * (I)g.ks; oz[1] = (I)k * (I)g.ks; oz[2] = (I)zm * (I)g.ks;
+        oz[0] = (I)zp * ksb; oz[1] = (I)k * ksb; oz[2] = (I)zm * ksb;
       } else {
         oz[0] = oz[1] = oz[2] = 0;
       }
@@ -360,10 +367,11 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       outx[0] = (i == g.n[0] - 1);
       outx[2] = (i == 0);
       const int f0 = a.slabs.slab_of[0][0], f1 = a.slabs.slab_of[0][1];
-      mu[0][1] = 1;
-      mu[0][2] = (I)a.slabs.face_n[f0][1];
+      const I fn1 = (I)a.slabs.face_n[f0][1];
+      mu[0][1] = (I)SB;
+      mu[0][2] = fn1 * (I)SB;
       const I base = (i == 0) ? (I)a.slabs.face_off[f0] : (I)a.slabs.face_off[f1];
-      tb[0] = base + (I)(j - a.slabs.face_lo[f0][1]) + (I)(k - a.slabs.face_lo[f0][2]) * mu[0][2];
+      tb[0] = (base + (I)(j - a.slabs.face_lo[f0][1]) + (I)(k - a.slabs.face_lo[f0][2]) * fn1) * (I)SB;
     } else if constexpr (!WRAP) {
       const int cc[3] = {i, j, k};
       bool* outs[3] = {outx, outy, outz};
@@ -379,14 +387,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         for (int e = 0; e < 3; ++e) {
           if (e == d) continue;
           t += (I)(cc[e] - a.slabs.face_lo[f][e]) * mul;
-          mu[d][e] = mul;
+          mu[d][e] = mul * (I)SB;
           mul *= (I)a.slabs.face_n[f][e];
         }
-        tb[d] = t;
+        tb[d] = t * (I)SB;
       }
     }
-    const I gofs = (I)g.pop_n;                          // start of the slabs
-    const I npad = (I)a.slabs.face_off[a.slabs.nface];
+    const I gofs = (I)g.pop_n * (I)SB;                  // start of the slabs
+    const I npad = (I)a.slabs.face_off[a.slabs.nface] * (I)SB;
     T f[LAT::Q];
 #pragma unroll
     for (int q = 0; q < LAT::Q; ++q) {
@@ -395,7 +403,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
         // from the plane's (uniform) base
         const uint32_t src = (uint32_t)(ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
         const R* plane = gin + (int64_t)q * g.qs;
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * (uint32_t)sizeof(R));
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * EB);
         // 3D: non-temporal loads (256^3 FP64 5.66 -> 5.94 TB/s, FP32 128^3
         // +1 %); on walled and 2D levels they cost 2-9 %
         if constexpr (LAT::D == 3) f[q] = pop_in_nt<ST>(p, 0, T(LAT::w[q]));
@@ -405,13 +413,13 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       // the source cell inside the level, or for a source beyond a
       // non-periodic face the slab of the lowest such dimension it leaves
       // (a per-lane select: one load per direction on every lane)
-      I off = (I)q * (I)g.qs + (I)g.off0 + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+      I off = ((I)q * (I)g.qs + (I)g.off0) * (I)SB + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
       if constexpr (XO) {
         if (LAT::cx[q] != 0) {
           const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
           off = outx[LAT::cx[q] + 1] ? gofs + (I)q * npad + t0 : off;
         }
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * (uint32_t)sizeof(R));
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
         f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
         continue;
       }
@@ -422,7 +430,7 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       const I t = o0 ? t0 : (o1 ? t1 : t2);
       off = (o0 | o1 | o2) ? gofs + (I)q * npad + t : off;
       if constexpr (O32) {
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * (uint32_t)sizeof(R));
+        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
         f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
       } else {
         f[q] = pop_in<ST>(gin, off, T(LAT::w[q]));
@@ -512,11 +520,11 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const T cfx = T(a.Fcol[0]), cfy = T(a.Fcol[1]), cfz = T(a.Fcol[2]);
     const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
     const T fac = T(1.0) - T(0.5) / T(a.tau);
-    const I self = (I)g.off0 + ox[1] + oy[1] + oz[1];
+    const I self = (I)g.off0 * (I)SB + ox[1] + oy[1] + oz[1];
     auto put = [&](int q, T gq) {
       if constexpr (O32) {
         // a 32-bit byte offset from the buffer base, as the loads
-        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(gout) + ((I)q * (I)g.qs + self) * (uint32_t)sizeof(R));
+        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(gout) + ((I)q * (I)g.qs * (I)SB + self) * EB);
         pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
       } else if (a.off32) {
         R* plane = gout + (int64_t)q * g.qs;
