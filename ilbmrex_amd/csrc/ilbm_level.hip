@@ -1043,6 +1043,10 @@ struct Level {
     a.slabs = slabs;
     a.tile_bx = (int)block.x;
     a.off32 = ((int64_t)popbytes * g.pop_n < ((int64_t)1 << 32)) ? 1 : 0;
+    a.ftau = (float)a.tau;
+    a.finv_tau = (float)a.inv_tau;
+    a.ffac = 1.0f - 0.5f / a.ftau;
+    for (int d = 0; d < 3; ++d) { a.fFmac[d] = (float)a.Fmac[d]; a.fFcol[d] = (float)a.Fcol[d]; }
     return a;
   }
 
