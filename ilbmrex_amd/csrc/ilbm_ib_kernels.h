@@ -1026,6 +1026,20 @@ __global__ void __launch_bounds__(kSumsBlock) k_sums_record(IbArgs a, RecArgs re
   sums_record_body(a, rec, nblk);
 }
 
+// A step's ghost slabs and the force record of its forcing in one launch
+// (levels that step on one stream): blocks [0, nghost) fill kSumsBlock slab
+// cells each, the last block writes the record.  Neither reads what the
+// other writes, and the stream-collide launch that follows needs both.
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kSumsBlock) k_ghosts_record(GhostArgs g, unsigned nghost, IbArgs a, RecArgs rec,
+                                                               int nblk) {
+  if (blockIdx.x < nghost) {
+    ghost_body<LAT, ST>(g, (int64_t)blockIdx.x * kSumsBlock + threadIdx.x);
+    return;
+  }
+  sums_record_body(a, rec, nblk);
+}
+
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of
 // the exported force and the marker sums, appended to the history.
 __device__ void write_force_record(const IbArgs& a, const RecArgs& rc, const double sum_in[3]) {

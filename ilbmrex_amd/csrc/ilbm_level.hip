@@ -286,6 +286,7 @@ using StepFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&, hipEvent_t, hi
 using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
 using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
+using GhostRecFn = void (*)(unsigned, hipStream_t, const GhostArgs&, unsigned, const IbArgs&, const RecArgs&, int);
 
 template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
@@ -313,13 +314,20 @@ static void launch_ghost(dim3 g, dim3 b, hipStream_t s, const GhostArgs& a) {
 }
 
 template <class LAT, class ST>
-static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf) {
+static void launch_ghost_rec(unsigned grid, hipStream_t s, const GhostArgs& g, unsigned nghost, const IbArgs& a,
+                             const RecArgs& r, int nblk) {
+  hipLaunchKernelGGL((k_ghosts_record<LAT, ST>), dim3(grid), dim3(kSumsBlock), 0, s, g, nghost, a, r, nblk);
+}
+
+template <class LAT, class ST>
+static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf, GhostRecFn& grf) {
   sf = (am == kAddrWrap) ? launch_step<LAT, ST, kAddrWrap>
      : (am == kAddrWallX32) ? launch_step<LAT, ST, kAddrWallX32>
      : (am == kAddrWall32) ? launch_step<LAT, ST, kAddrWall32> : launch_step<LAT, ST, kAddrWall64>;
   inf = launch_init<LAT, ST>;
   ff = launch_fix<LAT, ST>;
   gf = launch_ghost<LAT, ST>;
+  grf = launch_ghost_rec<LAT, ST>;
 }
 
 template <class LAT>
@@ -368,6 +376,13 @@ struct Level {
   InitFn initfn = nullptr;
   FixFn fixfn = nullptr;
   GhostFn ghostfn = nullptr;
+  GhostRecFn ghostrecfn = nullptr;
+  // one-stream steps: the force record of the forcing is held back
+  // (rec_to_ghost) and launched with the ghost slabs of the same step
+  bool rec_to_ghost = false, rec_held = false;
+  IbArgs held_a{};
+  RecArgs held_r{};
+  int held_nblk = 0;
   double tau = 0, dt = 0, t_new = 0, t_old = -1.0e100;
   int istep = 0;
   hipStream_t stream = nullptr;
@@ -619,9 +634,9 @@ struct Level {
 
     auto pk = [&](auto latt) {
       using L = decltype(latt);
-      if (st == 0) pick<L, StF64>(am, stepfn, initfn, fixfn, ghostfn);
-      else if (st == 1) pick<L, StF32>(am, stepfn, initfn, fixfn, ghostfn);
-      else pick<L, StF16S>(am, stepfn, initfn, fixfn, ghostfn);
+      if (st == 0) pick<L, StF64>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
+      else if (st == 1) pick<L, StF32>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
+      else pick<L, StF16S>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
     };
     if (lat == 0) pk(D2Q9{});
     else if (lat == 1) pk(D3Q19{});
@@ -998,6 +1013,9 @@ struct Level {
         xwait(stream_c, X_EXP, xsignal(stream, X_EXP));
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream_c, a, r, (int)gfix);
         rec_seq[ring_slot] = xsignal(stream_c, X_REC0 + ring_slot);
+      } else if (rec_enable && rec_to_ghost) {
+        rec_held = true;
+        held_a = a; held_r = r; held_nblk = (int)gfix;
       } else if (rec_enable) {
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream, a, r, (int)gfix);
       }
@@ -1365,11 +1383,22 @@ struct Level {
         HIPCHK(hipGetLastError());
         continue;
       }
-      if (ib) ib_update_forcing(time, s);
-      if (!wrap) {
+      const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
+      if (ib) {
+        rec_to_ghost = npad > 0;
+        ib_update_forcing(time, s);
+        rec_to_ghost = false;
+      }
+      if (npad) {
         GhostArgs ga = ghost_args();
-        const int64_t npad = slabs.face_off[slabs.nface];
-        if (npad) ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, ga);
+        if (rec_held) {
+          // one launch fewer per step (512x256 cylinder: 5 -> 4 launches)
+          const unsigned ng = (unsigned)((npad + kSumsBlock - 1) / kSumsBlock);
+          ghostrecfn(ng + 1, stream, ga, ng, held_a, held_r, held_nblk);
+          rec_held = false;
+        } else {
+          ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, ga);
+        }
       }
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
