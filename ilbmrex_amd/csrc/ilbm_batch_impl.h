@@ -24,6 +24,8 @@ namespace ilbm {
 using StepBFn = void (*)(dim3, hipStream_t, const StepArgs*, const BlockRef*);
 using GhostBFn = void (*)(dim3, hipStream_t, const GhostArgs*, const BlockRef*);
 using FixBFn = void (*)(dim3, hipStream_t, const IbCase*, const BlockRef*);
+using GhostRecBFn = void (*)(unsigned, hipStream_t, const GhostArgs*, const BlockRef*, unsigned, const IbCase*,
+                             const BlockRef*);
 
 template <class LAT, class ST, int AM>
 static void launch_step_b(dim3 g, hipStream_t s, const StepArgs* a, const BlockRef* m) {
@@ -41,6 +43,25 @@ static void launch_fix_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef*
 template <class LAT, class ST>
 static void launch_fixw_b(dim3 g, hipStream_t s, const IbCase* a, const BlockRef* m) {
   hipLaunchKernelGGL((k_ib_spread_export_w_b<LAT, ST>), g, dim3(64 * kSpreadWaves), 0, s, a, m);
+}
+
+template <class LAT, class ST>
+static void launch_ghost_rec_b(unsigned grid, hipStream_t s, const GhostArgs* ga, const BlockRef* gm, unsigned nghost,
+                               const IbCase* a, const BlockRef* rm) {
+  hipLaunchKernelGGL((k_ghosts_record_b<LAT, ST>), dim3(grid), dim3(kSumsBlock), 0, s, ga, gm, nghost, a, rm);
+}
+
+// the fused slab + record launch of a class (lattice, storage)
+static GhostRecBFn pick_ghost_rec_b(int lat, int st) {
+  GhostRecBFn f = nullptr;
+  auto pk = [&](auto latt) {
+    using L = decltype(latt);
+    f = (st == 0) ? launch_ghost_rec_b<L, StF64> : (st == 1) ? launch_ghost_rec_b<L, StF32> : launch_ghost_rec_b<L, StF16S>;
+  };
+  if (lat == 0) pk(D2Q9{});
+  else if (lat == 1) pk(D3Q19{});
+  else pk(D3Q27{});
+  return f;
 }
 
 template <class LAT, class ST>
@@ -81,6 +102,7 @@ struct BatchGroup {
   int lane = 0;        // stream the group's step chain runs on (0: stream, 1: stream2)
   StepBFn step = nullptr;
   GhostBFn ghost = nullptr;
+  GhostRecBFn ghost_rec = nullptr;  // slabs and force records in one launch
   FixBFn fix = nullptr;
   FixBFn fixw = nullptr;
   std::vector<int> members;  // indices into Batch::cases
@@ -426,6 +448,7 @@ struct Batch {
         BatchGroup g;
         g.lat = L.lat; g.st = L.st; g.am = L.am; g.lane = lane_of[i];
         pick_class(g.lat, g.st, g.am, g.step, g.ghost, g.fix, g.fixw);
+        g.ghost_rec = pick_ghost_rec_b(g.lat, g.st);
         groups.push_back(g);
         gi = (int)groups.size() - 1;
       }
@@ -524,8 +547,9 @@ struct Batch {
   }
 
   // ---------------- one batched step ----------------
-  // The single Level's order: IB forcing with the export / Guo fix-up and
-  // the force record, ghost slabs, stream-collide, Sum(Rho) and the clock.
+  // The single Level's order: IB forcing with the export / Guo fix-up, the
+  // force records with the ghost slabs (one launch when the group has
+  // slabs), stream-collide, Sum(Rho) and the clock.
   // One step of a group on its lane's stream; lane0_ib: set once lane 0's
   // first IB kernels after a fork are queued (and ev_phase recorded)
   void enqueue_group(BatchGroup& G, int p, hipStream_t stream, bool& lane0_ib) {
@@ -559,14 +583,20 @@ struct Batch {
       if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
       if (G.mcnt[BatchGroup::M_FIXW] > 0)
         G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
-      hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(kSumsBlock), 0, stream, A,
-                         m + G.moff[BatchGroup::M_IBCASE]);
+      // with slabs to fill, the records go with them (one launch fewer)
+      if (G.mcnt[BatchGroup::M_GHOST] == 0)
+        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(kSumsBlock), 0, stream, A,
+                           m + G.moff[BatchGroup::M_IBCASE]);
       if (G.lane == 0 && phase_pending && !lane0_ib) {
         HIPCHK(hipEventRecord(ev_phase, stream));
         lane0_ib = true;
       }
     }
-    if (G.mcnt[BatchGroup::M_GHOST] > 0)
+    if (G.mcnt[BatchGroup::M_GHOST] > 0 && G.mcnt[BatchGroup::M_IBCASE] > 0)
+      G.ghost_rec((unsigned)(G.mcnt[BatchGroup::M_GHOST] + G.mcnt[BatchGroup::M_IBCASE]), stream, G.dghost[p],
+                  m + G.moff[BatchGroup::M_GHOST], (unsigned)G.mcnt[BatchGroup::M_GHOST], G.dib[p],
+                  m + G.moff[BatchGroup::M_IBCASE]);
+    else if (G.mcnt[BatchGroup::M_GHOST] > 0)
       G.ghost(grid(BatchGroup::M_GHOST), stream, G.dghost[p], m + G.moff[BatchGroup::M_GHOST]);
     G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
     hipLaunchKernelGGL(k_case_tail_b, grid(BatchGroup::M_TAIL), dim3(1024), 0, stream, G.dtail);

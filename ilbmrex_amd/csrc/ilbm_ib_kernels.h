@@ -1187,4 +1187,24 @@ __global__ void __launch_bounds__(kSumsBlock) k_force_record_b(const IbCase* __r
   else force_record_body(c.ib, c.rec, c.nblk_fix);
 }
 
+// The ghost slabs and the force records of a batched step in one launch:
+// blocks [0, nghost) fill the slab cells of their cases (kBlock each), the
+// rest write one case's record each.  Neither reads what the other writes,
+// and the stream-collide launch that follows needs both.
+static_assert(kBlock == kSumsBlock, "slab and record blocks share the launch's block size");
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kSumsBlock) k_ghosts_record_b(const GhostArgs* __restrict__ GA,
+                                                               const BlockRef* __restrict__ gmap, unsigned nghost,
+                                                               const IbCase* __restrict__ A,
+                                                               const BlockRef* __restrict__ rmap) {
+  if (blockIdx.x < nghost) {
+    const BlockRef r = gmap[blockIdx.x];
+    ghost_body<LAT, ST>(GA[r.c], (int64_t)r.b * kBlock + threadIdx.x);
+    return;
+  }
+  const IbCase& c = A[rmap[blockIdx.x - nghost].c];
+  if (c.ib.nloop == 1) sums_record_body(c.ib, c.rec, c.nblk_fix);
+  else force_record_body(c.ib, c.rec, c.nblk_fix);
+}
+
 }  // namespace ilbm
