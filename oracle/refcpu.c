@@ -31,6 +31,22 @@
 
 #define RC_MAXQ 27
 
+/* Planted mistakes for the tests that show each pin fails on a plausible
+ * error (rc_set_mutation); 0 = the restatement as written. */
+static int g_mut = 0;
+enum {
+  MUT_NONE = 0,
+  MUT_EXTDIR_SIGN = 1,    /* feq - (f_np - feq_np) in applyExtDirMesoscopic */
+  MUT_SPREAD_NO_DV = 2,   /* spreading without the marker volume dv */
+  MUT_EXPORT_NO_DT = 3,   /* export F = rho * EFT without dt */
+  MUT_INTERP_NO_DV = 4,   /* interpolation without the cell volume */
+  MUT_CD_NO_HALF = 5,     /* Cd normalised by rho U^2 D */
+  MUT_FORCE_SIGN = 6,     /* explicit forcing (U - U_target) / dt */
+  MUT_MOMENT_SIGN = 7,    /* moments F x r instead of r x F */
+  MUT_GUO_NO_HALF = 8     /* forced macros without the +F/2 */
+};
+void rc_set_mutation(int id) { g_mut = id; }
+
 /* ------------------------------------------------------------------ */
 /* Array4-like fab: one contiguous array per component, x fastest.      */
 /* ------------------------------------------------------------------ */
@@ -214,7 +230,7 @@ static void meso_user(const rc_level *L, fab *f, const int iv[3]) {
       double eu_b = cx[n] * u_b + cy[n] * v_b + cz[n] * w_b;
       double uv_b = u_b * u_b + v_b * v_b + w_b * w_b;
       double feq = w[n] * rho_b * (1.0 + 3.0 * eu_b + 4.5 * eu_b * eu_b - 1.5 * uv_b);
-      AT(*f, iv[0], iv[1], iv[2], n) = feq + (fnp[n] - feqnp);
+      AT(*f, iv[0], iv[1], iv[2], n) = (g_mut == MUT_EXTDIR_SIGN) ? feq - (fnp[n] - feqnp) : feq + (fnp[n] - feqnp);
     }
   } else { /* applyFaceMirroredMesoscopic :209-261 */
     const double rho_g = fmax(1.0e-20, 2.0 * rho_b - rhonp);
@@ -642,7 +658,7 @@ static void ib_interp(rc_level *L, fab *E) {
   const rc_case *c = &L->c;
   const int kt = rc_delta_type_to_kernel(c->delta_type);
   const int renorm = (c->renormalize_delta != 0) && (c->coupling_method != RC_COUPLING_IVC);
-  const double d = (c->dim == 3) ? c->dx[0] * c->dx[1] * c->dx[2] : c->dx[0] * c->dx[1];
+  const double d = (g_mut == MUT_INTERP_NO_DV) ? 1.0 : (c->dim == 3) ? c->dx[0] * c->dx[1] * c->dx[2] : c->dx[0] * c->dx[1];
   for (int m = 0; m < ib->nm; ++m) {
     int b[3];
     marker_base(L, m, b);
@@ -728,9 +744,10 @@ static void ib_force_explicit(rc_level *L, double dt, double time) {
       const double mkk = ib->dv * s2;
       denom = dt * fmax(mkk, diag_eps);
     }
-    ib->fx[m] += relax * (Ux - ib->u[m]) / denom;
-    ib->fy[m] += relax * (Uy - ib->v[m]) / denom;
-    ib->fz[m] += relax * (Uz - ib->w[m]) / denom;
+    const double sg = (g_mut == MUT_FORCE_SIGN) ? -1.0 : 1.0;
+    ib->fx[m] += relax * (sg * (Ux - ib->u[m])) / denom;
+    ib->fy[m] += relax * (sg * (Uy - ib->v[m])) / denom;
+    ib->fz[m] += relax * (sg * (Uz - ib->w[m])) / denom;
   }
 }
 
@@ -809,12 +826,14 @@ static void ib_spread(rc_level *L, fab *E) {
   const double d = (c->dim == 3) ? c->dx[0] * c->dx[1] * c->dx[2] : c->dx[0] * c->dx[1];
   const double z0 = (c->dim == 3) ? c->z0 : 0.0;
   for (int m = 0; m < ib->nm; ++m) {
-    const double fx = ib->fx[m] * ib->dv, fy = ib->fy[m] * ib->dv, fz = ib->fz[m] * ib->dv;
+    const double dvm = (g_mut == MUT_SPREAD_NO_DV) ? 1.0 : ib->dv;
+    const double fx = ib->fx[m] * dvm, fy = ib->fy[m] * dvm, fz = ib->fz[m] * dvm;
     const double rx = ib->x[m] - c->x0, ry = ib->y[m] - c->y0;
     const double rz = (c->dim == 3) ? ib->z[m] - z0 : 0.0;
-    ib->mx[m] = ry * fz - rz * fy;
-    ib->my[m] = rz * fx - rx * fz;
-    ib->mz[m] = rx * fy - ry * fx;
+    const double ms = (g_mut == MUT_MOMENT_SIGN) ? -1.0 : 1.0;
+    ib->mx[m] = ms * (ry * fz - rz * fy);
+    ib->my[m] = ms * (rz * fx - rx * fz);
+    ib->mz[m] = ms * (rx * fy - ry * fx);
     int b[3];
     marker_base(L, m, b);
     double tu[5], tv[5], tw[5];
@@ -909,7 +928,7 @@ static void ib_update_forcing(rc_level *L, const fab *smS, fab *F, double dt, do
     for (int j = L->vlo[1]; j <= L->vhi[1]; ++j)
       for (int i = L->vlo[0]; i <= L->vhi[0]; ++i)
         for (int n = 0; n < 3; ++n)
-          AT(*F, i, j, k, n) = dt * AT(*smS, i, j, k, 0) * AT(EFT, i, j, k, n);
+          AT(*F, i, j, k, n) = ((g_mut == MUT_EXPORT_NO_DT) ? 1.0 : dt) * AT(*smS, i, j, k, 0) * AT(EFT, i, j, k, n);
   fill_boundary(L, F, 3);
   fab_free(&EV); fab_free(&EFI); fab_free(&EFT);
 }
@@ -1081,10 +1100,11 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
           m2 += c->cy[q] * fq;
           m3 += c->cz[q] * fq;
         }
+        const double hf = (g_mut == MUT_GUO_NO_HALF) ? 0.0 : 0.5;
         AT(L->m_new, i, j, k, 0) = r;
-        AT(L->m_new, i, j, k, 1) = (m1 + 0.5 * AT(fx_cc, i, j, k, 0)) / r;
-        AT(L->m_new, i, j, k, 2) = (m2 + 0.5 * AT(fx_cc, i, j, k, 1)) / r;
-        AT(L->m_new, i, j, k, 3) = (m3 + 0.5 * AT(fx_cc, i, j, k, 2)) / r;
+        AT(L->m_new, i, j, k, 1) = (m1 + hf * AT(fx_cc, i, j, k, 0)) / r;
+        AT(L->m_new, i, j, k, 2) = (m2 + hf * AT(fx_cc, i, j, k, 1)) / r;
+        AT(L->m_new, i, j, k, 3) = (m3 + hf * AT(fx_cc, i, j, k, 2)) / r;
       }
   fab_free(&fx_cc); fab_free(&sfS); fab_free(&smS);
 }
@@ -1123,8 +1143,9 @@ static void compute_ib_force(rc_level *L, double time, int step) {
   L->prev_body[0] = Fxb; L->prev_body[1] = Fyb; L->prev_body[2] = Fzb;
   L->have_prev = 1;
   const double rho_ref = 1.0, U = c->U0, D = 2.0 * c->R;
-  const double denom = (c->dim == 3) ? 0.5 * rho_ref * U * U * (rc_pi() * c->R * c->R)
-                                     : 0.5 * rho_ref * U * U * D;
+  const double half = (g_mut == MUT_CD_NO_HALF) ? 1.0 : 0.5;
+  const double denom = (c->dim == 3) ? half * rho_ref * U * U * (rc_pi() * c->R * c->R)
+                                     : half * rho_ref * U * U * D;
   double Cd = 0, Cl = 0, Cdr = 0, Clr = 0, Cde = 0, Cle = 0, Cdm = 0, Clm = 0, Cdx = 0, Clx = 0;
   double Cz = 0, Czr = 0, Cze = 0, Czm = 0, Czx = 0;
   if (denom > 0.0) {
@@ -1223,6 +1244,52 @@ int rc_build_markers(const rc_case *c, double time, double *xyz, int maxn, doubl
   if (dv_out) *dv_out = tmp.ib.dv;
   rc_ib_free(&tmp.ib);
   return nm;
+}
+
+/* Stage probes of update_forcing for the unit checks of the IB
+ * restatement.  Both place the markers at `time` first
+ * (update_lagrangian_marker, IBMarkerDF.cpp:499-609). */
+static fab probe_fab(const rc_level *L) {
+  int lo[3], hi[3];
+  for (int d = 0; d < 3; ++d) { lo[d] = L->vlo[d]; hi[d] = L->vhi[d]; }
+  for (int d = 0; d < L->c.dim; ++d) { lo[d] -= 2; hi[d] += 2; }
+  return fab_alloc(lo, hi, 3);
+}
+
+/* velocity_interpolation (:611-739) of a caller-given EulerVel over the
+ * valid box grown by 2 cells ([3][grown cells], x fastest); periodic ghost
+ * images are refilled from the valid region as FillBoundary does.
+ * U_out [nm][3]; returns the marker count. */
+int rc_ib_interp_probe(rc_level *L, double time, const double *ev, double *U_out) {
+  if (!L->c.use_ib) return -1;
+  fab E = probe_fab(L);
+  memcpy(E.p, ev, sizeof(double) * (size_t)(3 * E.ns));
+  ib_update_markers(L, time);
+  ib_interp(L, &E);
+  for (int m = 0; m < L->ib.nm; ++m) {
+    U_out[3 * m] = L->ib.u[m]; U_out[3 * m + 1] = L->ib.v[m]; U_out[3 * m + 2] = L->ib.w[m];
+  }
+  fab_free(&E);
+  return L->ib.nm;
+}
+
+/* force_spreading (:1330-1512) of caller-given marker forces F [nm][3]:
+ * EulerForce on the valid box after SumBoundary ([3][valid cells]) and the
+ * marker moments r x (F dv) [nm][3]. */
+int rc_ib_spread_probe(rc_level *L, double time, const double *F, double *efi_out, double *mom_out) {
+  if (!L->c.use_ib) return -1;
+  fab E = probe_fab(L);
+  ib_update_markers(L, time);
+  for (int m = 0; m < L->ib.nm; ++m) {
+    L->ib.fx[m] = F[3 * m]; L->ib.fy[m] = F[3 * m + 1]; L->ib.fz[m] = F[3 * m + 2];
+  }
+  ib_spread(L, &E);
+  get_valid(L, &E, 3, efi_out);
+  for (int m = 0; m < L->ib.nm; ++m) {
+    mom_out[3 * m] = L->ib.mx[m]; mom_out[3 * m + 1] = L->ib.my[m]; mom_out[3 * m + 2] = L->ib.mz[m];
+  }
+  fab_free(&E);
+  return L->ib.nm;
 }
 
 /* ------------------------------------------------------------------ */

@@ -164,6 +164,20 @@ void rc_cons_interp_test(const double *crse, const int *clo, const int *chi, int
 int rc_build_markers(const rc_case *c, double time, double *xyz, int maxn,
                      double *dv_out);
 
+/* Stage probes of IBMarkerDF::update_forcing (markers placed at `time`):
+ * interpolation of a caller-given EulerVel ([3][valid box grown by 2]) into
+ * marker U [nm][3], and spreading of caller-given marker forces [nm][3] into
+ * EulerForce [3][valid cells] with the moments r x (F dv) [nm][3]. */
+int rc_ib_interp_probe(rc_level *L, double time, const double *ev, double *U_out);
+int rc_ib_spread_probe(rc_level *L, double time, const double *F, double *efi_out, double *mom_out);
+
+/* Planted mistakes, for tests that show a pin fails on a plausible error:
+ * 1 ext_dir non-equilibrium sign, 2 spreading without dv, 3 export without
+ * dt, 4 interpolation without the cell volume, 5 Cd without the 1/2,
+ * 6 explicit forcing sign, 7 moment sign, 8 forced macros without F/2.
+ * 0 restores the restatement.  Process-global. */
+void rc_set_mutation(int id);
+
 #ifdef __cplusplus
 }
 #endif

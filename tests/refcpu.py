@@ -111,6 +111,9 @@ def lib():
         L.rc_get_macro_all.argtypes = [C.c_void_p, P(C.c_double)]
         L.rc_cons_interp_test.argtypes = [P(C.c_double), P(C.c_int), P(C.c_int), C.c_int, C.c_int,
                                           P(C.c_int), P(C.c_double)]
+        L.rc_ib_interp_probe.argtypes = [C.c_void_p, C.c_double, P(C.c_double), P(C.c_double)]
+        L.rc_ib_spread_probe.argtypes = [C.c_void_p, C.c_double, P(C.c_double), P(C.c_double), P(C.c_double)]
+        L.rc_set_mutation.argtypes = [C.c_int]
         _lib = L
     return _lib
 
@@ -224,6 +227,33 @@ class Oracle:
     def tau(self) -> float:
         return lib().rc_tau(self._h)
 
+    def grown_shape(self):
+        """(nz, ny, nx) of the valid box grown by the 2 ghost cells the IB
+        stages read."""
+        n = self.case.n
+        g = [2, 2, 2 if self.case.lattice.dim == 3 else 0]
+        return (n[2] + 2 * g[2], n[1] + 2 * g[1], n[0] + 2 * g[0])
+
+    def interp_probe(self, time: float, ev: np.ndarray) -> np.ndarray:
+        """Marker U [nm][3] interpolated from EulerVel ev [3][nz][ny][nx]
+        over the valid box grown by 2 (rc_ib_interp_probe)."""
+        ev = np.ascontiguousarray(ev, dtype=np.float64)
+        assert ev.shape == (3,) + self.grown_shape()
+        out = np.zeros(3 * self.num_markers())
+        lib().rc_ib_interp_probe(self._h, float(time), dptr(ev.reshape(-1)), dptr(out))
+        return out.reshape(-1, 3)
+
+    def spread_probe(self, time: float, F: np.ndarray):
+        """EulerForce [3][cells] and moments [nm][3] of marker forces F [nm][3]
+        (rc_ib_spread_probe)."""
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        nm = self.num_markers()
+        assert F.shape == (nm, 3)
+        efi = np.zeros(3 * self.case.ncells)
+        mom = np.zeros(3 * nm)
+        lib().rc_ib_spread_probe(self._h, float(time), dptr(F.reshape(-1)), dptr(efi), dptr(mom))
+        return efi.reshape(3, -1), mom.reshape(nm, 3)
+
 
 class AmrOracle:
     """FP64 CPU AMR hierarchy: level 0 = the case's domain, level l >= 1 one
@@ -333,6 +363,23 @@ def cons_interp(crse: np.ndarray, clo, dim: int, fiv) -> np.ndarray:
     out = np.zeros(nc)
     lib().rc_cons_interp_test(dptr(crse.reshape(-1)), lo, hi, dim, nc, f, dptr(out))
     return out
+
+
+class mutation:
+    """Context manager planting one of the oracle's test mistakes
+    (rc_set_mutation); the restatement is restored on exit."""
+    EXTDIR_SIGN, SPREAD_NO_DV, EXPORT_NO_DT, INTERP_NO_DV = 1, 2, 3, 4
+    CD_NO_HALF, FORCE_SIGN, MOMENT_SIGN, GUO_NO_HALF = 5, 6, 7, 8
+
+    def __init__(self, which: int):
+        self.which = int(which)
+
+    def __enter__(self):
+        lib().rc_set_mutation(self.which)
+        return self
+
+    def __exit__(self, *a):
+        lib().rc_set_mutation(0)
 
 
 def build_markers(case, time: float = 0.0):
