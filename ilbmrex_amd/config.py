@@ -293,6 +293,13 @@ def case_from_inputs(text: str, overrides: Sequence[str] = (), storage: str = "f
         ivc_rebuild_matrix=int(_get(pp, "ibm.ivc_rebuild_matrix", 0, int)),
         renormalize_delta=renorm,
         force_eval_method=fem,
+        # ibm.box_* (AmrCoreLBM.cpp:1317-1328)
+        box_lo=(_get(pp, "ibm.box_xlo", 0.0), _get(pp, "ibm.box_ylo", 0.0), _get(pp, "ibm.box_zlo", 0.0)),
+        box_hi=(_get(pp, "ibm.box_xhi", 0.0), _get(pp, "ibm.box_yhi", 0.0), _get(pp, "ibm.box_zhi", 0.0)),
+        box_ds=_get(pp, "ibm.box_ds", 0.0),
+        box_lid_only=int(_get(pp, "ibm.box_lid_only", 0, int)),
+        box_lid_u=(_get(pp, "ibm.box_lid_ux", 0.0), _get(pp, "ibm.box_lid_uy", 0.0), _get(pp, "ibm.box_lid_uz", 0.0)),
+        box_wall_tol=_get(pp, "ibm.box_wall_tol", 0.0),
         storage=STORAGE_NAMES[storage],
     )
     c.validate()
@@ -354,6 +361,33 @@ def sphere_3d(nx=256, ny=128, nz=128, D=24.0, center=(64.0, 64.0, 64.0), U=0.04,
                 coupling_method=coupling, loop_ns=loop_ns, delta_type=delta_type,
                 ivc_diag_reg=1.3, force_eval_method=FORCE_EVAL_ME, storage=storage,
                 name="sphere3d")
+
+
+def couette_channel(nx=4, ny=64, nu=0.064, U=0.1, storage: int = STORE_F64) -> Case:
+    """Plane Couette flow: ext_dir walls at y-lo (at rest) and y-hi
+    (ux = U), the wall conditions of Examples/Couette_flow/inputs, with x
+    periodic."""
+    z = (0.0,) * 4
+    return Case(lattice=D2Q9, n=(nx, ny, 1), nu=nu, U0=U, periodic=(1, 0, 1),
+                bc_lo=(INT_DIR, EXT_DIR, INT_DIR), bc_hi=(INT_DIR, EXT_DIR, INT_DIR),
+                bc_lo_val=(z, (1.0, 0.0, 0.0, 0.0), z), bc_hi_val=(z, (1.0, U, 0.0, 0.0), z),
+                storage=storage, name="couette")
+
+
+def cavity_ibm_box(lid_only: int = 0, coupling: int = COUPLING_IVC, storage: int = STORE_F64) -> Case:
+    """Examples/Cavity_flow/inputs_ibm_box: 64x96 D2Q9 with ext_dir walls at
+    rest, nu = 0.02, an immersed box of markers over [0, 64]^2 (clipped
+    inside the domain, spacing 1) whose top side moves at ux = 0.05, IVC
+    with ivc_diag_reg = 20 and one forcing iteration."""
+    w = (1.0, 0.0, 0.0, 0.0)
+    return Case(lattice=D2Q9, n=(64, 96, 1), nu=0.02, U0=0.1, periodic=(0, 0, 1),
+                bc_lo=(EXT_DIR, EXT_DIR, INT_DIR), bc_hi=(EXT_DIR, EXT_DIR, INT_DIR),
+                bc_lo_val=(w, w, (0.0,) * 4), bc_hi_val=(w, w, (0.0,) * 4),
+                use_ib=True, x0=32.0, y0=32.0, R=8.0, coupling_method=coupling, loop_ns=1,
+                delta_type=4, ivc_diag_reg=20.0, geometry_type=GEOM_BOX,
+                box_lo=(0.0, 0.0, 0.0), box_hi=(64.0, 64.0, 0.0), box_ds=1.0, box_lid_only=lid_only,
+                box_lid_u=(0.05, 0.0, 0.0), force_eval_method=FORCE_EVAL_ME, storage=storage,
+                name="cavity_ibm_box")
 
 
 def uniform_init(case: Case, rho: float = 1.0, u=(0.0, 0.0, 0.0)):

@@ -1,0 +1,145 @@
+"""The benchmark workloads at their stated sizes on the GPU.
+
+* Config 1: D2Q9 cylinder 512x256, D = 32 at (128, 128), 101 markers, FP64:
+  bit-exact against the oracle for 50 steps (explicit), IVC within 1e-10.
+* Config 2: D3Q19 sphere 256x128x128, D = 24, 1810 markers: FP64 bit-exact
+  against the oracle for 2 steps; FP32 and FP16-shifted storage against
+  the FP64 run over 1000 steps (velocity field, Cd, Cl).
+* Config 3: the pitching NACA 0012 hierarchy (coarse 512x256, chord 64
+  finest cells, 256 markers, regrid every 10 coarse steps) in FP64 against
+  the oracle's hierarchy, bit-exact through two regrids.
+* Config 4: the seeded 256-case ensemble, 200 steps in one batch; a sample
+  of cases equals the same cases run alone, bit for bit.
+* Config 5: serve() on a seeded Poisson stream of mixed cylinder / sphere
+  requests; every response equals the same request run alone.
+"""
+import numpy as np
+import pytest
+
+from ilbmrex_amd import config as cf
+import refcpu
+from test_gpu_parity import _ib_checks, assert_bitexact, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def config1(**kw):
+    return cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0, **kw)
+
+
+def test_config1_cylinder_bitexact_50_steps():
+    c = config1()
+    lev, orc = run_pair(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)), 50)
+    assert lev.num_markers() == orc.num_markers() == 101
+    _ib_checks(lev, orc)
+
+
+def test_config1_cylinder_ivc_within_tolerance():
+    c = config1(coupling=cf.COUPLING_IVC)
+    lev, orc = run_pair(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)), 20)
+    f, fo = lev.f(), orc.f()
+    assert np.abs(f - fo).max() <= 1e-12 * np.abs(fo).max()
+    _, _, fx = lev.markers()
+    _, _, ofx = orc.markers()
+    assert np.abs(fx - ofx).max() <= 1e-10 * np.abs(ofx).max()
+    assert abs(lev.force_history()[-1, 4] - orc.force_record()[4]) <= 1e-10 * abs(orc.force_record()[4])
+
+
+def config2(storage):
+    return cf.sphere_3d(nx=256, ny=128, nz=128, D=24.0, center=(64.0, 64.0, 64.0), U=0.04, Re=100.0,
+                        storage=storage)
+
+
+def test_config2_sphere_fp64_bitexact_2_steps():
+    c = config2(cf.STORE_F64)
+    lev, orc = run_pair(c, cf.uniform_init(c, u=(0.04, 0.0, 0.0)), 2)
+    assert lev.num_markers() == orc.num_markers() == 1810
+    _ib_checks(lev, orc)
+
+
+def test_config2_reduced_storage_against_fp64():
+    """1000 steps from rest-of-body uniform flow in each storage; tolerances:
+    velocity field relative L2 below 5e-4 (FP32) and 5e-3 (FP16-shifted),
+    mean Cd of the last 200 steps within 0.5 % of FP64."""
+    from ilbmrex_amd.native import Level
+    n = 1000
+    res = {}
+    for st in (cf.STORE_F64, cf.STORE_F32, cf.STORE_F16S):
+        c = config2(st)
+        print(f"[config2] storage {st}, {n} steps", flush=True)
+        lev = Level(c, cf.uniform_init(c, u=(0.04, 0.0, 0.0)))
+        lev.advance(n, want_state=True)
+        res[st] = (lev.macro()[1:4].copy(), lev.force_history())
+        lev.close()
+    u64, f64 = res[cf.STORE_F64]
+    cd64 = f64[-200:, 4].mean()
+    for st, vtol in ((cf.STORE_F32, 5e-4), (cf.STORE_F16S, 5e-3)):
+        u, fh = res[st]
+        verr = np.linalg.norm(u - u64) / np.linalg.norm(u64)
+        cderr = abs(fh[-200:, 4].mean() - cd64) / abs(cd64)
+        print(f"storage {st}: velocity rel L2 {verr:.3e}, mean Cd rel {cderr:.3e}")
+        assert np.isfinite(fh).all()
+        assert verr < vtol
+        assert cderr < 5e-3
+
+
+def test_config3_pitching_hierarchy_regrid_bitexact():
+    """Full-size three-level hierarchy in FP64 at Re = 100 (Re = 1000
+    diverges under BGK at this resolution, the oracle included), regrid
+    every 10 coarse steps with vorticity tagging: every level bit-exact
+    against the oracle after 21 coarse steps (two regrids)."""
+    from ilbmrex_amd.native import AmrHierarchy
+    c, boxes = cf.pitching_naca_amr(storage=cf.STORE_F64, Re=100.0)
+    regrid = dict(threshold_ratio=(0.85, 0.90, 0.90), blocking_factor=8)
+    gpu = AmrHierarchy(c, boxes, 10, (1.0, c.U0, 0.0, 0.0))
+    orc = refcpu.AmrOracle(c, boxes, 10, cf.uniform_init(c, u=(c.U0, 0.0, 0.0)))
+    gpu.set_regrid(**regrid)
+    orc.set_regrid(**regrid)
+    gpu.advance(21, want_state=True)
+    orc.advance(21)
+    gb, n = gpu.boxes()
+    assert gb == orc.boxes and n == orc.regrids()
+    for l in range(3):
+        assert_bitexact(gpu.f(l), orc.f(l), f"level {l} f")
+        assert_bitexact(gpu.macro(l), orc.macro(l), f"level {l} macro")
+    fh, rec = gpu.force_history(), orc.force_record()
+    floor = 1e-10 * np.abs(rec).max()
+    assert np.all(np.abs(fh[-1] - rec) <= 1e-12 * np.abs(rec) + floor)
+
+
+def test_config4_ensemble_matches_cases_alone():
+    from ilbmrex_amd import serving
+    from ilbmrex_amd.native import Batch, Level
+    cases = serving.ensemble_cases(256, seed=20261202)
+    n = 200
+    print(f"[config4] 256 cases, {n} steps", flush=True)
+    with Batch(64 << 30, max_cases=300) as B:
+        ids = [B.add(c, serving.initial_state(c), n) for c in cases]
+        done = 0
+        while done < n:
+            done += B.advance(n - done)
+        sample = list(range(0, 256, 37))
+        hist = {k: B.force_history(ids[k]) for k in sample}
+        mass = {k: B.mass_history(ids[k]) for k in sample}
+    for k in sample:
+        lev = Level(cases[k], serving.initial_state(cases[k]))
+        lev.advance(n)
+        assert_bitexact(hist[k], lev.force_history(), f"case {k} force history")
+        mh = lev.mass_history()
+        assert np.all(np.abs(mass[k] - mh) <= 1e-12 * np.abs(mh))
+        lev.close()
+
+
+def test_config5_serving_responses_match_requests_alone():
+    from ilbmrex_amd import serving
+    from ilbmrex_amd.native import Level
+    reqs = serving.serving_requests(12, rate=50.0, seed=11, steps=(200, 800), cyl_n=(64, 512),
+                                    sph_n=(32, 96))
+    assert {r.case.lattice.name for r in reqs} == {"D2Q9", "D3Q19"}
+    st = serving.serve(reqs, 16 << 30, chunk=100, keep_history=True)
+    assert st["requests"] == 12
+    for r in reqs:
+        lev = Level(r.case, serving.initial_state(r.case))
+        lev.advance(r.nsteps)
+        assert_bitexact(r.history, lev.force_history(), f"request {r.rid}")
+        lev.close()
