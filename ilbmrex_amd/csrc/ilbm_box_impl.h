@@ -640,3 +640,79 @@ int ilbm_box_sum(const ilbm_fab* fab, int comp, const int vlo[3], const int vhi[
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Allocation-free box sum.  The block partials and the result live in a
+// file-scope device array (one per GPU, held until the process ends), so
+// the call allocates nothing; its kernels are ilbm_box_sum's, so the two
+// return the same bits.  Calls that overlap on one GPU share the array and
+// race; every call rewrites every slot it reads, so nothing needs clearing.
+// ---------------------------------------------------------------------------
+namespace ilbm {
+constexpr int64_t kNoAllocPartials = 131072;                    // 1 MiB of partials
+constexpr int64_t kNoAllocMaxCells = (kNoAllocPartials - 1) * 256;
+}  // namespace ilbm
+
+static __device__ __align__(256) double scratch[ilbm::kNoAllocPartials];
+
+namespace ilbm {
+constexpr int kNoAllocMaxDevices = 64;
+static double* g_scratch_ptr[kNoAllocMaxDevices] = {};
+static std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device, looked up once per
+// device with that device current (each GPU holds its own copy).
+static double* scratch_on_current_device() {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kNoAllocMaxDevices) throw StatusError{ILBM_ERR_UNSUPPORTED, "device index"};
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    HIPCHK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw StatusError{ILBM_ERR_HIP, "hipGetSymbolAddress(scratch) returned null"};
+    g_scratch_ptr[dev] = static_cast<double*>(ptr);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace ilbm
+
+extern "C" {
+
+int ilbm_box_sum_noalloc(const ilbm_fab* fab, int comp, const int vlo[3], const int vhi[3], double* host_out,
+                         void* stream) {
+  if (!fab || !fab->p || !vlo || !vhi || !host_out || comp < 0 || comp >= fab->ncomp) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    BoxGeom G{};
+    G.dim = 3;
+    for (int dd = 0; dd < 3; ++dd) {
+      if (vlo[dd] > vhi[dd]) throw StatusError{ILBM_ERR_INVALID, "empty box"};
+      G.vlo[dd] = vlo[dd];
+      G.vhi[dd] = vhi[dd];
+    }
+    const Fab F = to_fab(*fab);
+    if (!F.contains(vlo[0], vlo[1], vlo[2]) || !F.contains(vhi[0], vhi[1], vhi[2]))
+      throw StatusError{ILBM_ERR_INVALID, "box outside the fab"};
+    const int64_t cells = box_cells(G);
+    if (cells > kNoAllocMaxCells)
+      throw StatusError{ILBM_ERR_INVALID, "box larger than the allocation-free sum's limit of " +
+                                              std::to_string(kNoAllocMaxCells) + " cells"};
+    const int64_t nb = (cells + 255) / 256;
+    double* part = scratch_on_current_device();
+    hipStream_t s = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_box_sum, dim3((unsigned)nb), dim3(256), 0, s, F, comp, G, part);
+    hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(1024), 0, s, (const double*)part, nb, part + nb);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(host_out, part + nb, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+  });
+}
+
+int ilbm_box_sum_noalloc_max_cells(int64_t* cells) {
+  if (!cells) return ILBM_ERR_INVALID;
+  *cells = ilbm::kNoAllocMaxCells;
+  return ILBM_OK;
+}
+
+}  // extern "C"

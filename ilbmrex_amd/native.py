@@ -68,7 +68,7 @@ EXPORTED = [
     "ilbm_level_checkpoint_size", "ilbm_level_checkpoint", "ilbm_level_restore", "ilbm_level_get_derived",
     "ilbm_batch_checkpoint_size", "ilbm_batch_checkpoint", "ilbm_batch_add_from_checkpoint",
     "ilbm_box_copy", "ilbm_box_setval", "ilbm_device_alloc", "ilbm_device_free", "ilbm_copy_to_device",
-    "ilbm_copy_to_host", "ilbm_device_synchronize",
+    "ilbm_copy_to_host", "ilbm_device_synchronize", "ilbm_box_sum_noalloc", "ilbm_box_sum_noalloc_max_cells",
 ]
 
 ERR_FULL = 6
@@ -154,6 +154,8 @@ def load():
     L.ilbm_box_stream_macro.argtypes = [P(CaseDesc), PF, PF, PF, PF, P(C.c_int), P(C.c_int), vp]
     L.ilbm_box_derived.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), vp]
     L.ilbm_box_sum.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
+    L.ilbm_box_sum_noalloc.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
+    L.ilbm_box_sum_noalloc_max_cells.argtypes = [P(C.c_int64)]
     L.ilbm_box_copy.argtypes = [PF, PF, P(C.c_int), P(C.c_int), C.c_int, vp]
     L.ilbm_box_setval.argtypes = [PF, C.c_double, vp]
     L.ilbm_device_alloc.argtypes = [C.c_int64, P(vp)]

@@ -327,6 +327,15 @@ int ilbm_device_synchronize(void);
  * the force integral of ComputeIBForce), fixed order; synchronizes */
 int ilbm_box_sum(const ilbm_fab* fab, int comp, const int lo[3], const int hi[3], double* host_out,
                  void* stream);
+/* ilbm_box_sum without any allocation: no counterpart in the reference; for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and
+ * bits as ilbm_box_sum; the partials live in a fixed 1 MiB device array per
+ * GPU, so boxes above ilbm_box_sum_noalloc_max_cells cells (33,554,176) are
+ * rejected with ILBM_ERR_INVALID, and calls that overlap on one GPU race. */
+int ilbm_box_sum_noalloc(const ilbm_fab* fab, int comp, const int lo[3], const int hi[3], double* host_out,
+                         void* stream);
+int ilbm_box_sum_noalloc_max_cells(int64_t* cells);
 
 const char* ilbm_status_string(int status);
 /* Text of the last error raised on the calling thread. */
