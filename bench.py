@@ -275,6 +275,9 @@ def main():
     OVERLAP = not args.no_overlap
     if args.workload == "ensemble":
         from ilbmrex_amd import serving
+        if args.probe:
+            ms = native.load_probe(1 << 30, 3)
+            log(f"[bench] copy probe 1 GiB: {ms:.3f} ms ({2 * (1 << 30) / (ms * 1e-3) / 1e9:.0f} GB/s)")
         cases = serving.ensemble_cases(256, seed=20261202)
         wall, lups, _, info = serving.run_ensemble(cases, args.ensemble_steps, 100 << 30)
         print(json.dumps({"workload": "ensemble of 256 D2Q9 cylinder cases, FP32", "steps": args.ensemble_steps,

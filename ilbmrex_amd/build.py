@@ -13,7 +13,7 @@ OUT = os.path.join(PKG_DIR, "_lib", "libilbm_hip.so")
 SOURCES = ["ilbm_level.hip"]
 # headers the one translation unit includes
 HEADERS = ["ilbm_lattice.h", "ilbm_lbm_kernels.h", "ilbm_ib_kernels.h", "ilbm_amr_kernels.h",
-           "ilbm_batch_impl.h", "ilbm_box_impl.h", "ilbm_amr_impl.h"]
+           "ilbm_batch_impl.h", "ilbm_box_impl.h", "ilbm_box_ib_impl.h", "ilbm_amr_impl.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",

@@ -339,7 +339,7 @@ struct Amr {
     Fab F{};
     F.p = P.p;
     for (int d = 0; d < 3; ++d) { F.lo[d] = P.lo[d]; F.hi[d] = P.hi[d]; }
-    F.js = P.js; F.ks = P.ks; F.ns = P.ns; F.nc = 4;
+    F.js = P.js; F.ks = P.ks; F.ns = P.ns; F.cells = P.ns; F.nc = 4;
     const int64_t cells = V.ncells;
     hipLaunchKernelGGL(k_fab_from_cells, grid_for(cells), dim3(256), 0, stream, F, (const double*)V.mout, V.c.n[0],
                        V.c.n[1], cells);

@@ -416,6 +416,10 @@ struct Batch {
     for (const BatchCase& bc : cases)
       if (bc.done < bc.target) active_cells += bc.L->ncells;
     nlanes = (active_cells >= kTwoLaneCells) ? 2 : 1;
+    // ILBM_BATCH_LANES=1 keeps every case on one lane (measurement of the
+    // lane split, and graph replays of large batches)
+    if (const char* nl = std::getenv("ILBM_BATCH_LANES"))
+      if (nl[0] == '1') nlanes = 1;
     if (nlanes == 2) {
       std::map<int64_t, std::vector<int>> cls;
       for (size_t i = 0; i < cases.size(); ++i) {

@@ -11,12 +11,15 @@
 // reference's FP64 results.
 #pragma once
 
+#include "ilbm_amr_kernels.h"
+
 namespace ilbm {
 
 struct Fab {
   double* p;
   int lo[3], hi[3];
-  int64_t js, ks, ns;
+  int64_t js, ks, ns;    // element strides
+  int64_t cells;         // cells of the box
   int nc;
   __host__ __device__ double& at(int i, int j, int k, int n) const {
     return p[(int64_t)(i - lo[0]) + (int64_t)(j - lo[1]) * js + (int64_t)(k - lo[2]) * ks + (int64_t)n * ns];
@@ -26,13 +29,25 @@ struct Fab {
   }
 };
 
+// Descriptor -> Fab.  Strides of 0 are the dense Array4 strides; explicit
+// strides must keep x fastest and the components apart (no two cells or
+// components of the box share an element).
 static Fab to_fab(const ilbm_fab& f) {
+  if (f.dtype != ILBM_FAB_F64) throw StatusError{ILBM_ERR_UNSUPPORTED, "per-box fabs are FP64 (dtype ILBM_FAB_F64)"};
   Fab F{};
-  F.p = f.p;
-  for (int d = 0; d < 3; ++d) { F.lo[d] = f.lo[d]; F.hi[d] = f.hi[d]; }
-  F.js = (int64_t)(f.hi[0] - f.lo[0] + 1);
-  F.ks = F.js * (int64_t)(f.hi[1] - f.lo[1] + 1);
-  F.ns = F.ks * (int64_t)(f.hi[2] - f.lo[2] + 1);
+  F.p = static_cast<double*>(f.p);
+  for (int d = 0; d < 3; ++d) {
+    F.lo[d] = f.lo[d];
+    F.hi[d] = f.hi[d];
+    if (f.hi[d] < f.lo[d]) throw StatusError{ILBM_ERR_INVALID, "fab box is empty"};
+  }
+  const int64_t nx = f.hi[0] - f.lo[0] + 1, ny = f.hi[1] - f.lo[1] + 1, nz = f.hi[2] - f.lo[2] + 1;
+  F.js = f.jstride ? f.jstride : nx;
+  F.ks = f.kstride ? f.kstride : F.js * ny;
+  F.ns = f.nstride ? f.nstride : F.ks * nz;
+  if (F.js < nx || F.ks < F.js * ny || (f.ncomp > 1 && F.ns < F.ks * nz))
+    throw StatusError{ILBM_ERR_INVALID, "fab strides overlap cells or components"};
+  F.cells = nx * ny * nz;
   F.nc = f.ncomp;
   return F;
 }
@@ -53,11 +68,12 @@ __device__ inline int nonper_out_cnt(const BoxGeom& G, const int iv[3]) {
   return c;
 }
 
-// cell of the fab at a linear index
+// cell of the fab box at a linear index (x fastest)
 __device__ inline void fab_cell(const Fab& F, int64_t t, int iv[3]) {
-  iv[0] = F.lo[0] + (int)(t % F.js);
-  iv[1] = F.lo[1] + (int)((t / F.js) % (F.hi[1] - F.lo[1] + 1));
-  iv[2] = F.lo[2] + (int)(t / F.ks);
+  const int nx = F.hi[0] - F.lo[0] + 1, ny = F.hi[1] - F.lo[1] + 1;
+  iv[0] = F.lo[0] + (int)(t % nx);
+  iv[1] = F.lo[1] + (int)((t / nx) % ny);
+  iv[2] = F.lo[2] + (int)(t / ((int64_t)nx * ny));
 }
 
 // InitEquilibrium (AmrCoreLBM.cpp:2146-2161) over a box
@@ -80,7 +96,7 @@ __global__ void k_box_equilibrium(Fab m, Fab f, int lo0, int lo1, int lo2, int n
 // valid box take its value (AMReX FillBoundary with the periodicity)
 __global__ void k_box_fill_periodic(Fab F, BoxGeom G) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= F.ns) return;
+  if (t >= F.cells) return;
   int iv[3];
   fab_cell(F, t, iv);
   bool valid = true, perout = false;
@@ -110,7 +126,7 @@ __global__ void k_box_fill_periodic(Fab F, BoxGeom G) {
 template <class LAT>
 __global__ void k_box_fill_physical(Fab F, BoxGeom G, int pass, int meso) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= F.ns) return;
+  if (t >= F.cells) return;
   int iv[3];
   fab_cell(F, t, iv);
   if (nonper_out_cnt(G, iv) != pass) return;
@@ -221,7 +237,7 @@ __global__ void k_box_collide(Fab f, Fab m, Fab F, double tau, BoxGeom G) {
 template <class LAT>
 __global__ void k_box_bounce_back(Fab f, BoxGeom G) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= f.ns) return;
+  if (t >= f.cells) return;
   int iv[3];
   fab_cell(f, t, iv);
   bool valid = true;
@@ -388,8 +404,10 @@ __global__ void k_box_copy(Fab S, Fab D, BoxGeom G, int ncomp) {
 
 __global__ void k_fab_setval(Fab F, double v) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= F.ns * F.nc) return;
-  F.p[t] = v;
+  if (t >= F.cells * F.nc) return;
+  int iv[3];
+  fab_cell(F, t % F.cells, iv);
+  F.at(iv[0], iv[1], iv[2], (int)(t / F.cells)) = v;
 }
 
 }  // namespace ilbm
@@ -420,7 +438,7 @@ int ilbm_box_setval(ilbm_fab* f, double value, void* stream) {
   return guarded([&] {
     using namespace ilbm;
     const Fab F = to_fab(*f);
-    hipLaunchKernelGGL(k_fab_setval, grid_for(F.ns * F.nc), dim3(256), 0, (hipStream_t)stream, F, value);
+    hipLaunchKernelGGL(k_fab_setval, grid_for(F.cells * F.nc), dim3(256), 0, (hipStream_t)stream, F, value);
     HIPCHK(hipGetLastError());
   });
 }
@@ -470,6 +488,7 @@ int ilbm_level_get_derived(ilbm_level* h, double* host_out) {
     }
     pf.ncomp = 4;
     of.ncomp = 6;
+    pf.dtype = of.dtype = ILBM_FAB_F64;
     Fab P = to_fab(pf), O = to_fab(of);
     double* buf = dev_alloc<double>(4 * P.ns + 6 * O.ns);
     P.p = buf;
@@ -479,11 +498,11 @@ int ilbm_level_get_derived(ilbm_level* h, double* host_out) {
     hipStream_t s = L.stream;
     hipLaunchKernelGGL(k_fab_from_cells, grid_for(cells), dim3(256), 0, s, P, (const double*)L.mout, c.n[0], c.n[1],
                        cells);
-    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(P.ns), dim3(256), 0, s, P, G);
+    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(P.cells), dim3(256), 0, s, P, G);
     with_lattice(L.lat, [&](auto lat) {
       using LAT = decltype(lat);
       for (int pass = 1; pass <= c.dim; ++pass)
-        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(P.ns), dim3(256), 0, s, P, G, pass, 0);
+        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(P.cells), dim3(256), 0, s, P, G, pass, 0);
     });
     hipLaunchKernelGGL(k_box_derived, grid_for(cells), dim3(256), 0, s, P, O, G, c.dx[0], c.dx[1], c.dx[2]);
     double* outd = dev_alloc<double>(2 * cells);
@@ -523,7 +542,7 @@ int ilbm_box_fill_periodic(const ilbm_case_desc* d, ilbm_fab* fab, const int vlo
     using namespace ilbm;
     const BoxGeom G = box_geom(*d, vlo, vhi);
     const Fab F = to_fab(*fab);
-    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G);
+    hipLaunchKernelGGL(k_box_fill_periodic, grid_for(F.cells), dim3(256), 0, (hipStream_t)stream, F, G);
     HIPCHK(hipGetLastError());
   });
 }
@@ -538,7 +557,7 @@ int ilbm_box_fill_physical(const ilbm_case_desc* d, ilbm_fab* fab, int mesoscopi
     with_lattice(box_lattice(*d), [&](auto L) {
       using LAT = decltype(L);
       for (int pass = 1; pass <= d->dim; ++pass)
-        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G,
+        hipLaunchKernelGGL(k_box_fill_physical<LAT>, grid_for(F.cells), dim3(256), 0, (hipStream_t)stream, F, G,
                            pass, mesoscopic ? 1 : 0);
     });
     HIPCHK(hipGetLastError());
@@ -574,7 +593,7 @@ int ilbm_box_bounce_back(const ilbm_case_desc* d, ilbm_fab* f, const int vlo[3],
     const Fab F = to_fab(*f);
     with_lattice(box_lattice(*d), [&](auto L) {
       using LAT = decltype(L);
-      hipLaunchKernelGGL(k_box_bounce_back<LAT>, grid_for(F.ns), dim3(256), 0, (hipStream_t)stream, F, G);
+      hipLaunchKernelGGL(k_box_bounce_back<LAT>, grid_for(F.cells), dim3(256), 0, (hipStream_t)stream, F, G);
     });
     HIPCHK(hipGetLastError());
   });
@@ -636,6 +655,239 @@ int ilbm_box_sum(const ilbm_fab* fab, int comp, const int vlo[3], const int vhi[
     dev_free(part);
     HIPCHK(e1);
     HIPCHK(e2);
+  });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Coarse-fine transfer on caller-owned fabs, refinement ratio 2
+// (Source/Core/AmrCoreLBM.cpp): FillPatchMesoscopic / FillPatchMacro of a
+// fine box at lev > 0 (:1545-1627, :2180-2197), InterpFromCoarseLevel of
+// MakeNewLevelFromCoarse (:2214-2237) and AverageDownTo (:1520-1526).
+// ---------------------------------------------------------------------------
+namespace ilbm {
+
+// GetDataMacro (:2239-2269): the coarse data at `time` -- new, old, or
+// FillPatchSingleLevel's a * old + b * new -- decided on the host
+struct TimeMix {
+  int mode;   // 0 old, 1 new, 2 a * old + b * new
+  double a, b;
+};
+
+static TimeMix time_mix(double t_old, double t_new, double time) {
+  const double sentinel = 1.0e50;   // a level not yet stepped keeps t_old = -1e200
+  if (std::fabs(t_old) > sentinel || std::fabs(t_new) > sentinel) return {1, 0.0, 0.0};
+  const double teps = std::fmax(1.0e-12, std::fabs(t_new - t_old) * 1.0e-3);
+  if (time >= t_new - teps && time <= t_new + teps) return {1, 0.0, 0.0};
+  if (time >= t_old - teps && time <= t_old + teps) return {0, 0.0, 0.0};
+  return {2, (t_new - time) / (t_new - t_old), (time - t_old) / (t_new - t_old)};
+}
+
+__device__ inline double crse_at(const Fab& Co, const Fab& Cn, const TimeMix& tm, int i, int j, int k, int n) {
+  if (tm.mode == 0) return Co.at(i, j, k, n);
+  if (tm.mode == 1) return Cn.at(i, j, k, n);
+  return tm.a * Co.at(i, j, k, n) + tm.b * Cn.at(i, j, k, n);
+}
+
+constexpr int kMaxInterpComp = 8;
+
+// AMReX cell_cons_interp at fine cell (I, J, K): centred slopes limited per
+// direction (minmod of the one-sided differences), one factor per direction
+// shared by the ncomp components, offsets of +-1/4 coarse cell.
+__device__ inline void cons_interp_fab(const Fab& Co, const Fab& Cn, const TimeMix& tm, int dim, int ncomp, int I,
+                                       int J, int K, double* out) {
+  const int ic = floordiv2(I), jc = floordiv2(J), kc = (dim == 3) ? floordiv2(K) : K;
+  double u0[kMaxInterpComp], s[3][kMaxInterpComp];
+  double sf[3] = {1.0, 1.0, 1.0};
+  for (int n = 0; n < ncomp; ++n) u0[n] = crse_at(Co, Cn, tm, ic, jc, kc, n);
+  for (int d = 0; d < dim; ++d) {
+    const int pi = ic + (d == 0), pj = jc + (d == 1), pk = kc + (d == 2);
+    const int mi = ic - (d == 0), mj = jc - (d == 1), mk = kc - (d == 2);
+    for (int n = 0; n < ncomp; ++n) {
+      const double up = crse_at(Co, Cn, tm, pi, pj, pk, n);
+      const double um = crse_at(Co, Cn, tm, mi, mj, mk, n);
+      const double dc = 0.5 * (up - um);
+      const double df = 2.0 * (up - u0[n]);
+      const double db = 2.0 * (u0[n] - um);
+      double sx = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
+      sx = copysign(1.0, dc) * fmin(sx, fabs(dc));
+      if (dc != 0.0) sf[d] = fmin(sf[d], sx / dc);
+      s[d][n] = dc;
+    }
+  }
+  const double ox = ((double)(I - ic * 2) + 0.5) / 2.0 - 0.5;
+  const double oy = ((double)(J - jc * 2) + 0.5) / 2.0 - 0.5;
+  const double oz = (dim == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
+  for (int n = 0; n < ncomp; ++n) {
+    double v = u0[n] + ox * (s[0][n] * sf[0]);
+    v = v + oy * (s[1][n] * sf[1]);
+    if (dim == 3) v = v + oz * (s[2][n] * sf[2]);
+    out[n] = v;
+  }
+}
+
+// Interpolated macros on the cells of `dom` (the f fab's box when f is
+// given, else the macro fab's) inside (ghosts = 0) or outside (ghosts = 1)
+// the valid box; f = feq of them where f is given.
+template <class LAT>
+__global__ void k_box_cf_fill(Fab Co, Fab Cn, TimeMix tm, Fab M, Fab Fq, Fab dom, int ncomp, BoxGeom G, int ghosts) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= dom.cells) return;
+  int iv[3];
+  fab_cell(dom, t, iv);
+  bool in_valid = true;
+  for (int d = 0; d < LAT::D; ++d)
+    if (iv[d] < G.vlo[d] || iv[d] > G.vhi[d]) in_valid = false;
+  if (in_valid == (ghosts != 0)) return;
+  double m[kMaxInterpComp];
+  cons_interp_fab(Co, Cn, tm, LAT::D, ncomp, iv[0], iv[1], iv[2], m);
+  if (M.p)
+    for (int n = 0; n < ncomp; ++n) M.at(iv[0], iv[1], iv[2], n) = m[n];
+  if (!Fq.p) return;
+  // FillPatchMesoscopic's equilibrium of the ghost cells (:1605-1620)
+  const double r = fmax(m[0], 1.0e-12);
+  const double u0 = m[1], v0 = m[2], w0 = m[3];
+  const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
+    Fq.at(iv[0], iv[1], iv[2], q) = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+  }
+}
+
+// amrex::average_down of f (Q comps) and macros (nmac comps) over the coarse
+// cells [clo, clo + cn): crse = (sum of the children, k then j then i) / 2^d
+template <class LAT>
+__global__ void k_box_average_down(Fab Ff, Fab Mf, Fab Fc, Fab Mc, int nmac, int clo0, int clo1, int clo2, int cnx,
+                                   int cny, int64_t ncov) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ncov) return;
+  const int i = clo0 + (int)(t % cnx), j = clo1 + (int)((t / cnx) % cny), k = clo2 + (int)(t / ((int64_t)cnx * cny));
+  const int ii = 2 * i, jj = 2 * j, kk = (LAT::D == 3) ? 2 * k : k;
+  const int nkr = (LAT::D == 3) ? 2 : 1;
+  const double volfrac = (LAT::D == 3) ? 0.125 : 0.25;
+  for (int q = 0; q < LAT::Q; ++q) {
+    double c = 0.0;
+    for (int kr = 0; kr < nkr; ++kr)
+      for (int jr = 0; jr < 2; ++jr)
+        for (int ir = 0; ir < 2; ++ir) c += Ff.at(ii + ir, jj + jr, kk + kr, q);
+    Fc.at(i, j, k, q) = volfrac * c;
+  }
+  for (int n = 0; n < nmac; ++n) {
+    double c = 0.0;
+    for (int kr = 0; kr < nkr; ++kr)
+      for (int jr = 0; jr < 2; ++jr)
+        for (int ir = 0; ir < 2; ++ir) c += Mf.at(ii + ir, jj + jr, kk + kr, n);
+    Mc.at(i, j, k, n) = volfrac * c;
+  }
+}
+
+// the coarse cells a fine region's interpolation reads (its parents grown by 1)
+static void check_coarse_cover(const ilbm_fab* crse, int ncomp, int dim, const int flo[3], const int fhi[3]) {
+  if (!crse || !crse->p) throw StatusError{ILBM_ERR_INVALID, "null coarse fab"};
+  if (crse->ncomp < ncomp) throw StatusError{ILBM_ERR_INVALID, "coarse fab has too few components"};
+  for (int d = 0; d < dim; ++d) {
+    const int lo = (flo[d] >= 0 ? flo[d] / 2 : -((1 - flo[d]) / 2)) - 1;
+    const int hi = (fhi[d] >= 0 ? fhi[d] / 2 : -((1 - fhi[d]) / 2)) + 1;
+    if (crse->lo[d] > lo || crse->hi[d] < hi)
+      throw StatusError{ILBM_ERR_INVALID, "coarse fab does not cover the parents of the fine cells and their neighbours"};
+  }
+}
+
+}  // namespace ilbm
+
+extern "C" {
+
+int ilbm_box_cf_fill(const ilbm_case_desc* d, const ilbm_fab* crse_old, const ilbm_fab* crse_new, double t_old,
+                     double t_new, double time, ilbm_fab* f_fine, ilbm_fab* macro_fine, int ncomp,
+                     const int valid_lo[3], const int valid_hi[3], void* stream) {
+  if (!d || !valid_lo || !valid_hi || (!f_fine && !macro_fine)) return ILBM_ERR_INVALID;
+  if (ncomp < (f_fine ? 4 : 1) || ncomp > ilbm::kMaxInterpComp) return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, valid_lo, valid_hi);
+    const int dim = d->dim;
+    const ilbm_fab* domf = f_fine ? f_fine : macro_fine;
+    if (f_fine) check_fab(f_fine, d->q, G, 0);
+    if (macro_fine) {
+      check_fab(macro_fine, ncomp, G, 0);
+      for (int e = 0; e < dim; ++e)
+        if (macro_fine->lo[e] > domf->lo[e] || macro_fine->hi[e] < domf->hi[e])
+          throw StatusError{ILBM_ERR_INVALID, "macro fab does not cover the f fab"};
+    }
+    // the fine ghost cells are coarse-fine ghosts: inside the level's domain
+    for (int e = 0; e < dim; ++e)
+      if (domf->lo[e] < 0 || domf->hi[e] > d->n[e] - 1)
+        throw StatusError{ILBM_ERR_UNSUPPORTED, "fine fab reaches outside the level's domain (physical ghosts)"};
+    check_coarse_cover(crse_old, ncomp, dim, domf->lo, domf->hi);
+    check_coarse_cover(crse_new, ncomp, dim, domf->lo, domf->hi);
+    const Fab Co = to_fab(*crse_old), Cn = to_fab(*crse_new), D = to_fab(*domf);
+    const Fab M = macro_fine ? to_fab(*macro_fine) : Fab{};
+    const Fab Fq = f_fine ? to_fab(*f_fine) : Fab{};
+    const TimeMix tm = time_mix(t_old, t_new, time);
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_cf_fill<LAT>, grid_for(D.cells), dim3(256), 0, (hipStream_t)stream, Co, Cn, tm, M, Fq,
+                         D, ncomp, G, 1);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_interp_from_coarse(const ilbm_case_desc* d, const ilbm_fab* crse, ilbm_fab* macro_fine, int ncomp,
+                                const int lo[3], const int hi[3], void* stream) {
+  if (!d || !crse || !macro_fine || !lo || !hi || ncomp < 1 || ncomp > ilbm::kMaxInterpComp)
+    return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const BoxGeom G = box_geom(*d, lo, hi);
+    const int dim = d->dim;
+    check_fab(macro_fine, ncomp, G, 0);
+    for (int e = 0; e < dim; ++e)
+      if (lo[e] < 0 || hi[e] > d->n[e] - 1) throw StatusError{ILBM_ERR_INVALID, "box outside the level's domain"};
+    check_coarse_cover(crse, ncomp, dim, G.vlo, G.vhi);
+    const Fab C = to_fab(*crse), M = to_fab(*macro_fine);
+    const TimeMix tm{1, 0.0, 0.0};
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_cf_fill<LAT>, grid_for(M.cells), dim3(256), 0, (hipStream_t)stream, C, C, tm, M, Fab{},
+                         M, ncomp, G, 0);
+    });
+    HIPCHK(hipGetLastError());
+  });
+}
+
+int ilbm_box_average_down(const ilbm_case_desc* d, const ilbm_fab* f_fine, const ilbm_fab* macro_fine,
+                          const int fine_lo[3], const int fine_hi[3], ilbm_fab* f_crse, ilbm_fab* macro_crse, int nmac,
+                          void* stream) {
+  if (!d || !f_fine || !macro_fine || !f_crse || !macro_crse || !fine_lo || !fine_hi || nmac < 1)
+    return ILBM_ERR_INVALID;
+  return guarded([&] {
+    using namespace ilbm;
+    const int dim = d->dim;
+    const BoxGeom GF = box_geom(*d, fine_lo, fine_hi);
+    int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+    for (int e = 0; e < dim; ++e) {
+      if (fine_lo[e] % 2 != 0 || fine_hi[e] % 2 != 1)
+        throw StatusError{ILBM_ERR_INVALID, "fine box must cover whole coarse cells"};
+      clo[e] = fine_lo[e] / 2;
+      chi[e] = (fine_hi[e] - 1) / 2;
+    }
+    const BoxGeom GC = box_geom(*d, clo, chi);
+    check_fab(f_fine, d->q, GF, 0);
+    check_fab(macro_fine, nmac, GF, 0);
+    check_fab(f_crse, d->q, GC, 0);
+    check_fab(macro_crse, nmac, GC, 0);
+    const int cnx = chi[0] - clo[0] + 1, cny = chi[1] - clo[1] + 1;
+    const int64_t ncov = (int64_t)cnx * cny * (chi[2] - clo[2] + 1);
+    with_lattice(box_lattice(*d), [&](auto L) {
+      using LAT = decltype(L);
+      hipLaunchKernelGGL(k_box_average_down<LAT>, grid_for(ncov), dim3(256), 0, (hipStream_t)stream, to_fab(*f_fine),
+                         to_fab(*macro_fine), to_fab(*f_crse), to_fab(*macro_crse), nmac, clo[0], clo[1], clo[2], cnx,
+                         cny, ncov);
+    });
+    HIPCHK(hipGetLastError());
   });
 }
 

@@ -76,6 +76,7 @@ struct IbArgs {
   int fuse_explicit;    // explicit / diag force evaluated by k_ib_interp
   int first_iter;
   const int* ne_cells;  // band cells that receive spreading entries
+  const int* ne_e;      // [2][n_ne]: each such cell's first and last + 1 entry
   int n_ne;
   int nloop;            // direct-forcing iterations per step
   int use_ivc;          // IVC solve (operator factorized) instead of explicit forcing
@@ -264,12 +265,11 @@ __device__ inline void target_velocity(const IbArgs& a, int m, double* ux, doubl
   *ux = a.ub[0] + cx; *uy = a.ub[1] + cy; *uz = a.ub[2] + cz;
 }
 
-// compute_lagrangian_force_explicit (:741-873): F += relax*(U_t - U)/denom
-__device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, double Uy_m, double Uz_m,
-                                      double& F0, double& F1, double& F2) {
+// compute_lagrangian_force_explicit (:741-873): F += relax*(U_t - U)/denom,
+// with the target velocity (Ux, Uy, Uz) of marker m given
+__device__ inline void explicit_force_tv(const IbArgs& a, int m, double Ux, double Uy, double Uz, double Ux_m,
+                                         double Uy_m, double Uz_m, double& F0, double& F1, double& F2) {
   const int nm = a.nm;
-  double Ux, Uy, Uz;
-  target_velocity(a, m, &Ux, &Uy, &Uz);
   double denom = a.dt;
   if (a.coupling == 2) {
     const double dV = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
@@ -305,9 +305,20 @@ __device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, doubl
   F2 += a.relax * (Uz - Uz_m) / denom;
 }
 
+__device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, double Uy_m, double Uz_m,
+                                      double& F0, double& F1, double& F2) {
+  double Ux, Uy, Uz;
+  target_velocity(a, m, &Ux, &Uy, &Uz);
+  explicit_force_tv(a, m, Ux, Uy, Uz, Ux_m, Uy_m, Uz_m, F0, F1, F2);
+}
+
 // One wavefront per marker: the lanes evaluate the stencil products in
 // parallel, then four lanes accumulate them in the reference's (ii, jj, kk)
 // order, so the sums are bit-identical to the serial loop (:642-681).
+// The loads that do not depend on the marker's cell -- the delta weights of
+// each lane's stencil points and lane 0's target velocity -- are issued
+// before the cell index arrives, so the chain of dependent memory accesses
+// is cell index -> band velocities -> marker U / F.
 constexpr int kInterpWaves = 4;
 __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
   __shared__ double prod[kInterpWaves][4][128];
@@ -317,10 +328,26 @@ __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
   const bool active = m < a.nm;
   const int nm = a.nm;
   const double d = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+  const int npts = (a.dim == 3) ? 125 : 25;
+  // delta weights of the lane's stencil points t = lane and lane + 64
+  double dval[2] = {0.0, 0.0};
+  for (int r = 0; r < 2; ++r) {
+    const int t = lane + 64 * r;
+    if (!active || t >= npts) continue;
+    int ii, jj, kk;
+    if (a.dim == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
+    else { ii = t / 5; jj = t % 5; kk = 0; }
+    const double tU = a.wts[(0 * 5 + ii) * nm + m];
+    const double tV = a.wts[(1 * 5 + jj) * nm + m];
+    dval[r] = (a.dim == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
+  }
+  double tvx = 0.0, tvy = 0.0, tvz = 0.0;
+  if (active && lane == 0 && a.fuse_explicit) target_velocity(a, m, &tvx, &tvy, &tvz);
   const int b0 = active ? a.base[m] : 0, b1 = active ? a.base[nm + m] : 0;
   const int b2 = active ? a.base[2 * nm + m] : 0;
-  const int npts = (a.dim == 3) ? 125 : 25;
-  for (int t = lane; active && t < npts; t += 64) {
+  for (int r = 0; r < 2; ++r) {
+    const int t = lane + 64 * r;
+    if (!active || t >= npts) continue;
     int ii, jj, kk;
     if (a.dim == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
     else { ii = t / 5; jj = t % 5; kk = 0; }
@@ -329,13 +356,10 @@ __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
     if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
     valid[wv][t] = ok ? 1 : 0;
     if (!ok) continue;
-    const double tU = a.wts[(0 * 5 + ii) * nm + m];
-    const double tV = a.wts[(1 * 5 + jj) * nm + m];
-    const double dval = (a.dim == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
-    prod[wv][0][t] = dval * d;
-    prod[wv][1][t] = dval * ev_value(a, p, 0) * d;
-    prod[wv][2][t] = dval * ev_value(a, p, 1) * d;
-    prod[wv][3][t] = (a.dim == 3) ? dval * ev_value(a, p, 2) * d : 0.0;
+    prod[wv][0][t] = dval[r] * d;
+    prod[wv][1][t] = dval[r] * ev_value(a, p, 0) * d;
+    prod[wv][2][t] = dval[r] * ev_value(a, p, 1) * d;
+    prod[wv][3][t] = (a.dim == 3) ? dval[r] * ev_value(a, p, 2) * d : 0.0;
   }
   __syncthreads();
   if (active && lane < 4) {
@@ -363,7 +387,7 @@ __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
     U0 += u; U1 += v; U2 += w;
     a.U[m] = U0; a.U[nm + m] = U1; a.U[2 * nm + m] = U2;
     double F0 = 0.0, F1 = 0.0, F2 = 0.0;
-    if (a.fuse_explicit) explicit_force(a, m, U0, U1, U2, F0, F1, F2);
+    if (a.fuse_explicit) explicit_force_tv(a, m, tvx, tvy, tvz, U0, U1, U2, F0, F1, F2);
     a.F[m] = F0; a.F[nm + m] = F1; a.F[2 * nm + m] = F2;
   }
 }
@@ -876,6 +900,9 @@ __global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, v
 // k_ib_spread does), exports F = dt rho EulerForce and adds the Guo term of
 // the coming collision, one population per lane.  Band cells without
 // entries keep F = 0.  Block partials of F in wave order for the record.
+// The cell's own inputs -- its entry range, macros and populations -- do not
+// depend on the spreading sum and are loaded first, so the dependent chain
+// is entries -> marker forces -> sum -> store.
 template <class LAT, class ST>
 __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
                                                         const RecArgs& rec, int blk, int nblk) {
@@ -890,7 +917,26 @@ __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const G
   const bool active = ci < a.n_ne;
   const int nm = a.nm;
   const int b = active ? a.ne_cells[ci] : 0;
-  const int e0 = active ? a.rowptr[b] : 0, e1 = active ? a.rowptr[b + 1] : 0;
+  const int e0 = active ? a.ne_e[ci] : 0, e1 = active ? a.ne_e[a.n_ne + ci] : 0;
+  // population `lane` of the cell and its macros, for the Guo term
+  R* gb = static_cast<R*>(gbuf);
+  int64_t o = 0;
+  T gq = T(0), ux = T(0), uy = T(0), uz = T(0);
+  double rho = 0.0;
+  if (active) {
+    const int bi = (int)(b % a.band_n[0]);
+    const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+    const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+    const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
+    if (lane < LAT::Q) {
+      o = (int64_t)lane * g.qs + g.idx(i, j, k);
+      gq = ST::load(gb, o, T(LAT::w[lane]));
+      ux = T(a.Mband[a.band_cells + b]);
+      uy = T(a.Mband[2 * a.band_cells + b]);
+      uz = T(a.Mband[3 * a.band_cells + b]);
+    }
+    if (lane == 0) rho = a.Mband[b];
+  }
   double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
   int cur = 0;
   if (lane == 0) rounds[wv] = (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
@@ -932,7 +978,6 @@ __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const G
     double fx = 0.0, fy = 0.0, fz = 0.0;
     if (active) {
       if (cur != 0) for (int c = 0; c < 3; ++c) tot[c] += gs[c];
-      const double rho = a.Mband[b];
       fx = a.dt * rho * tot[0];
       fy = a.dt * rho * tot[1];
       fz = a.dt * rho * tot[2];
@@ -946,19 +991,9 @@ __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const G
   if (active && lane < LAT::Q) {
     const double fx = fsh[wv][0], fy = fsh[wv][1], fz = fsh[wv][2];
     if (!(fx == 0.0 && fy == 0.0 && fz == 0.0)) {
-      const int bi = (int)(b % a.band_n[0]);
-      const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
-      const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
-      const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
-      const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
-      const T uz = T(a.Mband[3 * a.band_cells + b]);
       const T fac = T(1.0) - T(0.5) / T(tau);
-      R* gb = static_cast<R*>(gbuf);
-      const int q = lane;
-      const int64_t o = (int64_t)q * g.qs + g.idx(i, j, k);
-      T gq = ST::load(gb, o, T(LAT::w[q]));
-      gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
-      ST::store(gb, o, gq, T(LAT::w[q]));
+      gq += guo_q<LAT, T>(lane, fac, ux, uy, uz, T(fx), T(fy), T(fz));
+      ST::store(gb, o, gq, T(LAT::w[lane]));
     }
   }
   if (threadIdx.x == 0) {

@@ -174,9 +174,11 @@ struct StepArgs {
   int tile_bx;           // tile width of the batched launch (block = tile_bx x kBlock/tile_bx)
   int band_force;        // the band carries the IB force (else it only exports macros)
   // sub-box split of a level: 0 every cell; 1 the cells outside the box;
-  // 2 the cells of the box, the launch grid covering the box from sub_lo
+  // 2 the cells of the box, the launch grid covering the box from sub_lo;
+  // 3 as 2 without the cells of the inner box (in_lo, in_n)
   int sub_mode;
   int sub_lo[3], sub_n[3];
+  int in_lo[3], in_n[3];
   int off32;             // a plane's cells are addressable with 32-bit byte offsets
   // the uniform parameters rounded once to FP32 on the host, for shifted
   // storage (bound by vector ALU issue: no per-wavefront conversion or
@@ -342,7 +344,14 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
     const bool in_sub = (unsigned)(i - a.sub_lo[0]) < (unsigned)a.sub_n[0] &&
                         (unsigned)(j - a.sub_lo[1]) < (unsigned)a.sub_n[1] &&
                         (unsigned)(k - a.sub_lo[2]) < (unsigned)a.sub_n[2];
-    active = active && (a.sub_mode == 2 ? in_sub : !in_sub);
+    if (a.sub_mode == 3) {
+      const bool in_inner = (unsigned)(i - a.in_lo[0]) < (unsigned)a.in_n[0] &&
+                            (unsigned)(j - a.in_lo[1]) < (unsigned)a.in_n[1] &&
+                            (unsigned)(k - a.in_lo[2]) < (unsigned)a.in_n[2];
+      active = active && in_sub && !in_inner;
+    } else {
+      active = active && (a.sub_mode == 2 ? in_sub : !in_sub);
+    }
   }
   double rho_acc = 0.0;
   if (active) {
@@ -606,8 +615,8 @@ template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double lds[kBlock / 64];
   const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
-  const int o0 = (a.sub_mode == 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode == 2) ? a.sub_lo[1] : 0;
-  const int o2 = (a.sub_mode == 2) ? a.sub_lo[2] : 0;
+  const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
+  const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
   step_body<LAT, ST, AM>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x, o1 + blockIdx.y * blockDim.y + threadIdx.y,
                            o2 + blockIdx.z, blk, lds);
 }
@@ -946,15 +955,18 @@ __global__ void __launch_bounds__(1024) k_sum_partials(const double* p, int64_t 
 }
 
 // Sum(Rho) of consecutive steps whose block partials fill consecutive ring
-// slots: block b reduces slot b as k_mass_total does (the second array may
-// be empty) and writes out[b]
+// slots: block b reduces slot b of each partial array as k_mass_total does
+// (the second and third arrays may be empty) and writes out[b]
 __global__ void __launch_bounds__(1024) k_mass_ring(const double* p0, int64_t stride0, int64_t n0,
-                                                    const double* p1, int64_t stride1, int64_t n1, double* out) {
+                                                    const double* p1, int64_t stride1, int64_t n1,
+                                                    const double* p2, int64_t stride2, int64_t n2, double* out) {
   const int64_t b = blockIdx.x;
   const double a = sum_partials_body(p0 + b * stride0, n0);
   __syncthreads();
   const double c = (n1 > 0) ? sum_partials_body(p1 + b * stride1, n1) : 0.0;
-  if (threadIdx.x == 0) out[b] = a + c;
+  __syncthreads();
+  const double e = (n2 > 0) ? sum_partials_body(p2 + b * stride2, n2) : 0.0;
+  if (threadIdx.x == 0) out[b] = (a + c) + e;
 }
 
 // End of a batched step, one block per case: Sum(Rho) into the case's mass

@@ -429,6 +429,7 @@ struct Level {
   std::vector<double> ivc_xc;  // positions the operator was built for
   int* rowptr_d = nullptr;
   int* ne_d = nullptr;
+  int* nee_d = nullptr;
   int n_ne = 0;
   int* entm_d = nullptr;
   int* entc_d = nullptr;
@@ -468,7 +469,8 @@ struct Level {
   // memory operations, hipStreamWrite/WaitValue64, cost less host time but
   // run as runtime kernels of 3-5 us each on the GPU, which lengthens the
   // far stream's chain; the host has slack, the GPU has not.)
-  enum { X_GHOST = 0, X_NEAR = 1, X_EXP = 2, X_REC0 = 3, X_N = X_REC0 + 4 };
+  enum { X_GHOST = 0, X_NEAR = 1, X_EXP = 2, X_REC0 = 3, X_FAR0 = X_REC0 + 4, X_RING0 = X_FAR0 + 2,
+         X_N = X_RING0 + 2 };
   hipEvent_t xev[X_N] = {};
   uint64_t xseq[X_N] = {};
   uint64_t rec_seq[4] = {};
@@ -485,6 +487,16 @@ struct Level {
   // the near box keeps off the non-periodic faces: the ghost slabs of the
   // next step depend on the far cells only and are filled right after them
   bool near_off_walls = false;
+  // Three-region step (the near box grown by one more cell, its "ring
+  // box", keeps off the non-periodic faces too): the far cells outside the
+  // ring box never read what the near box or the IB forcing of any step
+  // writes, so they stream back to back on the second stream, and the ring
+  // between the boxes follows the near box on the first
+  bool ring_ok = false;
+  int ring_lo[3] = {0, 0, 0}, ring_n[3] = {0, 0, 0};
+  dim3 grid_ring;
+  int64_t nblocks_ring = 0;
+  double* rring = nullptr;
   bool ghost_ahead = false;               // this step's slabs are already queued
   uint64_t ahead_signal = 0;              // and signalled with this value
 
@@ -500,7 +512,7 @@ struct Level {
       if (e) (void)hipEventDestroy(e);
     if (stream_b) (void)hipStreamDestroy(stream_b);
     if (stream_c) (void)hipStreamDestroy(stream_c);
-    dev_free(mring); dev_free(nring);
+    dev_free(mring); dev_free(nring); dev_free(rring);
     dev_free(gbuf[0]); dev_free(gbuf[1]);
     for (int f = 0; f < 6; ++f) dev_free(faces.p[f]);
     dev_free(partial); dev_free(mout); dev_free(fout); dev_free(mass_dev); dev_free(force_dev);
@@ -508,7 +520,7 @@ struct Level {
     dev_free(xm_d); dev_free(ut_d); dev_free(U_d); dev_free(F_d); dev_free(Mm_d);
     dev_free(base_d); dev_free(wts_d); dev_free(scale_d); dev_free(rhs_d); dev_free(sums_d);
     dev_free(Ainv_d); dev_free(rowptr_d); dev_free(entm_d); dev_free(entc_d); dev_free(entw_d); dev_free(frstate_d);
-    dev_free(ne_d); dev_free(fpartial); dev_free(clock_d); dev_free(xref_d); dev_free(chunk_d);
+    dev_free(ne_d); dev_free(nee_d); dev_free(fpartial); dev_free(clock_d); dev_free(xref_d); dev_free(chunk_d);
     for (int b = 0; b < 2; ++b) { dev_free(Mw[b]); dev_free(Dw[b]); }
     if (stream && own_stream) (void)hipStreamDestroy(stream);
   }
@@ -821,9 +833,18 @@ struct Level {
     for (int64_t b = 0; b < band_cells; ++b) rowptr[b + 1] += rowptr[b];
     nent = (int64_t)ents.size();
     dev_free(ne_d);
+    dev_free(nee_d);
     n_ne = (int)ne.size();
+    // each non-empty cell's entry range beside its index (one load fewer in
+    // the dependent chain of the spreading kernels)
+    std::vector<int> nee(2 * (size_t)std::max(n_ne, 1), 0);
+    for (int ci = 0; ci < n_ne; ++ci) { nee[ci] = rowptr[ne[ci]]; nee[n_ne + ci] = rowptr[ne[ci] + 1]; }
     ne_d = dev_alloc<int>(std::max(n_ne, 1));
-    if (n_ne) HIPCHK(hipMemcpyAsync(ne_d, ne.data(), sizeof(int) * n_ne, hipMemcpyHostToDevice, stream));
+    nee_d = dev_alloc<int>(2 * std::max(n_ne, 1));
+    if (n_ne) {
+      HIPCHK(hipMemcpyAsync(ne_d, ne.data(), sizeof(int) * n_ne, hipMemcpyHostToDevice, stream));
+      HIPCHK(hipMemcpyAsync(nee_d, nee.data(), sizeof(int) * 2 * n_ne, hipMemcpyHostToDevice, stream));
+    }
     if (nent > ent_cap) {
       dev_free(entm_d); dev_free(entc_d); dev_free(entw_d);
       ent_cap = std::max<int64_t>(nent, 1) * 2;
@@ -907,7 +928,7 @@ struct Level {
     a.base = base_d; a.wts = wts_d; a.scale = scale_d;
     a.Mband = Mband; a.EV = EV; a.EFI = EFI; a.EFT = EFT; a.Fband = Fband;
     a.rowptr = rowptr_d; a.ent_m = entm_d; a.ent_code = entc_d; a.ent_w = entw_d;
-    a.ne_cells = ne_d; a.n_ne = n_ne;
+    a.ne_cells = ne_d; a.ne_e = nee_d; a.n_ne = n_ne;
     a.chunk_box = chunk_d;
     a.ev_from_macro = (c.coupling_method == 1 || std::max(c.loop_ns, 1) == 1) ? 1 : 0;
     a.use_ivc = (c.coupling_method == 1 && ivc_ready) ? 1 : 0;
@@ -1022,6 +1043,50 @@ struct Level {
       }
     } else if (rec_enable) {
       hipLaunchKernelGGL(k_force_record, dim3(1), dim3(256), 0, stream, a, r, (int)gfix);
+    }
+    HIPCHK(hipGetLastError());
+  }
+
+  // The stages of update_forcing without the export and the force record
+  // (the per-box path exports into the caller's fabs): kinematics, then per
+  // iteration interpolation with the explicit force, the IVC solve,
+  // spreading into EulerForceIter, the marker sums and moments, and (more
+  // than one iteration) the velocity correction.  The macros of the band
+  // (Mband) are the caller's.
+  void ib_stages(double time) {
+    if (dev_kin) {
+      const double twopi = 2.0 * kPi;
+      const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
+      const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
+      MoveArgs mv{};
+      mv.xref = xref_d; mv.xm = xm_d; mv.ut = ut_d; mv.nm = nm;
+      mv.omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
+      mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
+      mv.cs = std::cos(angle); mv.sn = std::sin(angle);
+      mv.z = 0.0;
+      hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, mv, ib_args());
+    } else if (moving) {
+      marker_positions(c, mg, time, xm_h, ut_h);
+      upload_markers();
+    }
+    IbArgs a = ib_args();
+    const int tb = 256;
+    const unsigned gb = (unsigned)((band_cells + tb - 1) / tb);
+    if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
+    for (int it = 0; it < a.nloop; ++it) {
+      a.first_iter = (it == 0) ? 1 : 0;
+      hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
+                         stream, a);
+      if (a.use_ivc) {
+        hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
+        hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
+      }
+      if (dev_kin)
+        hipLaunchKernelGGL(k_ib_spread_all, dim3(gb), dim3(tb), 0, stream, a);
+      else if (n_ne)
+        hipLaunchKernelGGL(k_ib_spread_t, dim3((n_ne + 255) / 256), dim3(256), 0, stream, a);
+      hipLaunchKernelGGL(k_ib_sums, dim3(1), dim3(kSumsBlock), 0, stream, a);
+      if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_correct, dim3(gb), dim3(tb), 0, stream, a);
     }
     HIPCHK(hipGetLastError());
   }
@@ -1234,6 +1299,22 @@ struct Level {
       for (int d = 0; d < c.dim; ++d)
         if (nonper[d] && (near_lo[d] <= 0 || near_lo[d] + near_n[d] >= c.n[d])) near_off_walls = false;
       nblocks_near = (int64_t)grid_near.x * grid_near.y * grid_near.z;
+      // the ring box: the near box grown by one cell, the whole extent of a
+      // periodic dimension the near box already spans
+      ring_ok = near_off_walls;
+      for (int d = 0; d < 3; ++d) {
+        int lo = near_lo[d] - 1, hi = near_lo[d] + near_n[d];
+        if (d >= c.dim) { lo = 0; hi = 0; }
+        else if (near_n[d] >= c.n[d]) { lo = 0; hi = c.n[d] - 1; }
+        else if (c.periodic[d] && (lo < 0 || hi > c.n[d] - 1)) ring_ok = false;
+        else if (nonper[d] && (lo <= 0 || hi >= c.n[d] - 1)) ring_ok = false;
+        ring_lo[d] = std::max(lo, 0);
+        ring_n[d] = std::min(hi, c.n[d] - 1) - ring_lo[d] + 1;
+      }
+      grid_ring = dim3((ring_n[0] + block.x - 1) / block.x, (ring_n[1] + block.y - 1) / block.y, ring_n[2]);
+      nblocks_ring = (int64_t)grid_ring.x * grid_ring.y * grid_ring.z;
+      const char* two = std::getenv("ILBM_OVERLAP_TWO_REGIONS");
+      if (two && two[0] == '1') ring_ok = false;
       HIPCHK(hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking));
       for (hipEvent_t* e : {&ev_a, &ev_b})
         HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
@@ -1311,6 +1392,86 @@ struct Level {
     ring_step(stream_b, s, more, true);
   }
 
+  // One step in three regions (ring_ok).  Stream B: a wait for the ring of
+  // the last step, the far cells (outside the ring box), the next step's
+  // ghost slabs.  Stream A: IB forcing (reads the band macros of the last
+  // near box, adds the Guo term to band cells, which no far or ring cell
+  // reads), the near box, a wait for the far cells of the last step, then
+  // the ring (the cells of the ring box outside the near box).  The far
+  // cells of step s read, outside the near box, what the far cells and the
+  // ring of step s - 1 wrote, and overwrite what those read; the ring of
+  // step s reads what the far cells of step s - 1 wrote next to it and
+  // overwrites what they read; the near box and the ring read and write
+  // only cells of the ring box grown by one, and never the cells of the
+  // other buffer that the far cells of the same step read or write.  So the
+  // far kernels run back to back while the IB forcing, the near box and the
+  // ring of the same step run beside them.
+  struct PendingMass { int base = 0, n = 0, s = -1; };
+  PendingMass pend{};
+
+  void flush_mass(hipStream_t st) {
+    if (pend.n <= 0) return;
+    hipLaunchKernelGGL(k_mass_ring, dim3(pend.n), dim3(1024), 0, st,
+                       (const double*)(mring + (int64_t)pend.base * nblocks), nblocks, nblocks,
+                       (const double*)(nring + (int64_t)pend.base * nblocks_near), nblocks_near, nblocks_near,
+                       (const double*)(rring + (int64_t)pend.base * nblocks_ring), nblocks_ring, nblocks_ring,
+                       mass_dev + pend.s + 1 - pend.n);
+    pend = PendingMass{};
+  }
+
+  uint64_t far_sig[2] = {0, 0}, ring_sig[2] = {0, 0};
+
+  void split_step3(int s, double time, bool want, bool more, bool first) {
+    StepArgs sf = step_args(want);
+    sf.sub_mode = 1;
+    for (int d = 0; d < 3; ++d) { sf.sub_lo[d] = ring_lo[d]; sf.sub_n[d] = ring_n[d]; }
+    const int slot = mring_used;
+    sf.partial = mring + (int64_t)slot * nblocks;
+    StepArgs sn = sf;
+    sn.sub_mode = 2;
+    for (int d = 0; d < 3; ++d) { sn.sub_lo[d] = near_lo[d]; sn.sub_n[d] = near_n[d]; }
+    sn.partial = nring + (int64_t)slot * nblocks_near;
+    StepArgs sr = sf;
+    sr.sub_mode = 3;
+    for (int d = 0; d < 3; ++d) { sr.in_lo[d] = near_lo[d]; sr.in_n[d] = near_n[d]; }
+    sr.partial = rring + (int64_t)slot * nblocks_ring;
+    const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
+    const dim3 ggrid((unsigned)((npad + kGhostBlock - 1) / kGhostBlock));
+    // stream B
+    if (!first) xwait(stream_b, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
+    flush_mass(stream_b);
+    if (!ghost_ahead && npad) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+    ghost_ahead = false;
+    stepfn(grid, block, stream_b, sf, nullptr, nullptr);
+    far_sig[s & 1] = xsignal(stream_b, X_FAR0 + (s & 1));
+    if (more && npad) {
+      cur ^= 1;
+      const GhostArgs ga = ghost_args();
+      cur ^= 1;
+      ghostfn(ggrid, dim3(kGhostBlock), stream_b, ga);
+      ghost_ahead = true;
+    }
+    // stream A
+    ring_slot = async_record() ? s % kRecRing : -1;
+    ib_update_forcing(time, s);
+    ring_slot = -1;
+    stepfn(grid_near, block, stream, sn, nullptr, nullptr);
+    if (!first) xwait(stream, X_FAR0 + ((s - 1) & 1), far_sig[(s - 1) & 1]);
+    stepfn(grid_ring, block, stream, sr, nullptr, nullptr);
+    ring_sig[s & 1] = xsignal(stream, X_RING0 + (s & 1));
+    // Sum(Rho) slots: a finished half is reduced on stream B once the ring
+    // of its last step is done (the next step's wait, or the advance's end)
+    ++mring_used;
+    const int n = mring_used - mring_base;
+    if (n == mring_slots / 2 || !more) {
+      pend.base = mring_base;
+      pend.n = n;
+      pend.s = s;
+      mring_base = mring_used;
+      if (mring_used == mring_slots || !more) mring_used = mring_base = 0;
+    }
+  }
+
   // After step s wrote ring slot mring_used: when half of the ring is
   // filled (or the advance ends) its steps' Sum(Rho) are reduced on `st`.
   // The halves alternate, so a near box queued ahead of the reduction
@@ -1319,9 +1480,12 @@ struct Level {
     ++mring_used;
     const int n = mring_used - mring_base;
     if (n == mring_slots / 2 || !more) {
+      const bool with_ring = with_near && rring;
       hipLaunchKernelGGL(k_mass_ring, dim3(n), dim3(1024), 0, st, (const double*)(mring + (int64_t)mring_base * nblocks),
                          nblocks, nblocks, (const double*)(with_near ? nring + (int64_t)mring_base * nblocks_near : nullptr),
                          with_near ? nblocks_near : (int64_t)0, with_near ? nblocks_near : (int64_t)0,
+                         (const double*)(with_ring ? rring + (int64_t)mring_base * nblocks_ring : nullptr),
+                         with_ring ? nblocks_ring : (int64_t)0, with_ring ? nblocks_ring : (int64_t)0,
                          mass_dev + s + 1 - n);
       mring_base = mring_used;
       if (mring_used == mring_slots || !more) mring_used = mring_base = 0;
@@ -1341,14 +1505,16 @@ struct Level {
 
   // ring slots for the mass partials of this level (and its near box)
   void ensure_mring(bool split) {
-    const int64_t per = nblocks + (split ? nblocks_near : 0);
-    if (mring && (!split || nring)) return;
-    dev_free(mring); dev_free(nring);
-    mring = nring = nullptr;
+    const bool three = split && ring_ok;
+    const int64_t per = nblocks + (split ? nblocks_near : 0) + (three ? nblocks_ring : 0);
+    if (mring && (!split || nring) && (!three || rring)) return;
+    dev_free(mring); dev_free(nring); dev_free(rring);
+    mring = nring = rring = nullptr;
     // two halves of at most 32 steps each, at most 64 MiB
     mring_slots = 2 * (int)std::max<int64_t>(1, std::min<int64_t>(32, ((int64_t)32 << 20) / (8 * per)));
     mring = dev_alloc<double>((size_t)nblocks * mring_slots);
     if (split) nring = dev_alloc<double>((size_t)nblocks_near * mring_slots);
+    if (three) rring = dev_alloc<double>((size_t)nblocks_ring * mring_slots);
     mring_used = mring_base = 0;
   }
 
@@ -1378,7 +1544,8 @@ struct Level {
       t_old = t_new;
       t_new += dt;
       if (split) {
-        split_step(s, time, want_state && s == nsteps - 1, s + 1 < nsteps);
+        if (ring_ok) split_step3(s, time, want_state && s == nsteps - 1, s + 1 < nsteps, s == 0);
+        else split_step(s, time, want_state && s == nsteps - 1, s + 1 < nsteps);
         cur ^= 1;
         ++istep;
         HIPCHK(hipGetLastError());
@@ -1412,6 +1579,12 @@ struct Level {
       HIPCHK(hipGetLastError());
     }
     if (split) {
+      if (ring_ok) {
+        // the last step's Sum(Rho) once its ring is done
+        const int last = nsteps - 1;
+        xwait(stream_b, X_RING0 + (last & 1), ring_sig[last & 1]);
+        flush_mass(stream_b);
+      }
       HIPCHK(hipEventRecord(ev_b, stream_b));
       HIPCHK(hipStreamWaitEvent(stream, ev_b, 0));
       if (stream_c) {
@@ -1749,4 +1922,5 @@ int ilbm_copy_probe(int64_t bytes, int iters, double* avg_ms) {
 
 #include "ilbm_batch_impl.h"
 #include "ilbm_box_impl.h"
+#include "ilbm_box_ib_impl.h"
 #include "ilbm_amr_impl.h"

@@ -69,13 +69,39 @@ EXPORTED = [
     "ilbm_batch_checkpoint_size", "ilbm_batch_checkpoint", "ilbm_batch_add_from_checkpoint",
     "ilbm_box_copy", "ilbm_box_setval", "ilbm_device_alloc", "ilbm_device_free", "ilbm_copy_to_device",
     "ilbm_copy_to_host", "ilbm_device_synchronize", "ilbm_box_sum_noalloc", "ilbm_box_sum_noalloc_max_cells",
+    "ilbm_box_cf_fill", "ilbm_box_interp_from_coarse", "ilbm_box_average_down",
+    "ilbm_ib_create", "ilbm_ib_destroy", "ilbm_ib_update_forcing", "ilbm_ib_num_markers", "ilbm_ib_force_sum",
+    "ilbm_ib_torque", "ilbm_ib_marker_state",
 ]
 
 ERR_FULL = 6
 
+FAB_F64, FAB_F32 = 0, 1
+
+
 class Fab(C.Structure):
-    """ilbm_fab (include/ilbm.h): an Array4 of FP64 in device memory."""
-    _fields_ = [("p", C.c_void_p), ("lo", I3), ("hi", I3), ("ncomp", C.c_int)]
+    """ilbm_fab (include/ilbm.h): an Array4 in device memory; strides of 0
+    are the dense strides of the box."""
+    _fields_ = [("p", C.c_void_p), ("lo", I3), ("hi", I3), ("ncomp", C.c_int), ("dtype", C.c_int),
+                ("jstride", C.c_int64), ("kstride", C.c_int64), ("nstride", C.c_int64)]
+
+
+def make_fab(ptr: int, lo, hi, ncomp: int, strides=(0, 0, 0), comp: int = 0) -> Fab:
+    """A descriptor of device memory at `ptr` (FP64).  With comp > 0 the fab
+    is the alias of component comp of a larger array (MultiFab make_alias):
+    the pointer moves by comp component strides."""
+    f = Fab()
+    nx, ny, nz = (hi[d] - lo[d] + 1 for d in range(3))
+    js = strides[0] or nx
+    ks = strides[1] or js * ny
+    ns = strides[2] or ks * nz
+    f.p = int(ptr) + 8 * comp * ns
+    f.lo = I3(*lo)
+    f.hi = I3(*hi)
+    f.ncomp = int(ncomp)
+    f.dtype = FAB_F64
+    f.jstride, f.kstride, f.nstride = (int(x) for x in strides)
+    return f
 
 
 _lib = None
@@ -156,6 +182,17 @@ def load():
     L.ilbm_box_sum.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
     L.ilbm_box_sum_noalloc.argtypes = [PF, C.c_int, P(C.c_int), P(C.c_int), P(C.c_double), vp]
     L.ilbm_box_sum_noalloc_max_cells.argtypes = [P(C.c_int64)]
+    L.ilbm_box_cf_fill.argtypes = [P(CaseDesc), PF, PF, C.c_double, C.c_double, C.c_double, PF, PF, C.c_int,
+                                   P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_interp_from_coarse.argtypes = [P(CaseDesc), PF, PF, C.c_int, P(C.c_int), P(C.c_int), vp]
+    L.ilbm_box_average_down.argtypes = [P(CaseDesc), PF, PF, P(C.c_int), P(C.c_int), PF, PF, C.c_int, vp]
+    L.ilbm_ib_create.argtypes = [P(CaseDesc), P(C.c_int), P(C.c_int), P(vp)]
+    L.ilbm_ib_destroy.argtypes = [vp]
+    L.ilbm_ib_update_forcing.argtypes = [vp, PF, PF, PF, PF, PF, PF, PF, C.c_double, C.c_double, vp]
+    L.ilbm_ib_num_markers.argtypes = [vp, P(C.c_int64)]
+    L.ilbm_ib_force_sum.argtypes = [vp, P(C.c_double)]
+    L.ilbm_ib_torque.argtypes = [vp, P(C.c_double)]
+    L.ilbm_ib_marker_state.argtypes = [vp, P(C.c_double), P(C.c_double), P(C.c_double)]
     L.ilbm_box_copy.argtypes = [PF, PF, P(C.c_int), P(C.c_int), C.c_int, vp]
     L.ilbm_box_setval.argtypes = [PF, C.c_double, vp]
     L.ilbm_device_alloc.argtypes = [C.c_int64, P(vp)]

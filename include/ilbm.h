@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define ILBM_ABI_VERSION 2
+#define ILBM_ABI_VERSION 3
 
 typedef enum {
   ILBM_OK = 0,
@@ -277,10 +277,22 @@ int ilbm_amr_boxes(ilbm_amr* A, int* box_lo, int* box_hi, int* n_regrids);
  * [0, n-1] of desc (the level's cell counts); tau = 3 nu / dx + 1/2.
  * `stream` is a hipStream_t (NULL: the default stream).
  * ------------------------------------------------------------------------ */
+typedef enum { ILBM_FAB_F64 = 0, ILBM_FAB_F32 = 1 } ilbm_fab_dtype;
+
+/* amrex::Array4: element (i, j, k, n) is at
+ * p[(i - lo0) + (j - lo1) jstride + (k - lo2) kstride + n nstride].
+ * Strides of 0 mean the dense Array4 strides of the box (x fastest, one
+ * contiguous array per component); a component alias (MultiFab make_alias)
+ * is p advanced by that component's offset with ncomp = 1.  The per-box
+ * path computes in FP64 (amrex::Real of the reference's PRECISION=DOUBLE
+ * build) and takes ILBM_FAB_F64 fabs; other dtypes are refused with
+ * ILBM_ERR_UNSUPPORTED. */
 typedef struct {
-  double* p;          /* device pointer to component 0 at cell lo */
+  void* p;            /* device pointer to component 0 at cell lo */
   int lo[3], hi[3];   /* fab box (valid box grown by the ghost cells) */
   int ncomp;
+  int dtype;          /* ilbm_fab_dtype */
+  int64_t jstride, kstride, nstride;   /* elements; 0: dense */
 } ilbm_fab;
 
 /* InitEquilibrium (AmrCoreLBM.cpp:2117-2164): f = feq(rho, u) on [lo, hi] */
@@ -327,6 +339,66 @@ int ilbm_device_synchronize(void);
  * the force integral of ComputeIBForce), fixed order; synchronizes */
 int ilbm_box_sum(const ilbm_fab* fab, int comp, const int lo[3], const int hi[3], double* host_out,
                  void* stream);
+/* ------------------------------------------------------------------------
+ * Coarse-fine transfer between two levels at refinement ratio 2, on
+ * caller-owned fabs (level indices; the fine box keeps its ghost cells
+ * inside the fine domain, so every fine ghost is a coarse-fine ghost).
+ * ------------------------------------------------------------------------ */
+/* FillPatchMesoscopic / FillPatchMacro for lev > 0 (AmrCoreLBM.cpp:
+ * 1545-1627, 2180-2197): every cell of the fine fab outside
+ * [valid_lo, valid_hi] gets the coarse macros at `time` -- GetDataMacro's
+ * choice of crse_old (at t_old), crse_new (at t_new) or their linear mix
+ * (:2239-2269) -- interpolated by cell_cons_interp over `ncomp` components
+ * with one limiter per direction; macro_fine (may be NULL) receives them,
+ * f_fine (may be NULL) receives their equilibrium (:1605-1620).  desc is the
+ * fine level (its n, lattice). */
+int ilbm_box_cf_fill(const ilbm_case_desc* desc, const ilbm_fab* crse_old, const ilbm_fab* crse_new, double t_old,
+                     double t_new, double time, ilbm_fab* f_fine, ilbm_fab* macro_fine, int ncomp,
+                     const int valid_lo[3], const int valid_hi[3], void* stream);
+/* InterpFromCoarseLevel of MakeNewLevelFromCoarse / FillCoarsePatchMacro
+ * (AmrCoreLBM.cpp:751-830, 2214-2237): cell_cons_interp of crse over the
+ * fine cells [lo, hi] */
+int ilbm_box_interp_from_coarse(const ilbm_case_desc* desc, const ilbm_fab* crse, ilbm_fab* macro_fine, int ncomp,
+                                const int lo[3], const int hi[3], void* stream);
+/* AverageDownTo (AmrCoreLBM.cpp:1520-1526): the coarse cells under the fine
+ * box [fine_lo, fine_hi] (whole coarse cells) get the mean of their 2^dim
+ * children, f (q comps) and macros (nmac comps) */
+int ilbm_box_average_down(const ilbm_case_desc* desc, const ilbm_fab* f_fine, const ilbm_fab* macro_fine,
+                          const int fine_lo[3], const int fine_hi[3], ilbm_fab* f_crse, ilbm_fab* macro_crse,
+                          int nmac, void* stream);
+
+/* ------------------------------------------------------------------------
+ * IBMarkerDF on caller-owned fabs (Source/IBM/IBMarkerDF.H:68-91).
+ * ------------------------------------------------------------------------ */
+typedef struct ilbm_ib ilbm_ib;
+/* IBMarkerDF(geom, dm, ba, x0, y0, z0, R, par) + build_markers
+ * (IBMarkerDF.cpp:186-497): desc is the level (n = its domain, dx, BCs, the
+ * ibm.* parameters); [valid_lo, valid_hi] its box (NULL: the domain).  A box
+ * inside the domain is a refined level's: its faces are coarse-fine and the
+ * marker stencils must stay inside it (ILBM_ERR_UNSUPPORTED otherwise). */
+int ilbm_ib_create(const ilbm_case_desc* desc, const int valid_lo[3], const int valid_hi[3], ilbm_ib** out);
+int ilbm_ib_destroy(ilbm_ib* ib);
+/* update_forcing(rhocc, ucc, vcc, wcc, Fx, Fy, Fz, dt, time)
+ * (IBMarkerDF.cpp:1522-1917): reads rho, u, v, w (one component each; w
+ * may be NULL in 2D) on the band around the marker stencils, as
+ * FillPatchMacro leaves them, and writes F = dt * rho * EulerForceTotal on
+ * every cell of the valid box of Fx, Fy, Fz (Fz may be NULL in 2D), zero
+ * outside the band.  Periodic images are taken from the valid cells and
+ * physical ghost values from the case's BCs (macroBcFill), which is what
+ * FillPatchMacro puts there.  The reference's prev_F* arguments are unused
+ * by it and have no counterpart. */
+int ilbm_ib_update_forcing(ilbm_ib* ib, const ilbm_fab* rho, const ilbm_fab* u, const ilbm_fab* v,
+                           const ilbm_fab* w, ilbm_fab* fx, ilbm_fab* fy, ilbm_fab* fz, double dt, double time,
+                           void* stream);
+int ilbm_ib_num_markers(ilbm_ib* ib, int64_t* out);          /* num_markers */
+int ilbm_ib_force_sum(ilbm_ib* ib, double out[3]);           /* last_marker_force_sum */
+/* sum over markers of the moments r x (F dv) about the body centre that the
+ * last spreading stored (IBMarkerDF.cpp:1500-1503; stored, never reduced,
+ * by the reference) */
+int ilbm_ib_torque(ilbm_ib* ib, double out[3]);
+/* positions, U and F per marker, [nm][3] each (any may be NULL) */
+int ilbm_ib_marker_state(ilbm_ib* ib, double* xyz, double* uvw, double* fxyz);
+
 /* ilbm_box_sum without any allocation: no counterpart in the reference; for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same kernels and

@@ -57,6 +57,15 @@ def signal(p):
     return h
 
 
+def recorded(fn):
+    """Force history [n][23] of a recorded series stored as [n][5] columns
+    step, time, Cd, Cl, t_conv."""
+    r = np.load(os.path.join(HERE, fn))
+    h = np.zeros((len(r), 23))
+    h[:, 0], h[:, 1], h[:, 4], h[:, 5], h[:, 22] = r[:, 0], r[:, 1], r[:, 2], r[:, 3], r[:, 4]
+    return h
+
+
 CASES = [
     dict(name="sinusoid_st0165", n=20000, D=32.0, U=0.05, st=0.165, amp=0.3, harm=0.05),
     dict(name="noisy_st0180", n=12000, D=32.0, U=0.05, st=0.18, amp=0.25, noise=0.08, seed=7),
@@ -103,9 +112,9 @@ def main():
     tmp = os.path.join(HERE, "_tmp_force.dat")
     for p in cases:
         if "recorded" in p:
-            h = np.load(os.path.join(HERE, p["recorded"]))
-            meta = json.load(open(os.path.join(HERE, p["recorded"][:-4] + ".json")))
-            p.update(meta)
+            h = recorded(p["recorded"])
+            with open(os.path.join(HERE, p["recorded"][:-4] + ".json")) as f:
+                p.update(json.load(f))
         else:
             h = signal(p)
         ff.write_force_dat(tmp, h, 2)
@@ -116,7 +125,7 @@ def main():
         st_peak, nper, msg_peak = af.estimate_st_peak(times, cls, p["D"], p["U"], 0.4, 0.4)
         st_fft, msg_fft = af.estimate_st_fft(times, cls, p["D"], p["U"], 0.05, 0.5)
         summary, text = ({}, "") if len(rows) < 20 else run_reference_main(af, tmp, p["U"], p["D"])
-        vectors.append({"params": {k: v for k, v in p.items() if k != "recorded"} if "recorded" not in p else p,
+        vectors.append({"params": p,
                         "rows": len(rows),
                         "st_peak": st_peak, "periods": nper, "peak_ok": msg_peak == "",
                         "st_fft": st_fft, "fft_ok": msg_fft == "",

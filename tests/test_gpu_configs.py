@@ -1,7 +1,8 @@
 """The benchmark workloads at their stated sizes on the GPU.
 
 * Config 1: D2Q9 cylinder 512x256, D = 32 at (128, 128), 101 markers, FP64:
-  bit-exact against the oracle for 50 steps (explicit), IVC within 1e-10.
+  bit-exact against the oracle for 50 steps (explicit), IVC within 1e-10;
+  Cd and St of the force history against the reference's acceptance bands.
 * Config 2: D3Q19 sphere 256x128x128, D = 24, 1810 markers: FP64 bit-exact
   against the oracle for 2 steps; FP32 and FP16-shifted storage against
   the FP64 run over 1000 steps (velocity field, Cd, Cl).
@@ -43,6 +44,54 @@ def test_config1_cylinder_ivc_within_tolerance():
     _, _, ofx = orc.markers()
     assert np.abs(fx - ofx).max() <= 1e-10 * np.abs(ofx).max()
     assert abs(lev.force_history()[-1, 4] - orc.force_record()[4]) <= 1e-10 * abs(orc.force_record()[4])
+
+
+def _analyse(fh, tmp_path, name):
+    """force.dat of a GPU history, read back and analysed as
+    Examples/Cylinder_flow/analyze_force.py does (t_conv time base, first
+    25 % dropped)."""
+    from ilbmrex_amd import forcefile as ff
+    path = tmp_path / f"{name}.dat"
+    ff.write_force_dat(path, fh, 2)
+    return ff.analyze(ff.read_force(path), 0.05, 32.0)
+
+
+def test_config1_symmetric_start_keeps_a_steady_wake(tmp_path):
+    """Config 1 as specified (cylinder centred at (128, 128), 20,000 steps):
+    the mirror-symmetric start sheds no vortices within the run (lift at
+    rounding level), and the steady drag is inside the reference's band."""
+    from ilbmrex_amd.native import Level
+    c = config1()
+    lev = Level(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)))
+    lev.advance(20000)
+    fh = lev.force_history()
+    lev.close()
+    a = _analyse(fh, tmp_path, "config1_centred")
+    print(f"config1 centred: {a}")
+    assert a["match_cd"] and abs(a["cl_rms"]) < 1e-10 and a["st"] is None
+
+
+def test_config1_cd_st_within_reference_bands(tmp_path):
+    """Config 1's cylinder with the centre offset of the reference's own
+    example (y0 = 127.9, Examples/Cylinder_flow/inputs) sheds; after 40,000
+    steps the force history, written as force.dat and analysed like
+    analyze_force.py, has Cd in [1.20, 1.80] and St in [0.14, 0.20]
+    (analyze_force.py:148-152).  FP32 storage: mean Cd within 0.5 % of
+    FP64, St within 2 %."""
+    from ilbmrex_amd.native import Level
+    res = {}
+    for st in (cf.STORE_F64, cf.STORE_F32):
+        c = config1(storage=st).replace(y0=127.9)
+        print(f"[config1] offset cylinder, storage {st}, 40000 steps", flush=True)
+        lev = Level(c, cf.uniform_init(c, u=(0.05, 0.0, 0.0)))
+        lev.advance(40000)
+        res[st] = _analyse(lev.force_history(), tmp_path, f"config1_{st}")
+        lev.close()
+        print(f"storage {st}: {res[st]}")
+    a64, a32 = res[cf.STORE_F64], res[cf.STORE_F32]
+    assert a64["match_cd"] and a64["match_st"] and a64["periodic_lift"]
+    assert abs(a32["cd_mean"] - a64["cd_mean"]) <= 5e-3 * abs(a64["cd_mean"])
+    assert a32["st"] is not None and abs(a32["st"] - a64["st"]) <= 0.02 * a64["st"]
 
 
 def config2(storage):

@@ -74,3 +74,24 @@ def test_overlap_sphere_off_walls_split_advances():
     np.testing.assert_allclose(ma, mb, rtol=1e-13)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("regions", ["three", "two"])
+def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
+    """70 overlapped steps (the Sum(Rho) ring halves fill and are reduced
+    twice, the alternating far / ring signals run through every parity)
+    against the single-stream step: bit-identical state and force history.
+    "three": far cells, near box and the ring between them as three
+    launches; "two": ILBM_OVERLAP_TWO_REGIONS=1 keeps far + near."""
+    from ilbmrex_amd.native import Level
+    monkeypatch.delenv("ILBM_OVERLAP_TWO_REGIONS", raising=False)
+    if regions == "two":
+        monkeypatch.setenv("ILBM_OVERLAP_TWO_REGIONS", "1")
+    c = cf.sphere_3d(nx=128, ny=96, nz=96, D=10.0, center=(48.0, 47.6, 48.3), U=0.04, Re=50.0,
+                     storage=cf.STORE_F32)
+    init = cf.uniform_init(c, u=(0.04, 0.0, 0.0))
+    print(f"[overlap] sphere 128x96x96 FP32, 70 steps, {regions} regions", flush=True)
+    (f1, m1, h1, s1), (f2, m2, h2, s2) = _pair(c, init, 70)
+    assert np.array_equal(f1, f2) and np.array_equal(m1, m2) and np.array_equal(h1, h2)
+    assert len(s1) == 70
+    np.testing.assert_allclose(s1, s2, rtol=1e-12)

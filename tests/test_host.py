@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(native.EXPORTED) <= set(names)
-    assert lib.ilbm_abi_version() == 2
+    assert lib.ilbm_abi_version() == 3
 
 
 def test_case_desc_layout_matches_binding():
@@ -165,3 +165,16 @@ def test_serving_workloads_are_seeded_and_in_range():
             assert 32 <= r.case.n[0] <= 192 and r.case.n[0] == r.case.n[1] == r.case.n[2]
         else:
             assert 64 <= r.case.n[0] <= 1024 and r.case.n[0] == r.case.n[1]
+
+
+def test_fab_descriptor_checks_without_gpu():
+    """Per-box entry points refuse non-FP64 fabs and overlapping strides
+    before any device work."""
+    from ilbmrex_amd import native
+    L = native.load()
+    f = native.make_fab(0x1000, (0, 0, 0), (7, 5, 0), 2)
+    f.dtype = native.FAB_F32
+    assert L.ilbm_box_setval(C.byref(f), 0.0, None) == 2          # unsupported
+    g = native.make_fab(0x1000, (0, 0, 0), (7, 5, 0), 2, strides=(4, 0, 0))
+    assert L.ilbm_box_setval(C.byref(g), 0.0, None) == 1          # rows overlap
+    assert C.sizeof(native.Fab) == 8 + 4 * 8 + 3 * 8
