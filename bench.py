@@ -141,10 +141,20 @@ def cpu_baseline(steps):
     wall = time.perf_counter() - t0
     orc.close()
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    model = "unknown CPU"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": case.ncells * steps / wall / 1e6, "unit": "MLUPS", "cores": threads,
-            "kind": "port",
+            "kind": "port", "cpu": model, "wall_s": round(wall, 2),
             "sample": f"{steps} steps (after 1 untimed) of the same 256x128x128 D3Q19 sphere case, "
-                      f"FP64, oracle/refcpu.c with OpenMP ({threads} threads)"}
+                      f"FP64, oracle/refcpu.c with OpenMP ({threads} threads; collision, streaming "
+                      f"and ghost fills parallel, the IB stages serial)"}
 
 
 def amr_variant(steps):
@@ -215,17 +225,27 @@ def batched_section(args):
         st["rate_per_s"] = rate
         sweep.append({k: (round(v, 4) if isinstance(v, float) else v) for k, v in st.items()})
     out["serving_f32"] = sweep
-    # the same short stream through both paths
-    rate = args.rates[-1]
-    reqs = serving.serving_requests(16, rate=rate, seed=7, steps=(200, 2000))
-    log(f"[bench] serving: 16 short requests at {rate}/s, batched FP32 vs plain FP64")
-    b = serving.serve(reqs, budget, chunk=args.chunk)
-    reqs = serving.serving_requests(16, rate=rate, seed=7, steps=(200, 2000))
-    p = serving.serve_plain(reqs, storage=cf.STORE_F64)
-    out["serving_short_stream"] = {
-        "requests": 16, "rate_per_s": rate, "steps": [200, 2000],
+    # config 5's own stream (2k-20k steps, 70/30 cylinder / sphere) through
+    # both paths: the batched FP32 server and the plain per-case FP64 path
+    rate = args.plain_rate
+    log(f"[bench] serving: {args.requests} requests at {rate}/s, batched FP32 vs plain FP64 on the same stream")
+    reqs_b = serving.serving_requests(args.requests, rate=rate, seed=20261202)
+    b = serving.serve(reqs_b, budget, chunk=args.chunk, keep_history=True)
+    reqs_p = serving.serving_requests(args.requests, rate=rate, seed=20261202, storage=cf.STORE_F64)
+    p = serving.serve_plain(reqs_p, storage=cf.STORE_F64, keep_history=True)
+    cd_err = []
+    for rb, rp in zip(reqs_b, reqs_p):
+        tail = max(1, rb.nsteps // 5)
+        c32, c64 = rb.history[-tail:, 4].mean(), rp.history[-tail:, 4].mean()
+        cd_err.append(abs(c32 - c64) / abs(c64))
+    out["serving_config5_stream"] = {
+        "requests": args.requests, "rate_per_s": rate, "steps": [2000, 20000], "mix": "70% D2Q9 cylinders, 30% D3Q19 spheres",
         "batched_f32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()},
-        "plain_f64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()}}
+        "plain_f64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()},
+        "throughput_ratio": round(b["mlups"] / p["mlups"], 3),
+        "cd_err_f32_vs_f64_max": float(max(cd_err)),
+        "cd_err_note": "relative error of each response's mean Cd over its last 20 % of steps, batched FP32 "
+                       "against the same request on the plain FP64 path; max over the requests"}
     return out
 
 
@@ -249,7 +269,7 @@ def main():
     ap.add_argument("--roofline-warmup", type=int, default=200)
     ap.add_argument("--accuracy-steps", type=int, default=10000,
                     help="steps of the sphere in FP64 / FP32 / FP16-shifted storage for the error report")
-    ap.add_argument("--cpu-steps", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=20)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
@@ -258,6 +278,8 @@ def main():
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])
     ap.add_argument("--chunk", type=int, default=200)
+    ap.add_argument("--plain-rate", type=float, default=8.0,
+                    help="arrival rate of the config-5 stream served by both the batched and the plain FP64 path")
     ap.add_argument("--no-overlap", action="store_true",
                     help="IB levels step on one stream (profiler runs of the whole-level kernel)")
     ap.add_argument("--workload", choices=["sphere", "ensemble", "amr"], default="sphere",

@@ -118,19 +118,28 @@ def run_ensemble(cases: List[cf.Case], nsteps: int, budget: int, chunk: int = 1 
 
 
 def serve(requests: List[Request], budget: int, chunk: int = 200, graph_steps: int = 8,
-          max_cases: int = 4096, keep_history: bool = False, clock=time.perf_counter):
-    """Continuous batching over arrival times given in seconds from the start."""
+          max_cases: int = 4096, keep_history: bool = False, clock=time.perf_counter, min_chunk: int = 16):
+    """Continuous batching over arrival times given in seconds from the start.
+
+    Between advances the loop retires finished cases and admits every
+    arrived request that fits the budget.  An advance runs at most `chunk`
+    steps, and fewer (down to `min_chunk`) when the next request is due
+    sooner: the steps until its arrival are estimated from the measured
+    time per step of the resident set, so an arrival waits about one short
+    chunk instead of a whole one."""
     from .native import Batch, BatchFull
     pending = deque(sorted(requests, key=lambda r: r.arrival))
     waiting: deque = deque()
     active = {}
     lups = 0.0
+    step_s = None          # seconds per step of the current resident set
     with Batch(budget, max_cases=max_cases, graph_steps=graph_steps) as B:
         t0 = clock()
         while pending or waiting or active:
             now = clock() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.popleft())
+            admitted = False
             while waiting:
                 r = waiting[0]
                 try:
@@ -142,11 +151,21 @@ def serve(requests: List[Request], budget: int, chunk: int = 200, graph_steps: i
                 waiting.popleft()
                 r.admit = clock() - t0
                 active[cid] = r
+                admitted = True
+            if admitted:
+                step_s = None
             if not active:
                 if pending:
                     time.sleep(max(0.0, min(pending[0].arrival - (clock() - t0), 0.01)))
                 continue
-            B.advance(chunk)
+            n = chunk
+            if pending and step_s:
+                due = pending[0].arrival - (clock() - t0)
+                n = int(min(chunk, max(min_chunk, due / step_s)))
+            ta = clock()
+            done = B.advance(n)
+            if done > 0:
+                step_s = (clock() - ta) / done
             for cid in B.finished():
                 r = active.pop(cid)
                 r.finish = clock() - t0
@@ -163,7 +182,8 @@ def serve(requests: List[Request], budget: int, chunk: int = 200, graph_steps: i
             "captures": info["captures"]}
 
 
-def serve_plain(requests: List[Request], storage: int = cf.STORE_F64, clock=time.perf_counter):
+def serve_plain(requests: List[Request], storage: int = cf.STORE_F64, clock=time.perf_counter,
+                keep_history: bool = False):
     """The plain per-case path as a server: requests run one at a time, in
     arrival order, each on its own Level with the given storage."""
     from .native import Level
@@ -179,7 +199,7 @@ def serve_plain(requests: List[Request], storage: int = cf.STORE_F64, clock=time
         lev = Level(c, initial_state(c))
         lev.advance(r.nsteps)
         lev.synchronize()
-        r.history = None
+        r.history = lev.force_history() if keep_history else None
         lev.close()
         r.finish = clock() - t0
         lups += c.ncells * r.nsteps

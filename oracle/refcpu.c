@@ -300,6 +300,9 @@ static int nonper_out(const rc_level *L, const int iv[3]) {
  * then 3 dimensions).  is_meso selects mesoscopicBcFill vs macroBcFill. */
 static void fill_patch(rc_level *L, fab *dst, const fab *src, int ncomp, int is_meso) {
   const int dim = L->c.dim;
+  /* every cell writes only itself, from src (first loop) or from cells of
+   * an earlier pass (the later loops): the loops run in parallel */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = dst->lo[2]; k <= dst->hi[2]; ++k)
     for (int j = dst->lo[1]; j <= dst->hi[1]; ++j)
       for (int i = dst->lo[0]; i <= dst->hi[0]; ++i) {
@@ -309,7 +312,8 @@ static void fill_patch(rc_level *L, fab *dst, const fab *src, int ncomp, int is_
         for (int d = 0; d < dim; ++d) s[d] = wrapc(L, d, iv[d]);
         for (int n = 0; n < ncomp; ++n) AT(*dst, i, j, k, n) = AT(*src, s[0], s[1], s[2], n);
       }
-  for (int pass = 1; pass <= dim; ++pass)
+  for (int pass = 1; pass <= dim; ++pass) {
+#pragma omp parallel for collapse(2) schedule(static)
     for (int k = dst->lo[2]; k <= dst->hi[2]; ++k)
       for (int j = dst->lo[1]; j <= dst->hi[1]; ++j)
         for (int i = dst->lo[0]; i <= dst->hi[0]; ++i) {
@@ -319,11 +323,14 @@ static void fill_patch(rc_level *L, fab *dst, const fab *src, int ncomp, int is_
           if (is_meso) meso_user(L, dst, iv);
           else macro_user(L, dst, iv, ncomp);
         }
+  }
 }
 
 /* FillBoundary: ghost cells whose periodic image is valid get that value. */
 static void fill_boundary(rc_level *L, fab *F, int ncomp) {
   const int dim = L->c.dim;
+  /* ghost cells take values of valid cells only: parallel */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = F->lo[2]; k <= F->hi[2]; ++k)
     for (int j = F->lo[1]; j <= F->hi[1]; ++j)
       for (int i = F->lo[0]; i <= F->hi[0]; ++i) {
@@ -1057,7 +1064,8 @@ static void advance_level(rc_level *L, double time, double dt_lev) {
       }
   fill_boundary(L, &sfS, Q);
 
-  /* bounce-back ghost fill, :265-373 */
+  /* bounce-back ghost fill, :265-373 (ghosts from valid cells: parallel) */
+#pragma omp parallel for collapse(2) schedule(static)
   for (int k = sfS.lo[2]; k <= sfS.hi[2]; ++k)
     for (int j = sfS.lo[1]; j <= sfS.hi[1]; ++j)
       for (int i = sfS.lo[0]; i <= sfS.hi[0]; ++i) {
