@@ -86,12 +86,20 @@ static void pick_class(int lat, int st, int am, StepBFn& sf, GhostBFn& gf, FixBF
   else pk(D3Q27{});
 }
 
+// Tiling of a case's batched stream-collide launch: tile width (0: linear
+// tiles of consecutive cells) and cells per thread.
+struct Tile {
+  int bx = 64;
+  int nc = 1;
+};
+
 struct BatchCase {
   int id = -1;
   std::unique_ptr<Level> L;
   int target = 0, done = 0;
   int64_t bytes = 0;
   int tile_bx = 64;
+  int tile_nc = 1;
   int base_cur = 0;         // L->cur when the batch tables were built
 };
 
@@ -166,8 +174,8 @@ static int64_t case_footprint(const ilbm_case_desc& c0, int nsteps) {
 // Autotuned tile width per (lattice, storage, layout, x-extent bucket),
 // shared by every batch of the process so a case gets the same tiling
 // whichever batch it joins.
-static std::map<int64_t, int>& tile_table() {
-  static std::map<int64_t, int> t;
+static std::map<int64_t, Tile>& tile_table() {
+  static std::map<int64_t, Tile> t;
   return t;
 }
 
@@ -251,31 +259,58 @@ struct Batch {
     return b;
   }
 
-  // Tile width of the stream-collide launch for a case of this class and
-  // x extent: linear tiles (0) and the widths 64 / 128 / 256 are timed on a
-  // scratch level of the case's own shape (no IB) and the fastest is kept
-  // for the process, so every later case of the bucket uses the same tiling.
-  int tile_for(const Level& ref) {
-    // ILBM_BATCH_TILE=0|64|128|256 fixes the tiling (tests of each path)
+  // Tiling of the stream-collide launch for a case of this class and x
+  // extent: linear tiles (0) and the widths 64 / 128 / 256, with one and
+  // (2D) two cells per thread, are timed on a scratch level of the case's x
+  // extent (no IB) and the fastest is kept for the process, so every later
+  // case of the bucket uses the same tiling.
+  Tile tile_for(const Level& ref) {
+    // ILBM_BATCH_TILE=0|64|128|256 and ILBM_BATCH_NC=1|2 fix the tiling
+    // (tests of each path); a fixed width alone keeps one cell per thread
+    Tile fixed{};
+    bool have_fixed = false;
     if (const char* fix = std::getenv("ILBM_BATCH_TILE")) {
       const int t = std::atoi(fix);
-      if (t == 0 || t == 64 || t == 128 || t == 256) return t;
+      if (t == 0 || t == 64 || t == 128 || t == 256) { fixed.bx = t; have_fixed = true; }
     }
-    const int64_t key = (((int64_t)ref.lat * 4 + ref.st) * 4 + ref.am) * 16 + bucket(ref.c.n[0]);
+    int fix_nc = 0;
+    if (const char* fn = std::getenv("ILBM_BATCH_NC")) {
+      const int v = std::atoi(fn);
+      if (v == 1 || v == 2) fix_nc = v;
+    }
+    if (ref.c.dim == 3) fix_nc = 0;   // one cell per thread on 3D lattices
+    if (have_fixed) {
+      fixed.nc = fix_nc ? fix_nc : 1;
+      return fixed;
+    }
+    const int64_t key = ((((int64_t)ref.lat * 4 + ref.st) * 4 + ref.am) * 16 + bucket(ref.c.n[0])) * 4 + fix_nc;
     auto& tiles = tile_table();
     auto it = tiles.find(key);
     if (it != tiles.end()) return it->second;
-    std::vector<int> cand{0};
-    for (int bx = 64; bx <= 256; bx *= 2) {
-      cand.push_back(bx);
-      if (bx >= ref.c.n[0]) break;
+    std::vector<Tile> cand;
+    for (int nc = 1; nc <= (ref.c.dim == 2 ? 2 : 1); ++nc) {
+      if (fix_nc && nc != fix_nc && ref.c.dim == 2) continue;
+      cand.push_back(Tile{0, nc});
+      for (int bx = 64; bx <= 256; bx *= 2) {
+        cand.push_back(Tile{bx, nc});
+        if (bx >= ref.c.n[0]) break;
+      }
     }
-    int best = cand[0];
+    Tile best = cand[0];
     if (cand.size() > 1) {
+      // the batched launch is bound by throughput, not by one case's
+      // latency: the scratch level keeps the case's x extent and grows in
+      // its outer dimension to at least 2^22 cells
       Level T;
       T.c = ref.c;
       T.c.use_ib = 0;
       T.c.use_prescribed_force = 0;
+      {
+        const int outer = (T.c.dim == 3) ? 2 : 1;
+        const int64_t plane = (int64_t)ref.ncells / std::max(1, T.c.n[outer]);
+        const int64_t want = ((int64_t)1 << 22) / std::max<int64_t>(1, plane);
+        if (want > T.c.n[outer]) T.c.n[outer] = (int)std::min<int64_t>(want, 1 << 16);
+      }
       T.stream = stream;
       T.own_stream = false;
       T.validate();
@@ -284,7 +319,7 @@ struct Batch {
       std::fill(m0.begin(), m0.begin() + T.ncells, 1.0);
       T.init(m0.data());
       int64_t maxblk = 0;
-      for (int bx : cand) maxblk = std::max(maxblk, T.tile_blocks(bx));
+      for (const Tile& t : cand) maxblk = std::max(maxblk, T.tile_blocks(t.bx, t.nc));
       T.ensure_partial(maxblk);
       StepBFn sf; GhostBFn gf; FixBFn ff, fw;
       pick_class(T.lat, T.st, T.am, sf, gf, ff, fw);
@@ -294,10 +329,11 @@ struct Batch {
       HIPCHK(hipEventCreate(&e0));
       HIPCHK(hipEventCreate(&e1));
       double best_ms = 1e30;
-      for (int bx : cand) {
+      for (const Tile& t : cand) {
         StepArgs a = T.step_args(false);
-        a.tile_bx = bx;
-        const int64_t nb = T.tile_blocks(bx);
+        a.tile_bx = t.bx;
+        a.tile_nc = t.nc;
+        const int64_t nb = T.tile_blocks(t.bx, t.nc);
         std::vector<BlockRef> m(nb);
         for (int64_t b = 0; b < nb; ++b) m[b] = BlockRef{0, (int)b};
         HIPCHK(hipMemcpyAsync(da, &a, sizeof(a), hipMemcpyHostToDevice, stream));
@@ -310,7 +346,7 @@ struct Batch {
         HIPCHK(hipEventSynchronize(e1));
         float ms = 0.f;
         HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        if (ms < best_ms) { best_ms = ms; best = bx; }
+        if (ms < best_ms) { best_ms = ms; best = t; }
       }
       (void)hipEventDestroy(e0);
       (void)hipEventDestroy(e1);
@@ -319,6 +355,15 @@ struct Batch {
     }
     tiles[key] = best;
     return best;
+  }
+
+  // A case whose stream-collide kernel writes its own ghost slabs gets them
+  // from the boundary layer once, for its first batched step.
+  void prime_slabs(Level& L) {
+    if (L.wrap || !L.gfuse) return;
+    const int64_t npad = L.slabs.face_off[L.slabs.nface];
+    L.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, L.ghost_args());
+    HIPCHK(hipGetLastError());
   }
 
   // ---------------- admission ----------------
@@ -339,8 +384,11 @@ struct Batch {
     L.setup_layout();
     L.setup_ib();
     L.init(init);
-    bc.tile_bx = tile_for(L);
-    L.ensure_partial(L.tile_blocks(bc.tile_bx));
+    prime_slabs(L);
+    const Tile t = tile_for(L);
+    bc.tile_bx = t.bx;
+    bc.tile_nc = t.nc;
+    L.ensure_partial(L.tile_blocks(t.bx, t.nc));
     L.grow_histories(nsteps, nsteps);
     const double clk[4] = {L.t_new, (double)L.istep, (double)L.istep, 0.0};
     HIPCHK(hipMemcpyAsync(L.clock_d, clk, sizeof(clk), hipMemcpyHostToDevice, stream));
@@ -376,8 +424,11 @@ struct Batch {
     std::fill(m0.begin(), m0.begin() + L.ncells, 1.0);
     L.init(m0.data());
     L.restore(buf, size);
-    bc.tile_bx = tile_for(L);
-    L.ensure_partial(L.tile_blocks(bc.tile_bx));
+    prime_slabs(L);
+    const Tile t = tile_for(L);
+    bc.tile_bx = t.bx;
+    bc.tile_nc = t.nc;
+    L.ensure_partial(L.tile_blocks(t.bx, t.nc));
     L.grow_histories(nsteps, nsteps);
     HIPCHK(hipStreamSynchronize(stream));
     bc.id = next_id++;
@@ -479,6 +530,7 @@ struct Batch {
         L.cur = bc.base_cur ^ p;
         sa[p][ci] = L.step_args(false);
         sa[p][ci].tile_bx = bc.tile_bx;
+        sa[p][ci].tile_nc = bc.tile_nc;
         ga[p][ci] = L.ghost_args();
         IbCase ic{};
         if (L.ib) {
@@ -497,10 +549,10 @@ struct Batch {
         ia[p][ci] = ic;
       }
       L.cur = bc.base_cur;
-      const int64_t nb = L.tile_blocks(bc.tile_bx);
+      const int64_t nb = L.tile_blocks(bc.tile_bx, bc.tile_nc);
       tails[ci] = CaseTail{L.partial, nb, L.mass_dev, L.clock_d, L.dt};
       for (int64_t b = 0; b < nb; ++b) maps[BatchGroup::M_STEP].push_back({c, (int)b});
-      if (!L.wrap) {
+      if (!L.wrap && !L.gfuse) {   // else the stream-collide kernel writes the slabs
         const int64_t npad = L.slabs.face_off[L.slabs.nface];
         for (int64_t b = 0; b < (npad + kBlock - 1) / kBlock; ++b) maps[BatchGroup::M_GHOST].push_back({c, (int)b});
       }

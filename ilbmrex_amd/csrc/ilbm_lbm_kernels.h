@@ -154,6 +154,9 @@ __host__ __device__ inline int64_t slab_index(const Slabs& S, const int c[3], in
   return t;
 }
 
+// boundary condition types (AMReX BCType and the reference's user kinds)
+enum { BC_INT_DIR = 0, BC_FOEXTRAP = 2, BC_EXT_DIR = 3, BC_USER_1 = 1001, BC_USER_2 = 1002 };
+
 struct StepArgs {
   GridGeom g;
   Slabs slabs;
@@ -172,12 +175,20 @@ struct StepArgs {
   int nonper[3];
   double* partial;       // per-block mass partials
   int tile_bx;           // tile width of the batched launch (block = tile_bx x kBlock/tile_bx)
+  int tile_nc;           // cells per thread of the batched launch (1 or 2)
+  // resident blocks of a persistent launch (k_step_p; 0: one block per tile)
+  int persist;
   int band_force;        // the band carries the IB force (else it only exports macros)
   // sub-box split of a level: 0 every cell; 1 the cells outside the box;
   // 2 the cells of the box, the launch grid covering the box from sub_lo;
   // 3 as 2 without the cells of the inner box (in_lo, in_n)
   int sub_mode;
   int sub_lo[3], sub_n[3];
+  // x-face ghost slabs written by k_step itself (see ghost_write): the BC
+  // kind of each x side and its boundary values (rho, u, v, w)
+  int gfuse;
+  int gkind[2];
+  double gval[2][4];
   int in_lo[3], in_n[3];
   int off32;             // a plane's cells are addressable with 32-bit byte offsets
   // the uniform parameters rounded once to FP32 on the host, for shifted
@@ -305,6 +316,97 @@ __device__ inline void finish_mass(const StepArgs& a, double s, int64_t blk) {
   if (threadIdx.x == 0 && threadIdx.y == 0) a.partial[blk] = s;
 }
 
+// Ghost value of the ext_dir / user_2 mesoscopic functor (bc_fill.H:303-347)
+// from the pre-collision populations fnp of its mirror cell: the
+// equilibrium of the ghost's macros plus the mirror's non-equilibrium part.
+// ext_dir imposes the boundary values bv = (rho, u, v, w); user_2 reflects
+// the mirror's macros about them.
+template <class LAT, class T>
+__device__ __forceinline__ void neep_ghost(int kind, const T* fnp, const double* bv, T* out) {
+  T rhonp = T(0), unp = T(0), vnp = T(0), wnp = T(0);
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    rhonp += fnp[q];
+    unp += T(LAT::cx[q]) * fnp[q];
+    vnp += T(LAT::cy[q]) * fnp[q];
+    wnp += T(LAT::cz[q]) * fnp[q];
+  }
+  rhonp = rhonp > T(1.0e-20) ? rhonp : T(1.0e-20);
+  unp /= rhonp; vnp /= rhonp; wnp /= rhonp;
+  const T rho_b = T(bv[0]), u_b = T(bv[1]), v_b = T(bv[2]), w_b = T(bv[3]);
+  T rg = rho_b, ug = u_b, vg = v_b, wg = w_b;
+  if (kind == BC_USER_2) {
+    const T t2 = T(2.0) * rho_b - rhonp;
+    rg = T(1.0e-20) > t2 ? T(1.0e-20) : t2;
+    ug = T(2.0) * u_b - unp; vg = T(2.0) * v_b - vnp; wg = T(2.0) * w_b - wnp;
+  }
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const T cx = T(LAT::cx[q]), cy = T(LAT::cy[q]), cz = T(LAT::cz[q]);
+    const T eu_np = cx * unp + cy * vnp + cz * wnp;
+    const T uv_np = unp * unp + vnp * vnp + wnp * wnp;
+    const T feqnp = T(LAT::w[q]) * rhonp * (T(1.0) + T(3.0) * eu_np + T(4.5) * eu_np * eu_np - T(1.5) * uv_np);
+    const T eu_g = cx * ug + cy * vg + cz * wg;
+    const T uv_g = ug * ug + vg * vg + wg * wg;
+    const T feqg = T(LAT::w[q]) * rg * (T(1.0) + T(3.0) * eu_g + T(4.5) * eu_g * eu_g - T(1.5) * uv_g);
+    out[q] = feqg + (fnp[q] - feqnp);
+  }
+}
+
+// The x-face ghost slabs of the next step, written by the boundary cells
+// themselves (StepArgs::gfuse: levels walled in x only whose x faces are
+// ext_dir, user_2 or foextrap).  Each ghost (-1 or n0, jg, kg) takes its
+// FillPatch value from one boundary cell: for ext_dir / user_2, whose
+// functor clamps the mirror, (0 or n0 - 1, clamp(jg), clamp(kg)); for
+// foextrap (FilccCell, then the periodic copy) the image (wrap(jg),
+// wrap(kg)).  fillpatch_value computes the same values from the boundary
+// layer.  Only the directions the next pull reads from a slab are written:
+// c_x = +1 at the lower face, -1 at the upper.
+template <class LAT, class ST, int SIDE>
+__device__ __forceinline__ void ghost_write(const StepArgs& a, int j, int k, const typename ST::compute_t* f) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  constexpr int CIN = SIDE ? -1 : 1;   // c_x of the directions pulled from this slab
+  const GridGeom& g = a.g;
+  R* out = static_cast<R*>(a.gout) + g.pop_n;
+  const int64_t npad = a.slabs.face_off[a.slabs.nface];
+  const int kind = a.gkind[SIDE];
+  T fnp[LAT::Q];
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) fnp[q] = ST::shifted ? f[q] + T(LAT::w[q]) : f[q];
+  T val[LAT::Q];
+  if (kind == BC_FOEXTRAP) {
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) val[q] = fnp[q];
+  } else {
+    neep_ghost<LAT, T>(kind, fnp, a.gval[SIDE], val);
+  }
+  const bool clamp = kind != BC_FOEXTRAP;
+  // the ghost rows of this cell in y (and z): its own, and the rows -1 and
+  // n that the rule maps onto it
+  const bool jlo = j == (clamp ? 0 : g.n[1] - 1), jhi = j == (clamp ? g.n[1] - 1 : 0);
+  const bool klo = k == (clamp ? 0 : g.n[2] - 1), khi = k == (clamp ? g.n[2] - 1 : 0);
+  const int fs = a.slabs.slab_of[0][SIDE];
+  const int64_t base = a.slabs.face_off[fs];
+  const int flo1 = a.slabs.face_lo[fs][1], flo2 = a.slabs.face_lo[fs][2];
+  const int64_t fn1 = a.slabs.face_n[fs][1];
+#pragma unroll
+  for (int bj = 0; bj < 3; ++bj) {
+    if ((bj == 1 && !jlo) || (bj == 2 && !jhi)) continue;
+    const int jj = (bj == 0) ? j : (bj == 1) ? -1 : g.n[1];
+#pragma unroll
+    for (int bk = 0; bk < (LAT::D == 3 ? 3 : 1); ++bk) {
+      if ((bk == 1 && !klo) || (bk == 2 && !khi)) continue;
+      const int kk = (bk == 0) ? k : (bk == 1) ? -1 : g.n[2];
+      const int64_t t = base + (jj - flo1) + (int64_t)(kk - flo2) * fn1;
+      static_for<LAT::Q>([&](auto qc) {
+        constexpr int q = decltype(qc)::value;
+        if constexpr (LAT::cx[q] == CIN) ST::store(out, (int64_t)q * npad + t, val[q], T(LAT::w[q]));
+      });
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // k_step: pull-stream + forced macros + BGK collision, one pass.
 // ---------------------------------------------------------------------------
@@ -328,16 +430,14 @@ __device__ __forceinline__ void accumulate_signed(T& acc, T x) {
   }
 }
 
-template <class LAT, class ST, int AM>
-__device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k, int64_t blk, double* lds) {
-  using T = typename ST::compute_t;
-  using R = typename ST::raw;
-  constexpr bool WRAP = AM == kAddrWrap;
-  constexpr bool XO = AM == kAddrWallX32;
-  // offsets in 32-bit unsigned arithmetic: every intermediate is exact
-  // modulo 2^32 and every final offset lies below 2^32
-  constexpr bool O32 = AM != kAddrWall64;
-  using I = std::conditional_t<O32, uint32_t, int64_t>;
+// Offsets of the stream-collide kernel: in the 32-bit modes unsigned 32-bit
+// arithmetic (every intermediate is exact modulo 2^32 and every final offset
+// lies below 2^32), else 64-bit element offsets.
+template <int AM>
+using StepOff = std::conditional_t<AM != kAddrWall64, uint32_t, int64_t>;
+
+// the cell takes part in this launch (inside the level and its sub-box part)
+__device__ __forceinline__ bool step_active(const StepArgs& a, int i, int j, int k) {
   const GridGeom& g = a.g;
   bool active = (i < g.n[0]) && (j < g.n[1]);
   if (a.sub_mode != 0) {
@@ -353,259 +453,328 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
       active = active && (a.sub_mode == 2 ? in_sub : !in_sub);
     }
   }
-  double rho_acc = 0.0;
-  if (active) {
-    const R* __restrict__ gin = static_cast<const R*>(a.gin);
-    R* __restrict__ gout = static_cast<R*>(a.gout);
-    // source offsets for c = -1, 0, +1 (source coordinate = x - c); periodic
-    // dimensions wrap.  Shifted storage (bound by vector ALU issue) holds
-    // byte offsets in the 32-bit modes (SB bytes per element), so that no
-    // direction scales its offset again (FP16-shifted 2D cylinder +3 %);
-    // the other formats scale each offset by EB where it is used.
-    constexpr uint32_t SB = (O32 && ST::shifted) ? (uint32_t)sizeof(R) : 1u;
-    constexpr uint32_t EB = O32 ? (uint32_t)sizeof(R) / SB : 1u;
-    I ox[3], oy[3], oz[3];
-    {
-      const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
-      const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
-      const I jsb = (I)g.js * (I)SB;
-      ox[0] = (I)xp * (I)SB; ox[1] = (I)i * (I)SB; ox[2] = (I)xm * (I)SB;
-      oy[0] = (I)yp * jsb; oy[1] = (I)j * jsb; oy[2] = (I)ym * jsb;
-      if constexpr (LAT::D == 3) {
-        const I ksb = (I)g.ks * (I)SB;
-        const int zm = (k == 0) ? g.n[2] - 1 : k - 1, zp = (k == g.n[2] - 1) ? 0 : k + 1;
-        oz[0] = (I)zp * ksb; oz[1] = (I)k * ksb; oz[2] = (I)zm * ksb;
-      } else {
-        oz[0] = oz[1] = oz[2] = 0;
-      }
-    }
-    // A source outside a non-periodic face is a ghost cell; its value (the
-    // pre-collision FillPatch value, or bounce-back) sits in the slabs that
-    // follow the Q population planes of the same buffer.  The BC functor
-    // clamps the mirror in every dimension (bc_fill.H:28-40), so a ghost
-    // that is also outside a periodic dimension is not wrapped.  All loads
-    // are issued unconditionally; only their offsets are selected.
-    bool outx[3] = {false, false, false}, outy[3] = {false, false, false}, outz[3] = {false, false, false};
-    I tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
-    if constexpr (XO) {
-      // both x faces span the same (y, z) range, so their strides are
-      // uniform; only the face's offset differs between the two sides
-      outx[0] = (i == g.n[0] - 1);
-      outx[2] = (i == 0);
-      const int f0 = a.slabs.slab_of[0][0], f1 = a.slabs.slab_of[0][1];
-      const I fn1 = (I)a.slabs.face_n[f0][1];
-      mu[0][1] = (I)SB;
-      mu[0][2] = fn1 * (I)SB;
-      const I base = (i == 0) ? (I)a.slabs.face_off[f0] : (I)a.slabs.face_off[f1];
-      tb[0] = (base + (I)(j - a.slabs.face_lo[f0][1]) + (I)(k - a.slabs.face_lo[f0][2]) * fn1) * (I)SB;
-    } else if constexpr (!WRAP) {
-      const int cc[3] = {i, j, k};
-      bool* outs[3] = {outx, outy, outz};
-#pragma unroll
-      for (int d = 0; d < LAT::D; ++d) {
-        if (!a.nonper[d]) continue;
-        outs[d][0] = (cc[d] == g.n[d] - 1);
-        outs[d][2] = (cc[d] == 0);
-        const int sd = (cc[d] == 0) ? 0 : 1;
-        const int f = a.slabs.slab_of[d][sd];
-        I t = (I)a.slabs.face_off[f], mul = 1;
-#pragma unroll
-        for (int e = 0; e < 3; ++e) {
-          if (e == d) continue;
-          t += (I)(cc[e] - a.slabs.face_lo[f][e]) * mul;
-          mu[d][e] = mul * (I)SB;
-          mul *= (I)a.slabs.face_n[f][e];
-        }
-        tb[d] = t * (I)SB;
-      }
-    }
-    const I gofs = (I)g.pop_n * (I)SB;                  // start of the slabs
-    const I npad = (I)a.slabs.face_off[a.slabs.nface] * (I)SB;
-    T f[LAT::Q];
-#pragma unroll
-    for (int q = 0; q < LAT::Q; ++q) {
-      if constexpr (WRAP) {
-        // all-periodic level with planes under 4 GiB: a 32-bit byte offset
-        // from the plane's (uniform) base
-        const uint32_t src = (uint32_t)(ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
-        const R* plane = gin + (int64_t)q * g.qs;
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * EB);
-        // 3D: non-temporal loads (256^3 FP64 5.66 -> 5.94 TB/s, FP32 128^3
-        // +1 %); on walled and 2D levels they cost 2-9 %
-        if constexpr (LAT::D == 3) f[q] = pop_in_nt<ST>(p, 0, T(LAT::w[q]));
-        else f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
-        continue;
-      }
-      // the source cell inside the level, or for a source beyond a
-      // non-periodic face the slab of the lowest such dimension it leaves
-      // (a per-lane select: one load per direction on every lane)
-      I off = ((I)q * (I)g.qs + (I)g.off0) * (I)SB + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
-      if constexpr (XO) {
-        if (LAT::cx[q] != 0) {
-          const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
-          off = outx[LAT::cx[q] + 1] ? gofs + (I)q * npad + t0 : off;
-        }
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
-        f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
-        continue;
-      }
-      const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
-      const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
-      const I t1 = tb[1] - (I)LAT::cx[q] * mu[1][0] - (I)LAT::cz[q] * mu[1][2];
-      const I t2 = tb[2] - (I)LAT::cx[q] * mu[2][0] - (I)LAT::cy[q] * mu[2][1];
-      const I t = o0 ? t0 : (o1 ? t1 : t2);
-      off = (o0 | o1 | o2) ? gofs + (I)q * npad + t : off;
-      if constexpr (O32) {
-        const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
-        f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
-      } else {
-        f[q] = pop_in<ST>(gin, off, T(LAT::w[q]));
-      }
-    }
-    // f holds f_q, or h_q = f_q - w_q for shifted storage (then the first
-    // moments are the same sums, since sum_q w_q c_q = 0, and rho = 1 +
-    // sum_q h_q)
-    constexpr bool SH = ST::shifted;
-    // moments in the reference's order (calculateMacroForcing.H:21-27); a
-    // term whose direction component is 0 is left out and -1 * f is
-    // subtracted, which leaves every finite sum bit-identical
-    T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
-    if constexpr (ST::exact) {
-      static_for<LAT::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        r += f[q];
-        if constexpr (LAT::cx[q] > 0) m1 += f[q];
-        else if constexpr (LAT::cx[q] < 0) m1 -= f[q];
-        if constexpr (LAT::cy[q] > 0) m2 += f[q];
-        else if constexpr (LAT::cy[q] < 0) m2 -= f[q];
-        if constexpr (LAT::cz[q] > 0) m3 += f[q];
-        else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
-      });
+  return active;
+}
+
+// First half of a cell's update: the pull loads of f^{n+1} (into f) and the
+// cell's own offset (self).
+template <class LAT, class ST, int AM>
+__device__ __forceinline__ void step_load(const StepArgs& a, int i, int j, int k, typename ST::compute_t* f,
+                                          StepOff<AM>& self) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  constexpr bool WRAP = AM == kAddrWrap;
+  constexpr bool XO = AM == kAddrWallX32;
+  constexpr bool O32 = AM != kAddrWall64;
+  using I = StepOff<AM>;
+  const GridGeom& g = a.g;
+  (void)XO;
+  const R* __restrict__ gin = static_cast<const R*>(a.gin);
+  // source offsets for c = -1, 0, +1 (source coordinate = x - c); periodic
+  // dimensions wrap.  Shifted storage (bound by vector ALU issue) holds
+  // byte offsets in the 32-bit modes (SB bytes per element), so that no
+  // direction scales its offset again (FP16-shifted 2D cylinder +3 %);
+  // the other formats scale each offset by EB where it is used.
+  constexpr uint32_t SB = (O32 && ST::shifted) ? (uint32_t)sizeof(R) : 1u;
+  constexpr uint32_t EB = O32 ? (uint32_t)sizeof(R) / SB : 1u;
+  I ox[3], oy[3], oz[3];
+  {
+    const int xm = (i == 0) ? g.n[0] - 1 : i - 1, xp = (i == g.n[0] - 1) ? 0 : i + 1;
+    const int ym = (j == 0) ? g.n[1] - 1 : j - 1, yp = (j == g.n[1] - 1) ? 0 : j + 1;
+    const I jsb = (I)g.js * (I)SB;
+    ox[0] = (I)xp * (I)SB; ox[1] = (I)i * (I)SB; ox[2] = (I)xm * (I)SB;
+    oy[0] = (I)yp * jsb; oy[1] = (I)j * jsb; oy[2] = (I)ym * jsb;
+    if constexpr (LAT::D == 3) {
+      const I ksb = (I)g.ks * (I)SB;
+      const int zm = (k == 0) ? g.n[2] - 1 : k - 1, zp = (k == g.n[2] - 1) ? 0 : k + 1;
+      oz[0] = (I)zp * ksb; oz[1] = (I)k * ksb; oz[2] = (I)zm * ksb;
     } else {
-      // reduced precision: each sum starts at its first term (0 + x cannot
-      // be folded away in IEEE arithmetic, -0 + 0 = +0)
-      static_for<LAT::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        if constexpr (q == 0) r = f[q];
-        else r += f[q];
-        accumulate_signed<first_nonzero(LAT::cx, LAT::Q), q, LAT::cx[q]>(m1, f[q]);
-        accumulate_signed<first_nonzero(LAT::cy, LAT::Q), q, LAT::cy[q]>(m2, f[q]);
-        accumulate_signed<first_nonzero(LAT::cz, LAT::Q), q, LAT::cz[q]>(m3, f[q]);
-      });
+      oz[0] = oz[1] = oz[2] = 0;
     }
-    if constexpr (SH) r += T(1);
-    const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
-    const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
-                         (unsigned)bk < (unsigned)a.band_n[2];
-    const int64_t bidx = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
-    T Fx = arg_t<ST>(a.Fmac[0], a.fFmac[0]), Fy = arg_t<ST>(a.Fmac[1], a.fFmac[1]), Fz = arg_t<ST>(a.Fmac[2], a.fFmac[2]);
-    if (in_band && a.band_force) {
-      Fx = T(a.Fband[bidx]);
-      Fy = T(a.Fband[a.band_cells + bidx]);
-      Fz = T(a.Fband[2 * a.band_cells + bidx]);
-    }
-    T u, v, w;
-    forced_velocity<ST>(r, m1, m2, m3, Fx, Fy, Fz, u, v, w);
-    rho_acc = (double)r;
-    const int64_t vcell = ((int64_t)k * g.n[1] + j) * g.n[0] + i;
-    const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
-    if (in_band) {
-      a.Mband[bidx] = (double)r;
-      a.Mband[a.band_cells + bidx] = (double)u;
-      a.Mband[2 * a.band_cells + bidx] = (double)v;
-      a.Mband[3 * a.band_cells + bidx] = (double)w;
-    }
-    if (a.mout) {
-      a.mout[vcell] = (double)r;
-      a.mout[ncell + vcell] = (double)u;
-      a.mout[2 * ncell + vcell] = (double)v;
-      a.mout[3 * ncell + vcell] = (double)w;
-    }
-    if (a.fout) {
-#pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q] + (SH ? LAT::w[q] : 0.0);
-    }
-    // pre-collision boundary layer for the next ghost fill
+  }
+  // A source outside a non-periodic face is a ghost cell; its value (the
+  // pre-collision FillPatch value, or bounce-back) sits in the slabs that
+  // follow the Q population planes of the same buffer.  The BC functor
+  // clamps the mirror in every dimension (bc_fill.H:28-40), so a ghost
+  // that is also outside a periodic dimension is not wrapped.  All loads
+  // are issued unconditionally; only their offsets are selected.
+  bool outx[3] = {false, false, false}, outy[3] = {false, false, false}, outz[3] = {false, false, false};
+  I tb[3] = {0, 0, 0}, mu[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  if constexpr (XO) {
+    // both x faces span the same (y, z) range, so their strides are
+    // uniform; only the face's offset differs between the two sides
+    outx[0] = (i == g.n[0] - 1);
+    outx[2] = (i == 0);
+    const int f0 = a.slabs.slab_of[0][0], f1 = a.slabs.slab_of[0][1];
+    const I fn1 = (I)a.slabs.face_n[f0][1];
+    mu[0][1] = (I)SB;
+    mu[0][2] = fn1 * (I)SB;
+    const I base = (i == 0) ? (I)a.slabs.face_off[f0] : (I)a.slabs.face_off[f1];
+    tb[0] = (base + (I)(j - a.slabs.face_lo[f0][1]) + (I)(k - a.slabs.face_lo[f0][2]) * fn1) * (I)SB;
+  } else if constexpr (!WRAP) {
     const int cc[3] = {i, j, k};
+    bool* outs[3] = {outx, outy, outz};
 #pragma unroll
     for (int d = 0; d < LAT::D; ++d) {
       if (!a.nonper[d]) continue;
-      int side = -1;
-      if (cc[d] == 0) side = 0;
-      else if (cc[d] == g.n[d] - 1) side = 1;
-      if (side < 0) continue;
-      R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
-      const int64_t fi = face_index(g, d, i, j, k);
-      face_store<LAT, ST, true>(fb, fi, f);
-      // a cell sitting on the lo and hi face of a 1-cell dimension writes both
-      if (g.n[d] == 1) face_store<LAT, ST, true>(static_cast<R*>(a.faces.p[2 * d + 1]), fi, f);
-    }
-    // BGK collision with the macros just computed (feq uses macro_old of the
-    // next step, AdvanceLBMAtLevel.cpp:225-238)
-    const T uv = usq<LAT, T>(u, v, w);
-    const T cfx = arg_t<ST>(a.Fcol[0], a.fFcol[0]), cfy = arg_t<ST>(a.Fcol[1], a.fFcol[1]), cfz = arg_t<ST>(a.Fcol[2], a.fFcol[2]);
-    const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
-    const T fac = ST::shifted ? T(a.ffac) : T(1.0) - T(0.5) / T(a.tau);
-    const I self = (I)g.off0 * (I)SB + ox[1] + oy[1] + oz[1];
-    auto put = [&](int q, T gq) {
-      if constexpr (O32) {
-        // a 32-bit byte offset from the buffer base, as the loads
-        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(gout) + ((I)q * (I)g.qs * (I)SB + self) * EB);
-        pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
-      } else if (a.off32) {
-        R* plane = gout + (int64_t)q * g.qs;
-        R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
-        pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
-      } else {
-        pop_out_nt<ST>(gout, (int64_t)q * g.qs + (int64_t)self, gq, T(LAT::w[q]));
-      }
-    };
-    if constexpr (ST::exact) {
+      outs[d][0] = (cc[d] == g.n[d] - 1);
+      outs[d][2] = (cc[d] == 0);
+      const int sd = (cc[d] == 0) ? 0 : 1;
+      const int f = a.slabs.slab_of[d][sd];
+      I t = (I)a.slabs.face_off[f], mul = 1;
 #pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) {
-        const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
-        T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
+      for (int e = 0; e < 3; ++e) {
+        if (e == d) continue;
+        t += (I)(cc[e] - a.slabs.face_lo[f][e]) * mul;
+        mu[d][e] = mul * (I)SB;
+        mul *= (I)a.slabs.face_n[f][e];
+      }
+      tb[d] = t * (I)SB;
+    }
+  }
+  const I gofs = (I)g.pop_n * (I)SB;                  // start of the slabs
+  const I npad = (I)a.slabs.face_off[a.slabs.nface] * (I)SB;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    if constexpr (WRAP) {
+      // all-periodic level with planes under 4 GiB: a 32-bit byte offset
+      // from the plane's (uniform) base
+      const uint32_t src = (uint32_t)(ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1]);
+      const R* plane = gin + (int64_t)q * g.qs;
+      const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(plane) + src * EB);
+      // 3D: non-temporal loads (256^3 FP64 5.66 -> 5.94 TB/s, FP32 128^3
+      // +1 %); on walled and 2D levels they cost 2-9 %
+      if constexpr (LAT::D == 3) f[q] = pop_in_nt<ST>(p, 0, T(LAT::w[q]));
+      else f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
+      continue;
+    }
+    // the source cell inside the level, or for a source beyond a
+    // non-periodic face the slab of the lowest such dimension it leaves
+    // (a per-lane select: one load per direction on every lane)
+    I off = ((I)q * (I)g.qs + (I)g.off0) * (I)SB + ox[LAT::cx[q] + 1] + oy[LAT::cy[q] + 1] + oz[LAT::cz[q] + 1];
+    if constexpr (XO) {
+      if (LAT::cx[q] != 0) {
+        const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
+        off = outx[LAT::cx[q] + 1] ? gofs + (I)q * npad + t0 : off;
+      }
+      const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
+      f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
+      continue;
+    }
+    const bool o0 = outx[LAT::cx[q] + 1], o1 = outy[LAT::cy[q] + 1], o2 = outz[LAT::cz[q] + 1];
+    const I t0 = tb[0] - (I)LAT::cy[q] * mu[0][1] - (I)LAT::cz[q] * mu[0][2];
+    const I t1 = tb[1] - (I)LAT::cx[q] * mu[1][0] - (I)LAT::cz[q] * mu[1][2];
+    const I t2 = tb[2] - (I)LAT::cx[q] * mu[2][0] - (I)LAT::cy[q] * mu[2][1];
+    const I t = o0 ? t0 : (o1 ? t1 : t2);
+    off = (o0 | o1 | o2) ? gofs + (I)q * npad + t : off;
+    if constexpr (O32) {
+      const R* p = reinterpret_cast<const R*>(reinterpret_cast<const char*>(gin) + off * EB);
+      f[q] = pop_in<ST>(p, 0, T(LAT::w[q]));
+    } else {
+      f[q] = pop_in<ST>(gin, off, T(LAT::w[q]));
+    }
+  }
+  self = (I)g.off0 * (I)SB + ox[1] + oy[1] + oz[1];
+}
+
+// Second half: macros, exports, boundary layer, collision and stores of the
+// loaded f; returns the cell's rho for Sum(Rho).
+template <class LAT, class ST, int AM>
+__device__ __forceinline__ double step_update(const StepArgs& a, int i, int j, int k,
+                                              const typename ST::compute_t* f, StepOff<AM> self) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  constexpr bool WRAP = AM == kAddrWrap;
+  constexpr bool XO = AM == kAddrWallX32;
+  constexpr bool O32 = AM != kAddrWall64;
+  using I = StepOff<AM>;
+  const GridGeom& g = a.g;
+  (void)WRAP;
+  R* __restrict__ gout = static_cast<R*>(a.gout);
+  constexpr uint32_t SB = (O32 && ST::shifted) ? (uint32_t)sizeof(R) : 1u;
+  constexpr uint32_t EB = O32 ? (uint32_t)sizeof(R) / SB : 1u;
+  // f holds f_q, or h_q = f_q - w_q for shifted storage (then the first
+  // moments are the same sums, since sum_q w_q c_q = 0, and rho = 1 +
+  // sum_q h_q)
+  constexpr bool SH = ST::shifted;
+  // moments in the reference's order (calculateMacroForcing.H:21-27); a
+  // term whose direction component is 0 is left out and -1 * f is
+  // subtracted, which leaves every finite sum bit-identical
+  T r = T(0), m1 = T(0), m2 = T(0), m3 = T(0);
+  if constexpr (ST::exact) {
+    static_for<LAT::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      r += f[q];
+      if constexpr (LAT::cx[q] > 0) m1 += f[q];
+      else if constexpr (LAT::cx[q] < 0) m1 -= f[q];
+      if constexpr (LAT::cy[q] > 0) m2 += f[q];
+      else if constexpr (LAT::cy[q] < 0) m2 -= f[q];
+      if constexpr (LAT::cz[q] > 0) m3 += f[q];
+      else if constexpr (LAT::cz[q] < 0) m3 -= f[q];
+    });
+  } else {
+    // reduced precision: each sum starts at its first term (0 + x cannot
+    // be folded away in IEEE arithmetic, -0 + 0 = +0)
+    static_for<LAT::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      if constexpr (q == 0) r = f[q];
+      else r += f[q];
+      accumulate_signed<first_nonzero(LAT::cx, LAT::Q), q, LAT::cx[q]>(m1, f[q]);
+      accumulate_signed<first_nonzero(LAT::cy, LAT::Q), q, LAT::cy[q]>(m2, f[q]);
+      accumulate_signed<first_nonzero(LAT::cz, LAT::Q), q, LAT::cz[q]>(m3, f[q]);
+    });
+  }
+  if constexpr (SH) r += T(1);
+  const int bi = i - a.band_lo[0], bj = j - a.band_lo[1], bk = k - a.band_lo[2];
+  const bool in_band = (unsigned)bi < (unsigned)a.band_n[0] && (unsigned)bj < (unsigned)a.band_n[1] &&
+                       (unsigned)bk < (unsigned)a.band_n[2];
+  const int64_t bidx = ((int64_t)bk * a.band_n[1] + bj) * a.band_n[0] + bi;
+  T Fx = arg_t<ST>(a.Fmac[0], a.fFmac[0]), Fy = arg_t<ST>(a.Fmac[1], a.fFmac[1]), Fz = arg_t<ST>(a.Fmac[2], a.fFmac[2]);
+  if (in_band && a.band_force) {
+    Fx = T(a.Fband[bidx]);
+    Fy = T(a.Fband[a.band_cells + bidx]);
+    Fz = T(a.Fband[2 * a.band_cells + bidx]);
+  }
+  T u, v, w;
+  forced_velocity<ST>(r, m1, m2, m3, Fx, Fy, Fz, u, v, w);
+  const double rho_out = (double)r;
+  const int64_t vcell = ((int64_t)k * g.n[1] + j) * g.n[0] + i;
+  const int64_t ncell = (int64_t)g.n[0] * g.n[1] * g.n[2];
+  if (in_band) {
+    a.Mband[bidx] = (double)r;
+    a.Mband[a.band_cells + bidx] = (double)u;
+    a.Mband[2 * a.band_cells + bidx] = (double)v;
+    a.Mband[3 * a.band_cells + bidx] = (double)w;
+  }
+  if (a.mout) {
+    a.mout[vcell] = (double)r;
+    a.mout[ncell + vcell] = (double)u;
+    a.mout[2 * ncell + vcell] = (double)v;
+    a.mout[3 * ncell + vcell] = (double)w;
+  }
+  if (a.fout) {
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q] + (SH ? LAT::w[q] : 0.0);
+  }
+  // pre-collision boundary layer for the next ghost fill
+  const int cc[3] = {i, j, k};
+#pragma unroll
+  for (int d = 0; d < LAT::D; ++d) {
+    if (!a.nonper[d]) continue;
+    int side = -1;
+    if (cc[d] == 0) side = 0;
+    else if (cc[d] == g.n[d] - 1) side = 1;
+    if (side < 0) continue;
+    R* fb = static_cast<R*>(a.faces.p[2 * d + side]);
+    const int64_t fi = face_index(g, d, i, j, k);
+    face_store<LAT, ST, true>(fb, fi, f);
+    // a cell sitting on the lo and hi face of a 1-cell dimension writes both
+    if (g.n[d] == 1) face_store<LAT, ST, true>(static_cast<R*>(a.faces.p[2 * d + 1]), fi, f);
+  }
+  if constexpr (XO) {
+    // (a non-periodic x extent is at least 2: a cell is on one face)
+    if (a.gfuse && i == 0) ghost_write<LAT, ST, 0>(a, j, k, f);
+    else if (a.gfuse && i == g.n[0] - 1) ghost_write<LAT, ST, 1>(a, j, k, f);
+  }
+  // BGK collision with the macros just computed (feq uses macro_old of the
+  // next step, AdvanceLBMAtLevel.cpp:225-238)
+  const T uv = usq<LAT, T>(u, v, w);
+  const T cfx = arg_t<ST>(a.Fcol[0], a.fFcol[0]), cfy = arg_t<ST>(a.Fcol[1], a.fFcol[1]), cfz = arg_t<ST>(a.Fcol[2], a.fFcol[2]);
+  const bool forced = !(cfx == T(0.0) && cfy == T(0.0) && cfz == T(0.0));
+  const T fac = ST::shifted ? T(a.ffac) : T(1.0) - T(0.5) / T(a.tau);
+  auto put = [&](int q, T gq) {
+    if constexpr (O32) {
+      // a 32-bit byte offset from the buffer base, as the loads
+      R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(gout) + ((I)q * (I)g.qs * (I)SB + self) * EB);
+      pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
+    } else if (a.off32) {
+      R* plane = gout + (int64_t)q * g.qs;
+      R* p = reinterpret_cast<R*>(reinterpret_cast<char*>(plane) + (uint32_t)self * (uint32_t)sizeof(R));
+      pop_out_nt<ST>(p, 0, gq, T(LAT::w[q]));
+    } else {
+      pop_out_nt<ST>(gout, (int64_t)q * g.qs + (int64_t)self, gq, T(LAT::w[q]));
+    }
+  };
+  if constexpr (ST::exact) {
+#pragma unroll
+    for (int q = 0; q < LAT::Q; ++q) {
+      const T fe = feq_q<LAT, T>(q, r, u, v, w, uv);
+      T gq = bgk<ST>(f[q], fe, a.tau, a.inv_tau);
+      if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+      put(q, gq);
+    }
+  } else {
+    // reduced-precision collision, a direction and its opposite together:
+    // feq = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u), then
+    // g = f + (feq - f) / tau with fused multiply-adds.  Shifted storage
+    // relaxes h = f - w towards feq - w (the same update, shifted).
+    const T base = T(1.0) - T(1.5) * uv;
+    const T om = arg_t<ST>(a.inv_tau, a.finv_tau);
+    static_for<LAT::Q>([&](auto qc) {
+      constexpr int q = decltype(qc)::value;
+      constexpr int p = opposite<LAT>(q);
+      if constexpr (p == q) {
+        const T fe = SH ? __builtin_fmaf(T(LAT::w[q]) * r, base, -T(LAT::w[q])) : T(LAT::w[q]) * r * base;
+        T gq = __builtin_fmaf(fe - f[q], om, f[q]);
         if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
         put(q, gq);
-      }
-    } else {
-      // reduced-precision collision, a direction and its opposite together:
-      // feq = w rho (1 - 1.5 u.u + 4.5 (c.u)^2) +- 3 w rho (c.u), then
-      // g = f + (feq - f) / tau with fused multiply-adds.  Shifted storage
-      // relaxes h = f - w towards feq - w (the same update, shifted).
-      const T base = T(1.0) - T(1.5) * uv;
-      const T om = arg_t<ST>(a.inv_tau, a.finv_tau);
-      static_for<LAT::Q>([&](auto qc) {
-        constexpr int q = decltype(qc)::value;
-        constexpr int p = opposite<LAT>(q);
-        if constexpr (p == q) {
-          const T fe = SH ? __builtin_fmaf(T(LAT::w[q]) * r, base, -T(LAT::w[q])) : T(LAT::w[q]) * r * base;
-          T gq = __builtin_fmaf(fe - f[q], om, f[q]);
-          if (forced) gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
-          put(q, gq);
-        } else if constexpr (p > q) {
-          // c.u from its nonzero components, starting at the first
-          constexpr int c0 = LAT::cx[q], c1 = LAT::cy[q], c2 = (LAT::D == 3) ? LAT::cz[q] : 0;
-          constexpr int k0 = c0 ? 0 : (c1 ? 1 : 2);
-          T eu = T(0);
-          accumulate_signed<k0, 0, c0>(eu, u);
-          accumulate_signed<k0, 1, c1>(eu, v);
-          accumulate_signed<k0, 2, c2>(eu, w);
-          const T wr = T(LAT::w[q]) * r;
-          const T A = SH ? __builtin_fmaf(wr, __builtin_fmaf(T(4.5) * eu, eu, base), -T(LAT::w[q]))
-                         : wr * __builtin_fmaf(T(4.5) * eu, eu, base);
-          const T B = (T(3.0) * wr) * eu;
-          T gq = __builtin_fmaf((A + B) - f[q], om, f[q]);
-          T gp = __builtin_fmaf((A - B) - f[p], om, f[p]);
-          if (forced) {
-            gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
-            gp += guo_q<LAT, T>(p, fac, u, v, w, cfx, cfy, cfz);
-          }
-          put(q, gq);
-          put(p, gp);
+      } else if constexpr (p > q) {
+        // c.u from its nonzero components, starting at the first
+        constexpr int c0 = LAT::cx[q], c1 = LAT::cy[q], c2 = (LAT::D == 3) ? LAT::cz[q] : 0;
+        constexpr int k0 = c0 ? 0 : (c1 ? 1 : 2);
+        T eu = T(0);
+        accumulate_signed<k0, 0, c0>(eu, u);
+        accumulate_signed<k0, 1, c1>(eu, v);
+        accumulate_signed<k0, 2, c2>(eu, w);
+        const T wr = T(LAT::w[q]) * r;
+        const T A = SH ? __builtin_fmaf(wr, __builtin_fmaf(T(4.5) * eu, eu, base), -T(LAT::w[q]))
+                       : wr * __builtin_fmaf(T(4.5) * eu, eu, base);
+        const T B = (T(3.0) * wr) * eu;
+        T gq = __builtin_fmaf((A + B) - f[q], om, f[q]);
+        T gp = __builtin_fmaf((A - B) - f[p], om, f[p]);
+        if (forced) {
+          gq += guo_q<LAT, T>(q, fac, u, v, w, cfx, cfy, cfz);
+          gp += guo_q<LAT, T>(p, fac, u, v, w, cfx, cfy, cfz);
         }
-      });
-    }
+        put(q, gq);
+        put(p, gp);
+      }
+    });
+  }
+  return rho_out;
+}
+
+// NC cells per thread: the loads of every cell are issued before the first
+// update, so that a wavefront keeps NC times the bytes in flight.
+template <class LAT, class ST, int AM, int NC>
+__device__ __forceinline__ void step_body_n(const StepArgs& a, const int* i, const int* j, const int* k, int64_t blk,
+                                            double* lds) {
+  using T = typename ST::compute_t;
+  T f[NC][LAT::Q];
+  StepOff<AM> self[NC];
+  bool act[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) {
+    act[c] = step_active(a, i[c], j[c], k[c]);
+    if (act[c]) step_load<LAT, ST, AM>(a, i[c], j[c], k[c], f[c], self[c]);
+  }
+  double rho_acc = 0.0;
+#pragma unroll
+  for (int c = 0; c < NC; ++c)
+    if (act[c]) rho_acc += step_update<LAT, ST, AM>(a, i[c], j[c], k[c], f[c], self[c]);
+  const double s = block_sum(rho_acc, lds);
+  finish_mass(a, s, blk);
+}
+
+template <class LAT, class ST, int AM>
+__device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k, int64_t blk, double* lds) {
+  using T = typename ST::compute_t;
+  double rho_acc = 0.0;
+  if (step_active(a, i, j, k)) {
+    T f[LAT::Q];
+    StepOff<AM> self;
+    step_load<LAT, ST, AM>(a, i, j, k, f, self);
+    rho_acc = step_update<LAT, ST, AM>(a, i, j, k, f, self);
   }
   const double s = block_sum(rho_acc, lds);
   finish_mass(a, s, blk);
@@ -621,38 +790,80 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
                            o2 + blockIdx.z, blk, lds);
 }
 
-// Tile (bx, by) of block b of a case's grid, x fastest, as k_step's 2D grid;
-// bx = 0: block b holds cells [256 b, 256 b + 256) of the case in x-fastest
-// order (no row padding; a cell past the end gets j = n[1], inactive).
-__device__ inline void batch_tile(const GridGeom& g, int bx, int b, int* i, int* j, int* k) {
+// k_step with a fixed number of resident blocks, each striding over the
+// tiles of the (tx, ty, tz) grid k_step would launch; tile t keeps k_step's
+// block index as its mass partial.  A launch with fewer blocks than the
+// device holds leaves wave slots on every CU to kernels of other streams
+// (the IB kernels beside the far cells of an overlapped step), which
+// otherwise wait until the whole grid has been dispatched.
+template <class LAT, class ST, int AM>
+__global__ void __launch_bounds__(kBlock) k_step_p(StepArgs a, uint32_t tx, uint32_t ty, uint32_t ntiles) {
+  __shared__ double lds[kBlock / 64];
+  const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
+  const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
+    step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x), o1 + (int)(by * blockDim.y + threadIdx.y),
+                           o2 + (int)bz, (int64_t)t, lds);
+  }
+}
+
+// Cell c (of nc per thread) of block b of a case's grid.  bx > 0: the block
+// covers a tile of bx x (nc * by) cells, by = kBlock / bx, x fastest as
+// k_step's 2D grid, cell c of a thread lying c * by rows below cell 0.
+// bx = 0: block b holds cells [nc 256 b, nc 256 (b + 1)) of the case in
+// x-fastest order, cell c of thread t at 256 c + t (no row padding; a cell
+// past the end gets j = n[1], inactive).
+__device__ inline void batch_tile(const GridGeom& g, int bx, int nc, int b, int c, int* i, int* j, int* k) {
+  const int t = threadIdx.x;
   if (bx == 0) {
-    const uint32_t lin = (uint32_t)b * kBlock + threadIdx.x;
+    // the block's first cell from one (uniform) division per coordinate;
+    // a lane then steps over at most kBlock / n[0] + 1 row ends
+    const uint32_t lin0 = ((uint32_t)b * (uint32_t)nc + (uint32_t)c) * kBlock;
     const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1];
-    const uint32_t row = lin / nx;
-    *i = (int)(lin - row * nx);
-    *j = (int)(row % ny);
-    *k = (int)(row / ny);
-    if (*k >= g.n[2]) *j = g.n[1];
+    const uint32_t row0 = lin0 / nx;
+    int ii = (int)(lin0 - row0 * nx) + t;
+    const uint32_t k0 = row0 / ny;
+    int jj = (int)(row0 - k0 * ny), kk = (int)k0;
+    while (ii >= (int)nx) {
+      ii -= (int)nx;
+      if (++jj == (int)ny) { jj = 0; ++kk; }
+    }
+    *i = ii;
+    *j = (kk >= g.n[2]) ? g.n[1] : jj;
+    *k = kk;
     return;
   }
-  const int by = kBlock / bx;
-  const int gx = (g.n[0] + bx - 1) / bx, gy = (g.n[1] + by - 1) / by;
+  const int sh = __builtin_ctz((unsigned)bx);   // bx is 64, 128 or 256
+  const int by = kBlock >> sh;
+  const int gx = (g.n[0] + bx - 1) >> sh, gy = (g.n[1] + nc * by - 1) / (nc * by);
   const int bxi = b % gx, rest = b / gx;
   const int byi = rest % gy, bz = rest / gy;
-  const int t = threadIdx.x;
-  *i = bxi * bx + (t & (bx - 1));
-  *j = byi * by + t / bx;
+  *i = (bxi << sh) + (t & (bx - 1));
+  *j = byi * (nc * by) + c * by + (t >> sh);
   *k = bz;
 }
 
-// Many cases of one lattice / storage class in one launch.
+// Many cases of one lattice / storage class in one launch.  On 2D lattices
+// a case's tile_nc (uniform over its blocks) selects one or two cells per
+// thread (nine loads per cell keep too few bytes in flight); 3D lattices
+// take one, which keeps their register budget.
 template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map) {
   __shared__ double lds[kBlock / 64];
   const BlockRef r = map[blockIdx.x];
   const StepArgs& a = A[r.c];
+  if constexpr (LAT::D == 2) {
+    if (a.tile_nc == 2) {
+      int i[2], j[2], k[2];
+      batch_tile(a.g, a.tile_bx, 2, r.b, 0, &i[0], &j[0], &k[0]);
+      batch_tile(a.g, a.tile_bx, 2, r.b, 1, &i[1], &j[1], &k[1]);
+      step_body_n<LAT, ST, AM, 2>(a, i, j, k, r.b, lds);
+      return;
+    }
+  }
   int i, j, k;
-  batch_tile(a.g, a.tile_bx, r.b, &i, &j, &k);
+  batch_tile(a.g, a.tile_bx, 1, r.b, 0, &i, &j, &k);
   step_body<LAT, ST, AM>(a, i, j, k, r.b, lds);
 }
 
@@ -744,7 +955,6 @@ __global__ void __launch_bounds__(kBlock) k_init(InitArgs ia) {
 // ---------------------------------------------------------------------------
 // k_fill_ghosts: one ghost layer of the padded layout (see header comment).
 // ---------------------------------------------------------------------------
-enum { BC_INT_DIR = 0, BC_FOEXTRAP = 2, BC_EXT_DIR = 3, BC_USER_1 = 1001, BC_USER_2 = 1002 };
 
 struct BcSpec {
   int periodic[3];
@@ -837,35 +1047,7 @@ __device__ void fillpatch_value(const GhostArgs& a, const int iv[3], typename ST
         for (int q = 0; q < LAT::Q; ++q) out[q] = fnp[q];
         return;
       }
-      T rhonp = T(0), unp = T(0), vnp = T(0), wnp = T(0);
-#pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) {
-        rhonp += fnp[q];
-        unp += T(LAT::cx[q]) * fnp[q];
-        vnp += T(LAT::cy[q]) * fnp[q];
-        wnp += T(LAT::cz[q]) * fnp[q];
-      }
-      rhonp = rhonp > T(1.0e-20) ? rhonp : T(1.0e-20);
-      unp /= rhonp; vnp /= rhonp; wnp /= rhonp;
-      const double* bv = lo_side ? a.bc.lo_val[bd] : a.bc.hi_val[bd];
-      const T rho_b = T(bv[0]), u_b = T(bv[1]), v_b = T(bv[2]), w_b = T(bv[3]);
-      T rg = rho_b, ug = u_b, vg = v_b, wg = w_b;
-      if (kind == BC_USER_2) {
-        const T t2 = T(2.0) * rho_b - rhonp;
-        rg = T(1.0e-20) > t2 ? T(1.0e-20) : t2;
-        ug = T(2.0) * u_b - unp; vg = T(2.0) * v_b - vnp; wg = T(2.0) * w_b - wnp;
-      }
-#pragma unroll
-      for (int q = 0; q < LAT::Q; ++q) {
-        const T cx = T(LAT::cx[q]), cy = T(LAT::cy[q]), cz = T(LAT::cz[q]);
-        const T eu_np = cx * unp + cy * vnp + cz * wnp;
-        const T uv_np = unp * unp + vnp * vnp + wnp * wnp;
-        const T feqnp = T(LAT::w[q]) * rhonp * (T(1.0) + T(3.0) * eu_np + T(4.5) * eu_np * eu_np - T(1.5) * uv_np);
-        const T eu_g = cx * ug + cy * vg + cz * wg;
-        const T uv_g = ug * ug + vg * vg + wg * wg;
-        const T feqg = T(LAT::w[q]) * rg * (T(1.0) + T(3.0) * eu_g + T(4.5) * eu_g * eu_g - T(1.5) * uv_g);
-        out[q] = feqg + (fnp[q] - feqnp);
-      }
+      neep_ghost<LAT, T>(kind, fnp, lo_side ? a.bc.lo_val[bd] : a.bc.hi_val[bd], out);
       return;
     }
     // FilccCell: the last foextrap dimension in which c is outside wins

@@ -291,6 +291,13 @@ using GhostRecFn = void (*)(unsigned, hipStream_t, const GhostArgs&, unsigned, c
 
 template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
+  const uint32_t nt = g.x * g.y * g.z;
+  if (a.persist > 0 && nt > (uint32_t)a.persist) {
+    const dim3 pg((unsigned)a.persist);
+    if (e0) hipExtLaunchKernelGGL((k_step_p<LAT, ST, AM>), pg, b, 0, s, e0, e1, 0, a, g.x, g.y, nt);
+    else hipLaunchKernelGGL((k_step_p<LAT, ST, AM>), pg, b, 0, s, a, g.x, g.y, nt);
+    return;
+  }
   if (e0) hipExtLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, e0, e1, 0, a);
   else hipLaunchKernelGGL((k_step<LAT, ST, AM>), g, b, 0, s, a);
 }
@@ -358,6 +365,9 @@ static int64_t plane_stride(int64_t cells, int64_t pop_bytes) {
   return ((cells * pop_bytes + 16383) / 16384 * 16384 + 8192) / pop_bytes;
 }
 
+// resident blocks per CU of the far-cell launch of an overlapped step
+constexpr int kFarBlocksPerCu = 4;
+
 struct Level {
   ilbm_case_desc c{};
   int lat = -1, st = -1;
@@ -381,6 +391,9 @@ struct Level {
   // one-stream steps: the force record of the forcing is held back
   // (rec_to_ghost) and launched with the ghost slabs of the same step
   bool rec_to_ghost = false, rec_held = false;
+  // k_step writes the x-face ghost slabs of the next step (ghost_write);
+  // slabs_fresh: the current buffer's slabs were written by the last step
+  bool gfuse = false, slabs_fresh = false;
   IbArgs held_a{};
   RecArgs held_r{};
   int held_nblk = 0;
@@ -493,6 +506,9 @@ struct Level {
   // writes, so they stream back to back on the second stream, and the ring
   // between the boxes follows the near box on the first
   bool ring_ok = false;
+  // resident blocks of the far-cell launch (k_step_p) of an overlapped step:
+  // ILBM_FAR_BLOCKS_PER_CU per CU (0: one block per tile)
+  int far_persist = 0;
   int ring_lo[3] = {0, 0, 0}, ring_n[3] = {0, 0, 0};
   dim3 grid_ring;
   int64_t nblocks_ring = 0;
@@ -623,6 +639,12 @@ struct Level {
     // every dimension)
     const char* gen = std::getenv("ILBM_ADDR_GENERIC");
     if (am == kAddrWall32 && nonper[0] && !nonper[1] && !nonper[2] && !(gen && gen[0] == '1')) am = kAddrWallX32;
+    // on such a level with ext_dir / user_2 / foextrap x faces the stream-
+    // collide kernel writes the ghost slabs itself (ILBM_GHOST_SEPARATE=1
+    // keeps the separate ghost kernel)
+    const char* gsep = std::getenv("ILBM_GHOST_SEPARATE");
+    auto fusable = [](int b) { return b == BC_FOEXTRAP || b == BC_EXT_DIR || b == BC_USER_2; };
+    gfuse = am == kAddrWallX32 && !cf && fusable(c.bc_lo[0]) && fusable(c.bc_hi[0]) && !(gsep && gsep[0] == '1');
     for (int b = 0; b < 2; ++b) {
       // Q population planes followed by the Q ghost slabs
       const size_t bytes = popbytes * ((size_t)g.pop_n + (size_t)npad * Q);
@@ -659,10 +681,12 @@ struct Level {
   }
 
   // number of blocks of a tile of width bx (height kBlock / bx)
-  int64_t tile_blocks(int bx) const {
-    if (bx == 0) return (ncells + kBlock - 1) / kBlock;  // linear tiles
+  // blocks of the batched stream-collide launch with tile width bx (0:
+  // linear tiles) and nc cells per thread
+  int64_t tile_blocks(int bx, int nc) const {
+    if (bx == 0) return (ncells + (int64_t)nc * kBlock - 1) / ((int64_t)nc * kBlock);  // linear tiles
     const int by = kBlock / bx;
-    return (int64_t)((c.n[0] + bx - 1) / bx) * ((c.n[1] + by - 1) / by) * c.n[2];
+    return (int64_t)((c.n[0] + bx - 1) / bx) * ((c.n[1] + nc * by - 1) / (nc * by)) * c.n[2];
   }
 
   void ensure_partial(int64_t n) {
@@ -1125,7 +1149,13 @@ struct Level {
     a.faces = faces;
     a.partial = partial;
     a.slabs = slabs;
+    a.gfuse = gfuse ? 1 : 0;
+    a.gkind[0] = c.bc_lo[0];
+    a.gkind[1] = c.bc_hi[0];
+    for (int k = 0; k < 4; ++k) { a.gval[0][k] = c.bc_lo_val[0][k]; a.gval[1][k] = c.bc_hi_val[0][k]; }
     a.tile_bx = (int)block.x;
+    a.tile_nc = 1;
+    a.persist = 0;
     a.off32 = ((int64_t)popbytes * g.pop_n < ((int64_t)1 << 32)) ? 1 : 0;
     a.ftau = (float)a.tau;
     a.finv_tau = (float)a.inv_tau;
@@ -1233,6 +1263,7 @@ struct Level {
       HIPCHK(hipMemcpy(frstate_d, p, sizeof(double) * 4, hipMemcpyHostToDevice));
     }
     t_new = h.t_new; t_old = h.t_old; istep = h.istep;
+    slabs_fresh = false;
     const double clk[4] = {t_new, (double)istep, (double)istep, 0.0};
     HIPCHK(hipMemcpy(clock_d, clk, sizeof(clk), hipMemcpyHostToDevice));
     state_valid = false;
@@ -1259,6 +1290,7 @@ struct Level {
     initfn(grid, block, stream, ia);
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(stream));
+    slabs_fresh = false;
     state_valid = small;
     t_new = 0.0;
     t_old = -1.0e100;
@@ -1286,6 +1318,12 @@ struct Level {
     // the overlap saves (512x256 D2Q9 FP64: 2.5 -> 1.9 GLUPS with it)
     if (ncells < ((int64_t)1 << 20)) return false;
     if (!stream_b) {
+      int per_cu = kFarBlocksPerCu;
+      if (const char* e = std::getenv("ILBM_FAR_BLOCKS_PER_CU")) per_cu = std::max(0, std::atoi(e));
+      int dev = 0, ncu = 0;
+      HIPCHK(hipGetDevice(&dev));
+      HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+      far_persist = per_cu * ncu;
       for (int d = 0; d < 3; ++d) {
         int lo = band_lo[d] - 1, hi = band_lo[d] + band_n[d];
         if (d >= c.dim) { lo = 0; hi = 0; }
@@ -1354,9 +1392,11 @@ struct Level {
   void split_step(int s, double time, bool want, bool more) {
     StepArgs sf = step_args(want);
     sf.sub_mode = 1;
+    sf.persist = far_persist;
     for (int d = 0; d < 3; ++d) { sf.sub_lo[d] = near_lo[d]; sf.sub_n[d] = near_n[d]; }
     sf.partial = mring + (int64_t)mring_used * nblocks;
     StepArgs sn = sf;
+    sn.persist = 0;
     sn.sub_mode = 2;
     sn.partial = nring + (int64_t)mring_used * nblocks_near;
     const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
@@ -1365,7 +1405,7 @@ struct Level {
     // the far cells of the last step, which read what it overwrites.
     uint64_t ghost_done = ahead_signal;
     if (!ghost_ahead) {
-      if (npad) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+      if (npad && !slabs_fresh) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
       ghost_done = xsignal(stream_b, X_GHOST);
     }
     ghost_ahead = false;
@@ -1379,7 +1419,7 @@ struct Level {
     if (more && (near_off_walls || !npad)) {
       // the next step's slabs (from the boundary layer of the far cells
       // just written) and its signal to the near box, ahead of the wait
-      if (npad) {
+      if (npad && !gfuse) {
         cur ^= 1;
         const GhostArgs ga = ghost_args();
         cur ^= 1;
@@ -1424,14 +1464,17 @@ struct Level {
   void split_step3(int s, double time, bool want, bool more, bool first) {
     StepArgs sf = step_args(want);
     sf.sub_mode = 1;
+    sf.persist = far_persist;
     for (int d = 0; d < 3; ++d) { sf.sub_lo[d] = ring_lo[d]; sf.sub_n[d] = ring_n[d]; }
     const int slot = mring_used;
     sf.partial = mring + (int64_t)slot * nblocks;
     StepArgs sn = sf;
+    sn.persist = 0;
     sn.sub_mode = 2;
     for (int d = 0; d < 3; ++d) { sn.sub_lo[d] = near_lo[d]; sn.sub_n[d] = near_n[d]; }
     sn.partial = nring + (int64_t)slot * nblocks_near;
     StepArgs sr = sf;
+    sr.persist = 0;
     sr.sub_mode = 3;
     for (int d = 0; d < 3; ++d) { sr.in_lo[d] = near_lo[d]; sr.in_n[d] = near_n[d]; }
     sr.partial = rring + (int64_t)slot * nblocks_ring;
@@ -1440,11 +1483,11 @@ struct Level {
     // stream B
     if (!first) xwait(stream_b, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
     flush_mass(stream_b);
-    if (!ghost_ahead && npad) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+    if (!ghost_ahead && npad && !slabs_fresh) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
     ghost_ahead = false;
     stepfn(grid, block, stream_b, sf, nullptr, nullptr);
     far_sig[s & 1] = xsignal(stream_b, X_FAR0 + (s & 1));
-    if (more && npad) {
+    if (more && npad && !gfuse) {
       cur ^= 1;
       const GhostArgs ga = ghost_args();
       cur ^= 1;
@@ -1546,6 +1589,7 @@ struct Level {
       if (split) {
         if (ring_ok) split_step3(s, time, want_state && s == nsteps - 1, s + 1 < nsteps, s == 0);
         else split_step(s, time, want_state && s == nsteps - 1, s + 1 < nsteps);
+        slabs_fresh = gfuse;
         cur ^= 1;
         ++istep;
         HIPCHK(hipGetLastError());
@@ -1557,7 +1601,7 @@ struct Level {
         ib_update_forcing(time, s);
         rec_to_ghost = false;
       }
-      if (npad) {
+      if (npad && !slabs_fresh) {
         GhostArgs ga = ghost_args();
         if (rec_held) {
           // one launch fewer per step (512x256 cylinder: 5 -> 4 launches)
@@ -1567,6 +1611,9 @@ struct Level {
         } else {
           ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, ga);
         }
+      } else if (rec_held) {
+        hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream, held_a, held_r, held_nblk);
+        rec_held = false;
       }
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
@@ -1574,6 +1621,7 @@ struct Level {
       // timing: the dispatch records the kernel's own begin / end
       stepfn(grid, block, stream, sa, timing ? ev_start[s] : nullptr, timing ? ev_stop[s] : nullptr);
       ring_step(stream, s, !last, false);
+      slabs_fresh = gfuse;
       cur ^= 1;
       ++istep;
       HIPCHK(hipGetLastError());
@@ -1855,7 +1903,8 @@ int ilbm_level_info(ilbm_level* h, double out[8]) {
   const ilbm::Level& L = h->L;
   out[0] = L.tau; out[1] = L.dt; out[2] = L.t_new; out[3] = L.istep;
   out[4] = (double)L.band_cells; out[5] = (double)L.popbytes;
-  out[6] = (L.am == ilbm::kAddrWrap) ? 0.0 : (L.am == ilbm::kAddrWall64) ? 1.0 : (L.am == ilbm::kAddrWall32) ? 2.0 : 3.0;
+  out[6] = (L.am == ilbm::kAddrWrap) ? 0.0 : (L.am == ilbm::kAddrWall64) ? 1.0 : (L.am == ilbm::kAddrWall32) ? 2.0
+           : L.gfuse ? 4.0 : 3.0;
   out[7] = (double)L.nblocks;
   return ILBM_OK;
 }

@@ -366,7 +366,8 @@ class Level:
         _check(load().ilbm_level_info(self._h, _dp(out)), "info")
         return {"tau": out[0], "dt": out[1], "time": out[2], "steps": int(out[3]),
                 "band_cells": int(out[4]), "pop_bytes": int(out[5]), "padded": bool(out[6]),
-                "addressing": ("wrap", "wall64", "wall32", "wall32x")[int(out[6])], "blocks": int(out[7])}
+                "addressing": ("wrap", "wall64", "wall32", "wall32x", "wall32x")[int(out[6])],
+                "ghosts_fused": int(out[6]) == 4, "blocks": int(out[7])}
 
     def set_timing(self, enable: bool):
         _check(load().ilbm_level_set_timing(self._h, int(enable)), "set_timing")

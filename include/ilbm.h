@@ -137,7 +137,9 @@ int ilbm_level_restore(ilbm_level* L, const void* host_buf, int64_t bytes);
  * [5] storage bytes per population, [6] layout: 0 wrapped (all periodic),
  * 1 padded with ghost slabs and 64-bit offsets, 2 padded with 32-bit offsets
  * (populations and slabs under 4 GiB), 3 as 2 with non-periodic faces in x
- * only, [7] stream-collide blocks. */
+ * only, 4 as 3 with the x-face ghost slabs written by the stream-collide
+ * kernel itself (ext_dir / user_2 / foextrap x faces), [7] stream-collide
+ * blocks. */
 int ilbm_level_info(ilbm_level* L, double out[8]);
 
 /* Timing hooks for measurement: average device time (ms) of the fused

@@ -198,18 +198,20 @@ def test_batch_large_3d_slice_layout_matches_level():
     assert np.all(np.abs(fh_big[-1] - rec) <= 1e-12 * np.abs(rec) + 1e-10 * np.abs(rec).max())
 
 
-@pytest.mark.parametrize("tile", [0, 64, 128])
-def test_batch_tilings_match_level(tile, monkeypatch):
+@pytest.mark.parametrize("tile,nc", [(0, 1), (64, 1), (128, 1), (0, 2), (64, 2), (128, 2)])
+def test_batch_tilings_match_level(tile, nc, monkeypatch):
     """Each stream-collide tiling of the batched launch (0: linear tiles of
-    256 cells with no row padding; 64 / 128: 2D tiles) gives the single
-    level's state: checkpoint bytes and force histories identical, mass
-    within 1e-13 (block partials differ).  Rows of 150, 80 and 72 cells are
-    not multiples of the tile widths; the 3D case spans slices in one tile."""
+    256 cells with no row padding; 64 / 128: 2D tiles), with one or two
+    cells per thread, gives the single level's state: checkpoint bytes and
+    force histories identical, mass within 1e-13 (block partials differ).
+    Rows of 150, 80 and 72 cells are not multiples of the tile widths; the
+    3D case spans slices in one tile."""
     from ilbmrex_amd.native import Batch, Level
     monkeypatch.setenv("ILBM_BATCH_TILE", str(tile))
+    monkeypatch.setenv("ILBM_BATCH_NC", str(nc))
     cases = mixed_cases(cf.STORE_F32)
     nsteps = 12
-    print(f"[batch] tiling {tile}: {len(cases)} cases, {nsteps} steps", flush=True)
+    print(f"[batch] tiling {tile}, {nc} cells per thread: {len(cases)} cases, {nsteps} steps", flush=True)
     with Batch(BUDGET, graph_steps=4) as B:
         ids = [B.add(c, _init(c), nsteps) for c in cases]
         assert all(B.tile(cid) == tile for cid in ids)

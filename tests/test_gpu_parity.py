@@ -272,3 +272,41 @@ def test_walled_addressing_modes_bitexact(mode, monkeypatch):
     assert lev.info()["addressing"] == want_ch
     assert_bitexact(lev.f(), orc.f(), "f")
     assert_bitexact(lev.macro(), orc.macro(), "macro")
+
+
+@pytest.mark.parametrize("lat,bx", [(cf.D2Q9, (cf.EXT_DIR, cf.FOEXTRAP)), (cf.D2Q9, (cf.USER_2, cf.USER_2)),
+                                    (cf.D3Q19, (cf.FOEXTRAP, cf.EXT_DIR)), (cf.D3Q19, (cf.USER_2, cf.FOEXTRAP)),
+                                    (cf.D3Q27, (cf.EXT_DIR, cf.USER_2))])
+def test_fused_ghost_slabs_bitexact(lat, bx, monkeypatch):
+    """Levels walled in x only with ext_dir / user_2 / foextrap x faces: the
+    stream-collide kernel writes the next step's x-face ghost slabs itself
+    (corner rows by the clamped mirror of ext_dir / user_2 and the periodic
+    image of foextrap).  FP64 against the oracle, and with
+    ILBM_GHOST_SEPARATE=1 (the separate ghost kernel) for every storage."""
+    from ilbmrex_amd.native import Level
+    monkeypatch.delenv("ILBM_GHOST_SEPARATE", raising=False)
+    monkeypatch.delenv("ILBM_ADDR_GENERIC", raising=False)
+    monkeypatch.delenv("ILBM_FORCE_ADDR64", raising=False)
+    n = (17, 11, 1) if lat.dim == 2 else (13, 9, 7)
+    vals = {cf.EXT_DIR: (1.0, 0.03, 0.004, -0.002), cf.USER_2: (1.01, 0.0, 0.02, 0.01), cf.FOEXTRAP: (0.0,) * 4}
+    per = (0, 1, 1)
+    c = cf.Case(lattice=lat, n=n, nu=0.04, periodic=per,
+                bc_lo=(bx[0], cf.INT_DIR, cf.INT_DIR), bc_hi=(bx[1], cf.INT_DIR, cf.INT_DIR),
+                bc_lo_val=(vals[bx[0]], (0.0,) * 4, (0.0,) * 4), bc_hi_val=(vals[bx[1]], (0.0,) * 4, (0.0,) * 4),
+                name=f"fused_{lat.name}_{bx[0]}_{bx[1]}")
+    init = cf.random_init(c, seed=23, du=0.02)
+    lev, orc = run_pair(c, init, 11)
+    assert lev.info()["ghosts_fused"]
+    assert_bitexact(lev.f(), orc.f(), "f")
+    assert_bitexact(lev.macro(), orc.macro(), "macro")
+    for st in (cf.STORE_F32, cf.STORE_F16S):
+        cs = c.replace(storage=st)
+        out = []
+        for sep in ("0", "1"):
+            monkeypatch.setenv("ILBM_GHOST_SEPARATE", sep)
+            L = Level(cs, init)
+            assert L.info()["ghosts_fused"] == (sep == "0")
+            L.advance(11, want_state=True)
+            out.append(L.f())
+            L.close()
+        assert_bitexact(out[0], out[1], f"storage {st} fused vs separate")
