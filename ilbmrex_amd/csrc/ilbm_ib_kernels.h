@@ -74,6 +74,9 @@ struct IbArgs {
   // macro velocity and EulerForceTotal is the one iteration's force
   int ev_from_macro;
   int fuse_explicit;    // explicit / diag force evaluated by k_ib_interp
+  // interpolation and spreading sums in a fixed wave-level reduction tree
+  // (levels with reduced storage) instead of the reference's serial order
+  int tree;
   int first_iter;
   const int* ne_cells;  // band cells that receive spreading entries
   const int* ne_e;      // [2][n_ne]: each such cell's first and last + 1 entry
@@ -320,7 +323,57 @@ __device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, doubl
 // before the cell index arrives, so the chain of dependent memory accesses
 // is cell index -> band velocities -> marker U / F.
 constexpr int kInterpWaves = 4;
+// Interpolation of one marker per wavefront with the sums of the stencil
+// products in a fixed tree over the lanes (wave_sum), not in the serial
+// order of the reference's loop: deterministic, and equal to the serial sum
+// to rounding.  No block barrier: a wave past the last marker leaves.
+__device__ __forceinline__ void interp_tree_body(const IbArgs& a, int blk) {
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+  const int m = blk * kInterpWaves + wv;
+  if (m >= a.nm) return;
+  const int nm = a.nm;
+  const double d = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+  const int npts = (a.dim == 3) ? 125 : 25;
+  const int b0 = a.base[m], b1 = a.base[nm + m], b2 = (a.dim == 3) ? a.base[2 * nm + m] : 0;
+  double tvx = 0.0, tvy = 0.0, tvz = 0.0;
+  if (lane == 0 && a.fuse_explicit) target_velocity(a, m, &tvx, &tvy, &tvz);
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
+  for (int r = 0; r < 2; ++r) {
+    const int t = lane + 64 * r;
+    if (t >= npts) continue;
+    int ii, jj, kk;
+    if (a.dim == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
+    else { ii = t / 5; jj = t % 5; kk = 0; }
+    const double tU = a.wts[(0 * 5 + ii) * nm + m];
+    const double tV = a.wts[(1 * 5 + jj) * nm + m];
+    const double dval = (a.dim == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
+    const int p[3] = {b0 + ii - 2, b1 + jj - 2, (a.dim == 3) ? b2 + kk - 2 : 0};
+    bool ok = !(p[0] < -2 || p[0] > a.n[0] + 1 || p[1] < -2 || p[1] > a.n[1] + 1);
+    if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
+    if (!ok) continue;
+    s0 += dval * d;
+    s1 += dval * ev_value(a, p, 0) * d;
+    s2 += dval * ev_value(a, p, 1) * d;
+    if (a.dim == 3) s3 += dval * ev_value(a, p, 2) * d;
+  }
+  s0 = wave_sum(s0);
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  s3 = wave_sum(s3);
+  if (lane != 0) return;
+  double u = s1, v = s2, w = (a.dim == 3) ? s3 : 0.0;
+  if (a.renorm_interp && s0 > 0.0) { u /= s0; v /= s0; w /= s0; }
+  a.U[m] = u; a.U[nm + m] = v; a.U[2 * nm + m] = w;
+  double F0 = 0.0, F1 = 0.0, F2 = 0.0;
+  if (a.fuse_explicit) explicit_force_tv(a, m, tvx, tvy, tvz, u, v, w, F0, F1, F2);
+  a.F[m] = F0; a.F[nm + m] = F1; a.F[2 * nm + m] = F2;
+}
+
 __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
+  if (a.tree) {
+    interp_tree_body(a, blk);
+    return;
+  }
   __shared__ double prod[kInterpWaves][4][128];
   __shared__ unsigned char valid[kInterpWaves][128];
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -939,7 +992,21 @@ __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const G
   }
   double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
   int cur = 0;
-  if (lane == 0) rounds[wv] = (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
+  if (a.tree) {
+    // the cell's contributions over the lanes, then a fixed tree (periodic
+    // images included in the same sum)
+    for (int e = e0 + lane; e < e1; e += 64) {
+      const int m = a.ent_m[e];
+      const double dval = a.ent_w[e];
+      tot[0] += dval * (a.F[m] * a.dv);
+      tot[1] += dval * (a.F[nm + m] * a.dv);
+      if (a.dim == 3) tot[2] += dval * (a.F[2 * nm + m] * a.dv);
+    }
+    tot[0] = wave_sum(tot[0]);
+    tot[1] = wave_sum(tot[1]);
+    tot[2] = wave_sum(tot[2]);
+  }
+  if (lane == 0) rounds[wv] = a.tree ? 0 : (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
   __syncthreads();
   int nr = 0;
   for (int w2 = 0; w2 < kSpreadWaves; ++w2) nr = rounds[w2] > nr ? rounds[w2] : nr;

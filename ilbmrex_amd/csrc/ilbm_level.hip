@@ -394,6 +394,10 @@ struct Level {
   // k_step writes the x-face ghost slabs of the next step (ghost_write);
   // slabs_fresh: the current buffer's slabs were written by the last step
   bool gfuse = false, slabs_fresh = false;
+  // IB sums in a fixed wave tree: levels with reduced storage, unless
+  // ILBM_IB_EXACT=1 keeps the reference's serial order (FP64 storage always
+  // keeps it)
+  bool ib_tree = false;
   IbArgs held_a{};
   RecArgs held_r{};
   int held_nblk = 0;
@@ -645,6 +649,8 @@ struct Level {
     const char* gsep = std::getenv("ILBM_GHOST_SEPARATE");
     auto fusable = [](int b) { return b == BC_FOEXTRAP || b == BC_EXT_DIR || b == BC_USER_2; };
     gfuse = am == kAddrWallX32 && !cf && fusable(c.bc_lo[0]) && fusable(c.bc_hi[0]) && !(gsep && gsep[0] == '1');
+    const char* ibx = std::getenv("ILBM_IB_EXACT");
+    ib_tree = st != 0 && !(ibx && ibx[0] == '1');
     for (int b = 0; b < 2; ++b) {
       // Q population planes followed by the Q ghost slabs
       const size_t bytes = popbytes * ((size_t)g.pop_n + (size_t)npad * Q);
@@ -958,6 +964,7 @@ struct Level {
     a.use_ivc = (c.coupling_method == 1 && ivc_ready) ? 1 : 0;
     // IVC whose LU failed falls back to explicit forcing (IBMarkerDF.cpp:1253-1259)
     a.fuse_explicit = (c.coupling_method != 1 || !ivc_ready) ? 1 : 0;
+    a.tree = ib_tree ? 1 : 0;
     a.nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
     a.first_iter = 1;
     for (int d = 0; d < 3; ++d) {

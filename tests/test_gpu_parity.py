@@ -310,3 +310,35 @@ def test_fused_ghost_slabs_bitexact(lat, bx, monkeypatch):
             out.append(L.f())
             L.close()
         assert_bitexact(out[0], out[1], f"storage {st} fused vs separate")
+
+
+@pytest.mark.parametrize("shape", ["cylinder", "sphere"])
+def test_reduced_storage_ib_tree_sums_match_serial_order(shape, monkeypatch):
+    """Levels with reduced storage sum the interpolation and spreading
+    products in a fixed wave tree; ILBM_IB_EXACT=1 keeps the reference's
+    serial order.  Both are FP64 arithmetic, so after 40 FP32-storage steps
+    the marker velocities and forces agree to 1e-12 relative, the force
+    record to 1e-10 and the FP32 populations to 1e-5 (a rounding flip of
+    the exported force can move an FP32 population by one ulp)."""
+    from ilbmrex_amd.native import Level
+    if shape == "cylinder":
+        c = cf.cylinder_2d(nx=192, ny=96, D=24.0, xc=48.0, yc=47.7, U=0.05, Re=100.0, storage=cf.STORE_F32)
+        u0 = (0.05, 0.0, 0.0)
+    else:
+        c = cf.sphere_3d(nx=64, ny=48, nz=48, D=12.0, center=(20.0, 24.2, 23.9), U=0.04, Re=50.0,
+                         storage=cf.STORE_F32)
+        u0 = (0.04, 0.0, 0.0)
+    out = {}
+    for exact in ("0", "1"):
+        monkeypatch.setenv("ILBM_IB_EXACT", exact)
+        lev = Level(c, cf.uniform_init(c, u=u0))
+        lev.advance(40, want_state=True)
+        out[exact] = (lev.markers(), lev.force_history(), lev.f())
+        lev.close()
+    (xt, ut, ft), ht, pt = out["0"]
+    (xe, ue, fe), he, pe = out["1"]
+    assert np.array_equal(xt, xe)
+    assert np.abs(ut - ue).max() <= 1e-12 * np.abs(ue).max()
+    assert np.abs(ft - fe).max() <= 1e-12 * np.abs(fe).max()
+    assert np.all(np.abs(ht - he) <= 1e-10 * np.abs(he) + 1e-12 * np.abs(he).max())
+    assert np.abs(pt - pe).max() <= 1e-5
