@@ -233,19 +233,28 @@ def batched_section(args):
     b = serving.serve(reqs_b, budget, chunk=args.chunk, keep_history=True)
     reqs_p = serving.serving_requests(args.requests, rate=rate, seed=20261202, storage=cf.STORE_F64)
     p = serving.serve_plain(reqs_p, storage=cf.STORE_F64, keep_history=True)
-    cd_err = []
+    cd_err, shedding = [], []
     for rb, rp in zip(reqs_b, reqs_p):
         tail = max(1, rb.nsteps // 5)
         c32, c64 = rb.history[-tail:, 4].mean(), rp.history[-tail:, 4].mean()
         cd_err.append(abs(c32 - c64) / abs(c64))
+        # a request whose FP64 lift oscillates sheds vortices; its shedding
+        # phase (seeded by rounding on a symmetric start) differs between the
+        # storages, so a window mean of Cd differs by more than precision
+        shedding.append(bool(rp.history[-tail:, 5].std() > 1e-3))
+    steady = [e for e, sh in zip(cd_err, shedding) if not sh]
     out["serving_config5_stream"] = {
         "requests": args.requests, "rate_per_s": rate, "steps": [2000, 20000], "mix": "70% D2Q9 cylinders, 30% D3Q19 spheres",
         "batched_f32": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in b.items()},
         "plain_f64": {k: (round(v, 4) if isinstance(v, float) else v) for k, v in p.items()},
         "throughput_ratio": round(b["mlups"] / p["mlups"], 3),
         "cd_err_f32_vs_f64_max": float(max(cd_err)),
+        "cd_err_f32_vs_f64_max_steady": float(max(steady)) if steady else None,
+        "requests_within_0p5pct": int(sum(e < 5e-3 for e in cd_err)),
+        "shedding_requests": [{"rid": r.rid, "cd_err": round(e, 5)} for r, e, sh in zip(reqs_b, cd_err, shedding) if sh],
         "cd_err_note": "relative error of each response's mean Cd over its last 20 % of steps, batched FP32 "
-                       "against the same request on the plain FP64 path; max over the requests"}
+                       "against the same request on the plain FP64 path; requests whose FP64 lift oscillates "
+                       "(std > 1e-3) shed vortices with a storage-dependent phase and are listed apart"}
     return out
 
 
@@ -269,7 +278,7 @@ def main():
     ap.add_argument("--roofline-warmup", type=int, default=200)
     ap.add_argument("--accuracy-steps", type=int, default=10000,
                     help="steps of the sphere in FP64 / FP32 / FP16-shifted storage for the error report")
-    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-steps", type=int, default=60)
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")

@@ -335,7 +335,7 @@ struct Batch {
         a.tile_nc = t.nc;
         const int64_t nb = T.tile_blocks(t.bx, t.nc);
         std::vector<BlockRef> m(nb);
-        for (int64_t b = 0; b < nb; ++b) m[b] = BlockRef{0, (int)b};
+        for (int64_t b = 0; b < nb; ++b) m[b] = step_ref(T.c.n, t.bx, t.nc, 0, b);
         HIPCHK(hipMemcpyAsync(da, &a, sizeof(a), hipMemcpyHostToDevice, stream));
         HIPCHK(hipMemcpyAsync(dm, m.data(), sizeof(BlockRef) * nb, hipMemcpyHostToDevice, stream));
         for (int r = 0; r < 3; ++r) sf(dim3((unsigned)nb), stream, da, dm);
@@ -551,7 +551,7 @@ struct Batch {
       L.cur = bc.base_cur;
       const int64_t nb = L.tile_blocks(bc.tile_bx, bc.tile_nc);
       tails[ci] = CaseTail{L.partial, nb, L.mass_dev, L.clock_d, L.dt};
-      for (int64_t b = 0; b < nb; ++b) maps[BatchGroup::M_STEP].push_back({c, (int)b});
+      for (int64_t b = 0; b < nb; ++b) maps[BatchGroup::M_STEP].push_back(step_ref(L.c.n, bc.tile_bx, bc.tile_nc, c, b));
       if (!L.wrap && !L.gfuse) {   // else the stream-collide kernel writes the slabs
         const int64_t npad = L.slabs.face_off[L.slabs.nface];
         for (int64_t b = 0; b < (npad + kBlock - 1) / kBlock; ++b) maps[BatchGroup::M_GHOST].push_back({c, (int)b});

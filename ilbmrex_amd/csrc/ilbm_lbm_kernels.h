@@ -215,7 +215,32 @@ __device__ __forceinline__ typename ST::compute_t arg_t(double d, float f) {
 struct BlockRef {
   int c;
   int b;
+  // stream-collide maps: the first cell of the block's tile (step_ref)
+  int i0, j0, k0;
 };
+
+// Tile width bx (0: linear tiles) and cells per thread nc of a case's
+// batched stream-collide launch: the first cell of block b (its tile's
+// corner; for linear tiles cell nc * 256 * b in x-fastest order).  The host
+// stores it in the block map, so the kernel needs no division.
+__host__ __device__ inline BlockRef step_ref(const int n[3], int bx, int nc, int c, int64_t b) {
+  BlockRef r{c, (int)b, 0, 0, 0};
+  if (bx == 0) {
+    const int64_t lin0 = b * nc * kBlock;
+    const int64_t row0 = lin0 / n[0];
+    r.i0 = (int)(lin0 - row0 * n[0]);
+    r.k0 = (int)(row0 / n[1]);
+    r.j0 = (int)(row0 - (int64_t)r.k0 * n[1]);
+    return r;
+  }
+  const int by = kBlock / bx;
+  const int gx = (n[0] + bx - 1) / bx, gy = (n[1] + nc * by - 1) / (nc * by);
+  const int bxi = (int)(b % gx), rest = (int)(b / gx);
+  r.i0 = (bxi) * bx;
+  r.j0 = (rest % gy) * (nc * by);
+  r.k0 = rest / gy;
+  return r;
+}
 
 // ---------------------------------------------------------------------------
 // Deterministic block reduction (fixed tree for a fixed block shape).
@@ -814,20 +839,26 @@ __global__ void __launch_bounds__(kBlock) k_step_p(StepArgs a, uint32_t tx, uint
 // bx = 0: block b holds cells [nc 256 b, nc 256 (b + 1)) of the case in
 // x-fastest order, cell c of thread t at 256 c + t (no row padding; a cell
 // past the end gets j = n[1], inactive).
-__device__ inline void batch_tile(const GridGeom& g, int bx, int nc, int b, int c, int* i, int* j, int* k) {
+// Cell c (of nc per thread) of the block whose map entry is r: bx > 0, the
+// tile of bx x (nc * by) cells at r's corner, cell c of a thread c * by rows
+// below cell 0; bx = 0, cells [r, r + nc * 256) in x-fastest order, cell c
+// of thread t at 256 c + t (no row padding; a cell past the end gets
+// j = n[1], inactive).
+__device__ inline void batch_tile(const GridGeom& g, int bx, const BlockRef& r, int c, int* i, int* j, int* k) {
   const int t = threadIdx.x;
   if (bx == 0) {
-    // the block's first cell from one (uniform) division per coordinate;
-    // a lane then steps over at most kBlock / n[0] + 1 row ends
-    const uint32_t lin0 = ((uint32_t)b * (uint32_t)nc + (uint32_t)c) * kBlock;
-    const uint32_t nx = (uint32_t)g.n[0], ny = (uint32_t)g.n[1];
-    const uint32_t row0 = lin0 / nx;
-    int ii = (int)(lin0 - row0 * nx) + t;
-    const uint32_t k0 = row0 / ny;
-    int jj = (int)(row0 - k0 * ny), kk = (int)k0;
-    while (ii >= (int)nx) {
-      ii -= (int)nx;
-      if (++jj == (int)ny) { jj = 0; ++kk; }
+    const int nx = g.n[0], ny = g.n[1];
+    // the chunk's first cell (uniform), then the lane's: each steps over at
+    // most kBlock / n[0] + 1 row ends
+    int i0 = r.i0 + c * kBlock, j0 = r.j0, k0 = r.k0;
+    while (i0 >= nx) {
+      i0 -= nx;
+      if (++j0 == ny) { j0 = 0; ++k0; }
+    }
+    int ii = i0 + t, jj = j0, kk = k0;
+    while (ii >= nx) {
+      ii -= nx;
+      if (++jj == ny) { jj = 0; ++kk; }
     }
     *i = ii;
     *j = (kk >= g.n[2]) ? g.n[1] : jj;
@@ -835,13 +866,9 @@ __device__ inline void batch_tile(const GridGeom& g, int bx, int nc, int b, int 
     return;
   }
   const int sh = __builtin_ctz((unsigned)bx);   // bx is 64, 128 or 256
-  const int by = kBlock >> sh;
-  const int gx = (g.n[0] + bx - 1) >> sh, gy = (g.n[1] + nc * by - 1) / (nc * by);
-  const int bxi = b % gx, rest = b / gx;
-  const int byi = rest % gy, bz = rest / gy;
-  *i = (bxi << sh) + (t & (bx - 1));
-  *j = byi * (nc * by) + c * by + (t >> sh);
-  *k = bz;
+  *i = r.i0 + (t & (bx - 1));
+  *j = r.j0 + c * (kBlock >> sh) + (t >> sh);
+  *k = r.k0;
 }
 
 // Many cases of one lattice / storage class in one launch.  On 2D lattices
@@ -856,14 +883,14 @@ __global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ 
   if constexpr (LAT::D == 2) {
     if (a.tile_nc == 2) {
       int i[2], j[2], k[2];
-      batch_tile(a.g, a.tile_bx, 2, r.b, 0, &i[0], &j[0], &k[0]);
-      batch_tile(a.g, a.tile_bx, 2, r.b, 1, &i[1], &j[1], &k[1]);
+      batch_tile(a.g, a.tile_bx, r, 0, &i[0], &j[0], &k[0]);
+      batch_tile(a.g, a.tile_bx, r, 1, &i[1], &j[1], &k[1]);
       step_body_n<LAT, ST, AM, 2>(a, i, j, k, r.b, lds);
       return;
     }
   }
   int i, j, k;
-  batch_tile(a.g, a.tile_bx, 1, r.b, 0, &i, &j, &k);
+  batch_tile(a.g, a.tile_bx, r, 0, &i, &j, &k);
   step_body<LAT, ST, AM>(a, i, j, k, r.b, lds);
 }
 

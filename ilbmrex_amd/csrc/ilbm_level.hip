@@ -365,8 +365,16 @@ static int64_t plane_stride(int64_t cells, int64_t pop_bytes) {
   return ((cells * pop_bytes + 16383) / 16384 * 16384 + 8192) / pop_bytes;
 }
 
-// resident blocks per CU of the far-cell launch of an overlapped step
-constexpr int kFarBlocksPerCu = 4;
+// Overlapped step of an IB level: the far cells start once this step's
+// interpolation is done (1; 2: the export; 0: at once) as a launch of one
+// block per tile (0) or a persistent launch of that many blocks per CU.
+// The interpolation then runs on an idle device, and the spreading kernel
+// is dispatched beside the far cells' launch instead of behind it.
+// Measured on the 256x128x128 sphere (GLUPS, FP32 / FP16-shifted / FP64):
+// at once with 4 blocks per CU 29.0 / 28.3 / 12.9; after the interpolation,
+// full grid 30.7 / 46.1 / 16.3.
+constexpr int kFarBlocksPerCu = 0;
+constexpr int kFarAfter = 1;
 
 struct Level {
   ilbm_case_desc c{};
@@ -487,7 +495,7 @@ struct Level {
   // run as runtime kernels of 3-5 us each on the GPU, which lengthens the
   // far stream's chain; the host has slack, the GPU has not.)
   enum { X_GHOST = 0, X_NEAR = 1, X_EXP = 2, X_REC0 = 3, X_FAR0 = X_REC0 + 4, X_RING0 = X_FAR0 + 2,
-         X_N = X_RING0 + 2 };
+         X_IB = X_RING0 + 2, X_N = X_IB + 1 };
   hipEvent_t xev[X_N] = {};
   uint64_t xseq[X_N] = {};
   uint64_t rec_seq[4] = {};
@@ -513,6 +521,12 @@ struct Level {
   // resident blocks of the far-cell launch (k_step_p) of an overlapped step:
   // ILBM_FAR_BLOCKS_PER_CU per CU (0: one block per tile)
   int far_persist = 0;
+  // the far cells of a three-region step start after this step's
+  // interpolation (1) or export (2), 0: at once (ILBM_FAR_AFTER=interp |
+  // export | none)
+  int far_after = 0;
+  int ib_mark = 0;          // ib_update_forcing signals X_IB after that stage
+  uint64_t ib_sig = 0;
   int ring_lo[3] = {0, 0, 0}, ring_n[3] = {0, 0, 0};
   dim3 grid_ring;
   int64_t nblocks_ring = 0;
@@ -1035,6 +1049,7 @@ struct Level {
       a.first_iter = (it == 0) ? 1 : 0;
       hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
                          stream, a);
+      if (ib_mark == 1 && nloop == 1 && !use_ivc) ib_sig = xsignal(stream, X_IB);
       if (use_ivc) {
         hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
         hipLaunchKernelGGL(k_ib_ivc_matvec, dim3((nm + 3) / 4), dim3(256), 0, stream, a);
@@ -1058,6 +1073,7 @@ struct Level {
     const int mode = fix_mode();
     const unsigned gfix = fix_blocks();
     if (gfix) fixfn(mode, dim3(gfix), dim3(tb), stream, a, g, gbuf[cur], tau, r);
+    if (ib_mark == 2 || (ib_mark == 1 && (nloop != 1 || use_ivc))) ib_sig = xsignal(stream, X_IB);
     if (nloop == 1) {
       // the marker sums feed only the force record (a sub-step without a
       // record is followed by one that has it)
@@ -1331,6 +1347,11 @@ struct Level {
       HIPCHK(hipGetDevice(&dev));
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
       far_persist = per_cu * ncu;
+      far_after = kFarAfter;
+      if (const char* e = std::getenv("ILBM_FAR_AFTER")) {
+        const std::string v(e);
+        far_after = (v == "interp") ? 1 : (v == "export") ? 2 : 0;
+      }
       for (int d = 0; d < 3; ++d) {
         int lo = band_lo[d] - 1, hi = band_lo[d] + band_n[d];
         if (d >= c.dim) { lo = 0; hi = 0; }
@@ -1487,11 +1508,22 @@ struct Level {
     sr.partial = rring + (int64_t)slot * nblocks_ring;
     const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
     const dim3 ggrid((unsigned)((npad + kGhostBlock - 1) / kGhostBlock));
+    // with far_after, stream A's IB forcing is enqueued first and signals
+    // the stage the far cells wait for (the far cells read and write
+    // nothing the forcing touches; the wait only orders the two)
+    if (far_after) {
+      ring_slot = async_record() ? s % kRecRing : -1;
+      ib_mark = far_after;
+      ib_update_forcing(time, s);
+      ib_mark = 0;
+      ring_slot = -1;
+    }
     // stream B
     if (!first) xwait(stream_b, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
     flush_mass(stream_b);
     if (!ghost_ahead && npad && !slabs_fresh) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
     ghost_ahead = false;
+    if (far_after) xwait(stream_b, X_IB, ib_sig);
     stepfn(grid, block, stream_b, sf, nullptr, nullptr);
     far_sig[s & 1] = xsignal(stream_b, X_FAR0 + (s & 1));
     if (more && npad && !gfuse) {
@@ -1502,9 +1534,11 @@ struct Level {
       ghost_ahead = true;
     }
     // stream A
-    ring_slot = async_record() ? s % kRecRing : -1;
-    ib_update_forcing(time, s);
-    ring_slot = -1;
+    if (!far_after) {
+      ring_slot = async_record() ? s % kRecRing : -1;
+      ib_update_forcing(time, s);
+      ring_slot = -1;
+    }
     stepfn(grid_near, block, stream, sn, nullptr, nullptr);
     if (!first) xwait(stream, X_FAR0 + ((s - 1) & 1), far_sig[(s - 1) & 1]);
     stepfn(grid_ring, block, stream, sr, nullptr, nullptr);

@@ -76,17 +76,31 @@ def test_overlap_sphere_off_walls_split_advances():
     b.close()
 
 
-@pytest.mark.parametrize("regions", ["three", "two"])
+@pytest.mark.parametrize("regions", ["three", "two", "three_at_once", "three_after_export",
+                                     "three_persistent", "three_separate_slabs"])
 def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
     """70 overlapped steps (the Sum(Rho) ring halves fill and are reduced
     twice, the alternating far / ring signals run through every parity)
     against the single-stream step: bit-identical state and force history.
     "three": far cells, near box and the ring between them as three
-    launches; "two": ILBM_OVERLAP_TWO_REGIONS=1 keeps far + near."""
+    launches, the far cells after the step's interpolation; "two":
+    ILBM_OVERLAP_TWO_REGIONS=1 keeps far + near; the far cells at once or
+    after the step's export (ILBM_FAR_AFTER), as a persistent launch of 4
+    blocks per CU (ILBM_FAR_BLOCKS_PER_CU=4), and with the separate ghost
+    slab kernel (ILBM_GHOST_SEPARATE=1)."""
     from ilbmrex_amd.native import Level
-    monkeypatch.delenv("ILBM_OVERLAP_TWO_REGIONS", raising=False)
+    for v in ("ILBM_OVERLAP_TWO_REGIONS", "ILBM_FAR_AFTER", "ILBM_FAR_BLOCKS_PER_CU", "ILBM_GHOST_SEPARATE"):
+        monkeypatch.delenv(v, raising=False)
     if regions == "two":
         monkeypatch.setenv("ILBM_OVERLAP_TWO_REGIONS", "1")
+    if regions == "three_at_once":
+        monkeypatch.setenv("ILBM_FAR_AFTER", "none")
+    if regions == "three_after_export":
+        monkeypatch.setenv("ILBM_FAR_AFTER", "export")
+    if regions == "three_persistent":
+        monkeypatch.setenv("ILBM_FAR_BLOCKS_PER_CU", "4")
+    if regions == "three_separate_slabs":
+        monkeypatch.setenv("ILBM_GHOST_SEPARATE", "1")
     c = cf.sphere_3d(nx=128, ny=96, nz=96, D=10.0, center=(48.0, 47.6, 48.3), U=0.04, Re=50.0,
                      storage=cf.STORE_F32)
     init = cf.uniform_init(c, u=(0.04, 0.0, 0.0))
