@@ -161,7 +161,9 @@ def amr_variant(steps):
     from ilbmrex_amd.native import AmrHierarchy
     # Re = 1000 at this resolution diverges under BGK (the FP64 oracle of
     # the same hierarchy blows up within 12 coarse steps, explicit or IVC
-    # forcing); Re = 100 is the run of the same workload that stays finite.
+    # forcing).  Of Re = 100, 110, 125, 150, ..., 1000 over 10,000 coarse
+    # steps, 100 is the largest whose forces stay physical (110 and 125
+    # stay finite with |Cl| >> 1, 150 and above go non-finite).
     case, boxes = cf.pitching_naca_amr(storage=cf.STORE_F32, Re=100.0)
     log(f"[bench] amr3_pitching_naca_f32: {steps} coarse steps ({4 * steps} finest)")
     h = AmrHierarchy(case, boxes, 10, (1.0, case.U0, 0.0, 0.0))
@@ -179,7 +181,12 @@ def amr_variant(steps):
     h.close()
     cells = info_end["cells"]
     updates = cells[0] + 2 * cells[1] + 4 * cells[2]
+    tail = fh[-min(2000, len(fh)):]
+    forces = {"window_coarse_steps": len(tail), "cd_mean": float(np.mean(tail[:, 4])),
+              "cd_std": float(np.std(tail[:, 4])), "cl_mean": float(np.mean(tail[:, 5])),
+              "cl_rms": float(np.sqrt(np.mean((tail[:, 5] - np.mean(tail[:, 5])) ** 2)))}
     return {"mlups_all_levels": updates * steps / wall / 1e6, "ms_per_coarse_step": 1e3 * wall / steps,
+            "coarse_steps": steps, "finest_steps": 4 * steps, "forces_last_window": forces,
             "finest_steps_per_s": 4 * steps / wall, "cells_per_level": cells,
             "cells_per_level_start": info["cells"], "boxes_end": boxes_end, "regrids_changing_boxes": nregrid,
             "markers": 256, "cd_mean_last_100": float(np.mean(fh[-100:, 4])),
@@ -282,7 +289,8 @@ def main():
     ap.add_argument("--no-variants", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--amr-steps", type=int, default=1000)
+    ap.add_argument("--amr-steps", type=int, default=10000,
+                    help="timed coarse steps of the pitching hierarchy (4 finest steps each)")
     ap.add_argument("--ensemble-steps", type=int, default=5000)
     ap.add_argument("--requests", type=int, default=32)
     ap.add_argument("--rates", type=float, nargs="+", default=[2.0, 8.0, 32.0])

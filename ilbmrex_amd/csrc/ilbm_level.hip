@@ -375,6 +375,7 @@ static int64_t plane_stride(int64_t cells, int64_t pop_bytes) {
 // full grid 30.7 / 46.1 / 16.3.
 constexpr int kFarBlocksPerCu = 0;
 constexpr int kFarAfter = 1;
+constexpr int kRingStream = 0;   // Level::RING_NEAR
 
 struct Level {
   ilbm_case_desc c{};
@@ -495,7 +496,7 @@ struct Level {
   // run as runtime kernels of 3-5 us each on the GPU, which lengthens the
   // far stream's chain; the host has slack, the GPU has not.)
   enum { X_GHOST = 0, X_NEAR = 1, X_EXP = 2, X_REC0 = 3, X_FAR0 = X_REC0 + 4, X_RING0 = X_FAR0 + 2,
-         X_IB = X_RING0 + 2, X_N = X_IB + 1 };
+         X_IB = X_RING0 + 2, X_NEAR0 = X_IB + 1, X_N = X_NEAR0 + 2 };
   hipEvent_t xev[X_N] = {};
   uint64_t xseq[X_N] = {};
   uint64_t rec_seq[4] = {};
@@ -526,6 +527,14 @@ struct Level {
   // export | none)
   int far_after = 0;
   int ib_mark = 0;          // ib_update_forcing signals X_IB after that stage
+  // the stream of a three-region step's ring (ILBM_RING_STREAM=near | far |
+  // own): the near box's, the far cells' after them, or a fourth stream
+  // beside both
+  enum { RING_NEAR = 0, RING_FAR = 1, RING_OWN = 2 };
+  int ring_stream = RING_NEAR;
+  bool ring_on_far = false;
+  hipStream_t stream_d = nullptr;
+  hipEvent_t ev_d = nullptr;
   uint64_t ib_sig = 0;
   int ring_lo[3] = {0, 0, 0}, ring_n[3] = {0, 0, 0};
   dim3 grid_ring;
@@ -539,13 +548,15 @@ struct Level {
     for (auto e : ev_stop) (void)hipEventDestroy(e);
     if (stream_b) (void)hipStreamSynchronize(stream_b);
     if (stream_c) (void)hipStreamSynchronize(stream_c);
-    for (hipEvent_t e : {ev_a, ev_b, ev_c})
+    if (stream_d) (void)hipStreamSynchronize(stream_d);
+    for (hipEvent_t e : {ev_a, ev_b, ev_c, ev_d})
       if (e) (void)hipEventDestroy(e);
     for (int k = 1; k < kRecRing; ++k) { dev_free(F_ring[k]); dev_free(fp_ring[k]); }
     for (hipEvent_t e : xev)
       if (e) (void)hipEventDestroy(e);
     if (stream_b) (void)hipStreamDestroy(stream_b);
     if (stream_c) (void)hipStreamDestroy(stream_c);
+    if (stream_d) (void)hipStreamDestroy(stream_d);
     dev_free(mring); dev_free(nring); dev_free(rring);
     dev_free(gbuf[0]); dev_free(gbuf[1]);
     for (int f = 0; f < 6; ++f) dev_free(faces.p[f]);
@@ -1348,6 +1359,12 @@ struct Level {
       HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
       far_persist = per_cu * ncu;
       far_after = kFarAfter;
+      ring_stream = kRingStream;
+      if (const char* e = std::getenv("ILBM_RING_STREAM")) {
+        const std::string v(e);
+        ring_stream = (v == "far") ? RING_FAR : (v == "own") ? RING_OWN : RING_NEAR;
+      }
+      ring_on_far = ring_stream == RING_FAR;
       if (const char* e = std::getenv("ILBM_FAR_AFTER")) {
         const std::string v(e);
         far_after = (v == "interp") ? 1 : (v == "export") ? 2 : 0;
@@ -1384,6 +1401,10 @@ struct Level {
       HIPCHK(hipStreamCreateWithFlags(&stream_b, hipStreamNonBlocking));
       for (hipEvent_t* e : {&ev_a, &ev_b})
         HIPCHK(hipEventCreateWithFlags(e, hipEventDisableTiming));
+      if (ring_ok && ring_stream == RING_OWN) {
+        HIPCHK(hipStreamCreateWithFlags(&stream_d, hipStreamNonBlocking));
+        HIPCHK(hipEventCreateWithFlags(&ev_d, hipEventDisableTiming));
+      }
       for (hipEvent_t& e : xev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
       if (async_record()) {
         HIPCHK(hipStreamCreateWithFlags(&stream_c, hipStreamNonBlocking));
@@ -1487,7 +1508,7 @@ struct Level {
     pend = PendingMass{};
   }
 
-  uint64_t far_sig[2] = {0, 0}, ring_sig[2] = {0, 0};
+  uint64_t far_sig[2] = {0, 0}, ring_sig[2] = {0, 0}, near_sig[2] = {0, 0};
 
   void split_step3(int s, double time, bool want, bool more, bool first) {
     StepArgs sf = step_args(want);
@@ -1508,6 +1529,99 @@ struct Level {
     sr.partial = rring + (int64_t)slot * nblocks_ring;
     const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
     const dim3 ggrid((unsigned)((npad + kGhostBlock - 1) / kGhostBlock));
+    if (stream_d) {
+      // Stream A: IB forcing, a wait for the ring of the last step, the near
+      // box.  Stream B: a wait for the ring of the last step, the far cells
+      // (after the forcing's stage far_after).  Stream D: waits for the far
+      // cells and the near box of the last step, the ring, the Sum(Rho) of a
+      // finished half of the slot ring, the ring's signal.  The ring of step
+      // s reads only cells of step s - 1's output and overwrites only what
+      // step s - 1's launches read, so it runs beside step s's far cells and
+      // near box; both of those of step s + 1 wait for it.
+      ring_slot = async_record() ? s % kRecRing : -1;
+      ib_mark = far_after;
+      ib_update_forcing(time, s);
+      ib_mark = 0;
+      ring_slot = -1;
+      if (!first) xwait(stream_b, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
+      if (!ghost_ahead && npad && !slabs_fresh) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+      ghost_ahead = false;
+      if (far_after) xwait(stream_b, X_IB, ib_sig);
+      stepfn(grid, block, stream_b, sf, nullptr, nullptr);
+      far_sig[s & 1] = xsignal(stream_b, X_FAR0 + (s & 1));
+      if (more && npad && !gfuse) {
+        cur ^= 1;
+        const GhostArgs ga = ghost_args();
+        cur ^= 1;
+        ghostfn(ggrid, dim3(kGhostBlock), stream_b, ga);
+        ghost_ahead = true;
+      }
+      if (!first) {
+        xwait(stream_d, X_FAR0 + ((s - 1) & 1), far_sig[(s - 1) & 1]);
+        xwait(stream_d, X_NEAR0 + ((s - 1) & 1), near_sig[(s - 1) & 1]);
+      }
+      stepfn(grid_ring, block, stream_d, sr, nullptr, nullptr);
+      flush_mass(stream_d);
+      ring_sig[s & 1] = xsignal(stream_d, X_RING0 + (s & 1));
+      if (!first) xwait(stream, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
+      stepfn(grid_near, block, stream, sn, nullptr, nullptr);
+      near_sig[s & 1] = xsignal(stream, X_NEAR0 + (s & 1));
+      ++mring_used;
+      const int n = mring_used - mring_base;
+      if (n == mring_slots / 2 || !more) {
+        pend.base = mring_base;
+        pend.n = n;
+        pend.s = s;
+        mring_base = mring_used;
+        if (mring_used == mring_slots || !more) mring_used = mring_base = 0;
+      }
+      return;
+    }
+    if (ring_on_far) {
+      // Stream A: IB forcing, a wait for the ring of the last step, the near
+      // box.  Stream B: the far cells (after the forcing's stage far_after),
+      // a wait for the near box of the last step, the ring, the Sum(Rho) of
+      // a finished half of the slot ring, the ring's signal.  The ring of
+      // step s reads what the far cells, the ring and the near box of step
+      // s - 1 wrote next to it and overwrites what they read; the near box
+      // of step s + 1 reads the ring's cells; the IB forcing reads only the
+      // band, inside the near box.
+      ring_slot = async_record() ? s % kRecRing : -1;
+      ib_mark = far_after;
+      ib_update_forcing(time, s);
+      ib_mark = 0;
+      ring_slot = -1;
+      if (!ghost_ahead && npad && !slabs_fresh) ghostfn(ggrid, dim3(kGhostBlock), stream_b, ghost_args());
+      ghost_ahead = false;
+      if (far_after) xwait(stream_b, X_IB, ib_sig);
+      stepfn(grid, block, stream_b, sf, nullptr, nullptr);
+      if (more && npad && !gfuse) {
+        cur ^= 1;
+        const GhostArgs ga = ghost_args();
+        cur ^= 1;
+        ghostfn(ggrid, dim3(kGhostBlock), stream_b, ga);
+        ghost_ahead = true;
+      }
+      if (!first) xwait(stream_b, X_NEAR0 + ((s - 1) & 1), near_sig[(s - 1) & 1]);
+      stepfn(grid_ring, block, stream_b, sr, nullptr, nullptr);
+      // the pending half (steps up to s - 1: their near boxes are waited for
+      // above) before the signal, so a near box that reuses its slots waits
+      flush_mass(stream_b);
+      ring_sig[s & 1] = xsignal(stream_b, X_RING0 + (s & 1));
+      if (!first) xwait(stream, X_RING0 + ((s - 1) & 1), ring_sig[(s - 1) & 1]);
+      stepfn(grid_near, block, stream, sn, nullptr, nullptr);
+      near_sig[s & 1] = xsignal(stream, X_NEAR0 + (s & 1));
+      ++mring_used;
+      const int n = mring_used - mring_base;
+      if (n == mring_slots / 2 || !more) {
+        pend.base = mring_base;
+        pend.n = n;
+        pend.s = s;
+        mring_base = mring_used;
+        if (mring_used == mring_slots || !more) mring_used = mring_base = 0;
+      }
+      return;
+    }
     // with far_after, stream A's IB forcing is enqueued first and signals
     // the stage the far cells wait for (the far cells read and write
     // nothing the forcing touches; the wait only orders the two)
@@ -1612,6 +1726,7 @@ struct Level {
     if (split) {
       HIPCHK(hipEventRecord(ev_a, stream));
       HIPCHK(hipStreamWaitEvent(stream_b, ev_a, 0));
+      if (stream_d) HIPCHK(hipStreamWaitEvent(stream_d, ev_a, 0));
     }
     if (timing) {
       while ((int)ev_start.size() < nsteps) {
@@ -1669,10 +1784,19 @@ struct Level {
     }
     if (split) {
       if (ring_ok) {
-        // the last step's Sum(Rho) once its ring is done
+        // the last step's Sum(Rho) once its ring (and near box) is done
         const int last = nsteps - 1;
-        xwait(stream_b, X_RING0 + (last & 1), ring_sig[last & 1]);
-        flush_mass(stream_b);
+        if (stream_d) {
+          xwait(stream_d, X_NEAR0 + (last & 1), near_sig[last & 1]);
+          xwait(stream_d, X_FAR0 + (last & 1), far_sig[last & 1]);
+          flush_mass(stream_d);
+          HIPCHK(hipEventRecord(ev_d, stream_d));
+          HIPCHK(hipStreamWaitEvent(stream, ev_d, 0));
+        } else {
+          if (ring_on_far) xwait(stream_b, X_NEAR0 + (last & 1), near_sig[last & 1]);
+          else xwait(stream_b, X_RING0 + (last & 1), ring_sig[last & 1]);
+          flush_mass(stream_b);
+        }
       }
       HIPCHK(hipEventRecord(ev_b, stream_b));
       HIPCHK(hipStreamWaitEvent(stream, ev_b, 0));

@@ -77,7 +77,8 @@ def test_overlap_sphere_off_walls_split_advances():
 
 
 @pytest.mark.parametrize("regions", ["three", "two", "three_at_once", "three_after_export",
-                                     "three_persistent", "three_separate_slabs"])
+                                     "three_persistent", "three_separate_slabs", "three_ring_on_far",
+                                     "three_ring_on_near", "three_ring_own_stream"])
 def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
     """70 overlapped steps (the Sum(Rho) ring halves fill and are reduced
     twice, the alternating far / ring signals run through every parity)
@@ -87,9 +88,11 @@ def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
     ILBM_OVERLAP_TWO_REGIONS=1 keeps far + near; the far cells at once or
     after the step's export (ILBM_FAR_AFTER), as a persistent launch of 4
     blocks per CU (ILBM_FAR_BLOCKS_PER_CU=4), and with the separate ghost
-    slab kernel (ILBM_GHOST_SEPARATE=1)."""
+    slab kernel (ILBM_GHOST_SEPARATE=1); the ring on the far cells', the
+    near box's or its own stream (ILBM_RING_STREAM)."""
     from ilbmrex_amd.native import Level
-    for v in ("ILBM_OVERLAP_TWO_REGIONS", "ILBM_FAR_AFTER", "ILBM_FAR_BLOCKS_PER_CU", "ILBM_GHOST_SEPARATE"):
+    for v in ("ILBM_OVERLAP_TWO_REGIONS", "ILBM_FAR_AFTER", "ILBM_FAR_BLOCKS_PER_CU", "ILBM_GHOST_SEPARATE",
+              "ILBM_RING_STREAM"):
         monkeypatch.delenv(v, raising=False)
     if regions == "two":
         monkeypatch.setenv("ILBM_OVERLAP_TWO_REGIONS", "1")
@@ -101,6 +104,12 @@ def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
         monkeypatch.setenv("ILBM_FAR_BLOCKS_PER_CU", "4")
     if regions == "three_separate_slabs":
         monkeypatch.setenv("ILBM_GHOST_SEPARATE", "1")
+    if regions == "three_ring_on_far":
+        monkeypatch.setenv("ILBM_RING_STREAM", "far")
+    if regions == "three_ring_on_near":
+        monkeypatch.setenv("ILBM_RING_STREAM", "near")
+    if regions == "three_ring_own_stream":
+        monkeypatch.setenv("ILBM_RING_STREAM", "own")
     c = cf.sphere_3d(nx=128, ny=96, nz=96, D=10.0, center=(48.0, 47.6, 48.3), U=0.04, Re=50.0,
                      storage=cf.STORE_F32)
     init = cf.uniform_init(c, u=(0.04, 0.0, 0.0))
