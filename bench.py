@@ -21,16 +21,22 @@ Also reported:
   against the FP64 run;
 - the periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6; FP32, FP64,
   FP16-shifted; 200 warm-up and 1000 timed steps);
+- the 512x256 D2Q9 cylinder (FP64, explicit and IVC, and as a one-case
+  graph-replayed batch);
+- the three-level pitching NACA 0012 hierarchy (FP32, Re = 100, regrid every
+  10 coarse steps, 10,000 coarse steps) with force statistics;
 - a plain copy kernel's bandwidth;
-- the CPU oracle (FP64, OpenMP), timed on a bounded sample.
+- the CPU oracle (FP64, OpenMP), timed on a bounded sample of the headline
+  case (60 steps), with the host CPU's model string.
 
 ``batched``: the 256-case D2Q9 cylinder ensemble (FP32, 5000 steps, one
 launch per kernel for all cases of a lane, two lanes on two streams) against
 the plain per-case FP64 path, with the Cd error of FP32 against FP64 on a
-sample; and the
-continuous-batching server on Poisson arrivals of mixed cylinder / sphere
-requests (p50 / p99 latency, aggregate MLUPS), plus the same short request
-stream served by the plain per-case FP64 path.
+sample; the continuous-batching server on Poisson arrivals of mixed
+cylinder / sphere requests at 2, 8 and 32 requests/s (p50 / p99 latency,
+aggregate MLUPS); and one stream of that mix (2,000-20,000 steps per
+request) served by both the batched FP32 server and the plain per-case FP64
+path, with the throughput ratio and each response's Cd error.
 """
 import argparse
 import json

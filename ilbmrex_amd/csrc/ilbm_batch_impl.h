@@ -140,9 +140,9 @@ static int64_t case_footprint(const ilbm_case_desc& c0, int nsteps) {
   const int Q = c.q;
   const int64_t cells = (int64_t)c.n[0] * c.n[1] * c.n[2];
   const int pb = (c.storage == 0) ? 8 : (c.storage == 1) ? 4 : 2;
-  const int64_t sl = (int64_t)c.n[0] * c.n[1];
+  const int64_t sl = row_stride(c.dim, c.n[0]) * c.n[1];
   const int64_t qs = slice_layout(c.dim, cells) ? ((sl * pb + 16383) / 16384 * 16384 + 8192) / pb * c.n[2]
-                                                 : plane_stride(cells, pb);
+                                                 : plane_stride(sl * c.n[2], pb);
   int64_t slab = 0, faces = 0;
   for (int d = 0; d < c.dim; ++d) {
     if (c.periodic[d]) continue;
@@ -179,6 +179,9 @@ static std::map<int64_t, Tile>& tile_table() {
   return t;
 }
 
+// two-lane batches run their lanes in phase (Batch::lanes_together)
+constexpr bool kLanesTogether = false;
+
 struct Batch {
   hipStream_t stream = nullptr;
   // A class of two or more cases is split into two lanes of balanced cell
@@ -194,6 +197,11 @@ struct Batch {
   // stream-collide) instead of in phase
   hipEvent_t ev_phase = nullptr;
   bool phase_pending = false;
+  // lanes in phase (ILBM_BATCH_PHASE=together): both lanes' IB kernels of a
+  // step first, then both stream-collide launches once both are done, so no
+  // IB kernel waits behind the dispatch of the other lane's stream-collide
+  bool lanes_together = kLanesTogether;
+  hipEvent_t ev_ib[2] = {nullptr, nullptr};
   // two lanes when the active cases hold at least 2^22 cells: a step is then
   // long enough for eager launches, and a graph replay would run the two
   // lanes' branches one after the other on one queue (measured), so two-lane
@@ -224,6 +232,8 @@ struct Batch {
     if (ev_fork) (void)hipEventDestroy(ev_fork);
     if (ev_join) (void)hipEventDestroy(ev_join);
     if (ev_phase) (void)hipEventDestroy(ev_phase);
+    for (hipEvent_t e : ev_ib)
+      if (e) (void)hipEventDestroy(e);
     if (stream2) (void)hipStreamDestroy(stream2);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -609,6 +619,13 @@ struct Batch {
   // One step of a group on its lane's stream; lane0_ib: set once lane 0's
   // first IB kernels after a fork are queued (and ev_phase recorded)
   void enqueue_group(BatchGroup& G, int p, hipStream_t stream, bool& lane0_ib) {
+    enqueue_group_ib(G, p, stream, lane0_ib);
+    enqueue_group_lbm(G, p, stream);
+  }
+
+  // the IB forcing with the export / Guo fix-up and the force records of a
+  // group without slabs
+  void enqueue_group_ib(BatchGroup& G, int p, hipStream_t stream, bool& lane0_ib) {
     const BlockRef* m = G.dmap;
     auto grid = [&](int k) { return dim3((unsigned)G.mcnt[k]); };
     if (G.mcnt[BatchGroup::M_IBCASE] > 0) {
@@ -648,6 +665,13 @@ struct Batch {
         lane0_ib = true;
       }
     }
+  }
+
+  // the slabs (with the records when the group has slabs), stream-collide,
+  // Sum(Rho) and the clock
+  void enqueue_group_lbm(BatchGroup& G, int p, hipStream_t stream) {
+    const BlockRef* m = G.dmap;
+    auto grid = [&](int k) { return dim3((unsigned)G.mcnt[k]); };
     if (G.mcnt[BatchGroup::M_GHOST] > 0 && G.mcnt[BatchGroup::M_IBCASE] > 0)
       G.ghost_rec((unsigned)(G.mcnt[BatchGroup::M_GHOST] + G.mcnt[BatchGroup::M_IBCASE]), stream, G.dghost[p],
                   m + G.moff[BatchGroup::M_GHOST], (unsigned)G.mcnt[BatchGroup::M_GHOST], G.dib[p],
@@ -660,6 +684,20 @@ struct Batch {
 
   void enqueue_step(int p) {
     bool lane0_ib = false;
+    if (lanes_together && nlanes > 1) {
+      for (int lane = 0; lane < 2; ++lane)
+        for (auto& G : groups)
+          if (G.lane == lane) enqueue_group_ib(G, p, lane ? stream2 : stream, lane0_ib);
+      HIPCHK(hipEventRecord(ev_ib[0], stream));
+      HIPCHK(hipEventRecord(ev_ib[1], stream2));
+      HIPCHK(hipStreamWaitEvent(stream, ev_ib[1], 0));
+      HIPCHK(hipStreamWaitEvent(stream2, ev_ib[0], 0));
+      phase_pending = false;
+      for (int lane = 0; lane < 2; ++lane)
+        for (auto& G : groups)
+          if (G.lane == lane) enqueue_group_lbm(G, p, lane ? stream2 : stream);
+      return;
+    }
     for (int lane = 0; lane < 2; ++lane) {
       for (auto& G : groups) {
         if (G.lane != lane) continue;
@@ -792,6 +830,8 @@ int ilbm_batch_create(int64_t hbm_budget_bytes, int max_cases, int graph_steps, 
     HIPCHK(hipEventCreateWithFlags(&h->B.ev_fork, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->B.ev_join, hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&h->B.ev_phase, hipEventDisableTiming));
+    for (hipEvent_t& e : h->B.ev_ib) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (const char* e = std::getenv("ILBM_BATCH_PHASE")) h->B.lanes_together = std::string(e) == "together";
   });
   if (st != ILBM_OK) {
     delete h;

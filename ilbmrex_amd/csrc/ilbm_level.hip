@@ -357,8 +357,40 @@ static bool lattice_matches(const ilbm_case_desc& c) {
 // +8 KiB): 256x128x128 FP32 4.62 -> 5.10, FP64 5.30 -> 5.40; 256^3 FP32
 // 5.64 -> 5.61, FP64 5.47 -> 5.41, FP16 unchanged.  Odd multiples of 4 KiB
 // did as well on the FP32 box but cost 7 % at 256^3 FP64.
-// 3D levels of at least 2^20 cells store z-slices of all populations.
-static bool slice_layout(int dim, int64_t cells) { return dim == 3 && cells >= ((int64_t)1 << 20); }
+// 3D levels of at least kSliceMinCells cells store z-slices of all
+// populations (ILBM_SLICE_MIN_CELLS overrides the threshold).  Every 3D
+// level does: with padded rows (below) a 98^3 FP32 level went from 2.9 to
+// 4.2 TB/s in slices, against 3.8 in planes.
+constexpr int64_t kSliceMinCells = 0;
+static int64_t slice_min_cells() {
+  static const int64_t v = [] {
+    const char* e = std::getenv("ILBM_SLICE_MIN_CELLS");
+    return e ? (int64_t)std::atoll(e) : kSliceMinCells;
+  }();
+  return v;
+}
+static bool slice_layout(int dim, int64_t cells) { return dim == 3 && cells >= slice_min_cells(); }
+
+// Rows of 3D levels padded to a multiple of kRowAlign cells (ILBM_ROW_ALIGN
+// overrides; 1: none): every row starts on a 128-byte boundary in FP32, so
+// a wavefront's 64 consecutive cells touch two cache lines, not three.
+// D3Q19 FP32 cubes, TB/s unpadded -> padded: 98^3 2.9 -> 4.2, 130^3 3.3 ->
+// 4.5; the batch of config 5's 36 spheres 19.8 -> 25.1 GLUPS.  2D levels
+// keep dense rows: the 2D ensemble lost 5 % with padding (its batched
+// linear tiles run across row ends).
+constexpr int kRowAlign = 32;
+static int row_align() {
+  static const int v = [] {
+    const char* e = std::getenv("ILBM_ROW_ALIGN");
+    return e ? std::max(1, std::atoi(e)) : kRowAlign;
+  }();
+  return v;
+}
+static int64_t row_stride(int dim, int n0) {
+  if (dim != 3) return n0;   // 2D: the batched linear tiles span dense rows
+  const int a = row_align();
+  return ((int64_t)n0 + a - 1) / a * a;
+}
 
 static int64_t plane_stride(int64_t cells, int64_t pop_bytes) {
   if (cells < ((int64_t)1 << 20)) return (cells + 127) / 128 * 128;
@@ -605,12 +637,12 @@ struct Level {
     g.dim = c.dim;
     for (int d = 0; d < 3; ++d) g.n[d] = c.n[d];
     g.pad = 0;
-    g.js = c.n[0];
+    g.js = row_stride(c.dim, c.n[0]);
     g.off0 = 0;
     {
       const int64_t pb = (st == 0) ? 8 : (st == 1) ? 4 : 2;
-      const int64_t slice = (int64_t)c.n[0] * c.n[1], cells = slice * c.n[2];
-      if (slice_layout(c.dim, cells)) {
+      const int64_t slice = g.js * c.n[1], cells = slice * c.n[2];
+      if (slice_layout(c.dim, (int64_t)c.n[0] * c.n[1] * c.n[2])) {
         // [z][q][y][x]: a z-slice holds the Q populations of its cells, so a
         // cell's 19 pulls and 19 stores fall in 3 slices instead of 19
         // planes (a trivial 19+19-stream kernel: 5.7 -> 5.95 TB/s)

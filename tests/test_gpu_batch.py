@@ -228,11 +228,14 @@ def test_batch_tilings_match_level(tile, nc, monkeypatch):
             lev.close()
 
 
-def test_two_lane_batch_matches_levels():
+@pytest.mark.parametrize("phase", ["staggered", "together"])
+def test_two_lane_batch_matches_levels(phase, monkeypatch):
     """A batch of at least 2^22 active cells splits each class into two lanes
-    on two streams (advanced eagerly); every case's state and force history
-    equal the single level's, mass within 1e-13."""
+    on two streams (advanced eagerly), half a step apart or in phase
+    (ILBM_BATCH_PHASE); every case's state and force history equal the
+    single level's, mass within 1e-13."""
     from ilbmrex_amd.native import Batch, Level
+    monkeypatch.setenv("ILBM_BATCH_PHASE", phase)
     st = cf.STORE_F32
     cases = [
         cf.cylinder_2d(nx=1024, ny=1024, D=24.0, xc=200.0, yc=512.0, U=0.05, Re=100.0, storage=st),
