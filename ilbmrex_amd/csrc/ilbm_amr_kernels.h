@@ -46,34 +46,51 @@ struct Window {
   double a, b;
 };
 
+// MODE is W.mode, fixed at compile time so that no branch separates the
+// loads of one interpolation
+template <int MODE>
 __device__ inline double window_val(const Window& W, int i, int j, int k, int comp) {
   const int64_t w = ((int64_t)(k - W.lo[2]) * W.n[1] + (j - W.lo[1])) * W.n[0] + (i - W.lo[0]);
   const double* o = (comp < 4) ? W.Mo + comp * W.cells : W.Do + (comp - 4) * W.cells;
   const double* nw = (comp < 4) ? W.Mn + comp * W.cells : W.Dn + (comp - 4) * W.cells;
-  if (W.mode == 0) return o[w];
-  if (W.mode == 1) return nw[w];
-  return W.a * o[w] + W.b * nw[w];
+  if constexpr (MODE == 0) return o[w];
+  else if constexpr (MODE == 1) return nw[w];
+  else return W.a * o[w] + W.b * nw[w];
 }
 
 __device__ inline int floordiv2(int i) { return (i >= 0) ? i / 2 : -((1 - i) / 2); }
 
 // cell_cons_interp at fine global cell (I, J, K): slopes limited per
-// direction with one factor shared by the ncomp components; all ncomp
-// components are written to out.
-__device__ inline void cons_interp(const Window& W, int dim, int ncomp, int I, int J, int K, double* out) {
-  const int ic = floordiv2(I), jc = floordiv2(J), kc = (dim == 3) ? floordiv2(K) : 0;
-  double u0[6], s[3][6];
-  double sf[3] = {1.0, 1.0, 1.0};
-  for (int n = 0; n < ncomp; ++n) u0[n] = window_val(W, ic, jc, kc, n);
-  for (int d = 0; d < dim; ++d) {
+// direction with one factor shared by the NC components; all NC components
+// are written to out.  Every window value is loaded before the slopes are
+// formed, so the loads of all components and directions are in flight
+// together.
+template <int DIM, int NC, int MODE>
+__device__ inline void cons_interp_m(const Window& W, int I, int J, int K, double* out) {
+  const int ic = floordiv2(I), jc = floordiv2(J), kc = (DIM == 3) ? floordiv2(K) : 0;
+  double u0[NC], up[DIM][NC], um[DIM][NC];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) u0[n] = window_val<MODE>(W, ic, jc, kc, n);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
     const int pi = ic + (d == 0), pj = jc + (d == 1), pk = kc + (d == 2);
     const int mi = ic - (d == 0), mj = jc - (d == 1), mk = kc - (d == 2);
-    for (int n = 0; n < ncomp; ++n) {
-      const double up = window_val(W, pi, pj, pk, n);
-      const double um = window_val(W, mi, mj, mk, n);
-      const double dc = 0.5 * (up - um);
-      const double df = 2.0 * (up - u0[n]);
-      const double db = 2.0 * (u0[n] - um);
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      up[d][n] = window_val<MODE>(W, pi, pj, pk, n);
+      um[d][n] = window_val<MODE>(W, mi, mj, mk, n);
+    }
+  }
+  double s[DIM][NC];
+  double sf[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    sf[d] = 1.0;
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
+      const double dc = 0.5 * (up[d][n] - um[d][n]);
+      const double df = 2.0 * (up[d][n] - u0[n]);
+      const double db = 2.0 * (u0[n] - um[d][n]);
       double sx = (df * db >= 0.0) ? fmin(fabs(df), fabs(db)) : 0.0;
       sx = copysign(1.0, dc) * fmin(sx, fabs(dc));
       if (dc != 0.0) sf[d] = fmin(sf[d], sx / dc);
@@ -82,12 +99,31 @@ __device__ inline void cons_interp(const Window& W, int dim, int ncomp, int I, i
   }
   const double ox = ((double)(I - ic * 2) + 0.5) / 2.0 - 0.5;
   const double oy = ((double)(J - jc * 2) + 0.5) / 2.0 - 0.5;
-  const double oz = (dim == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
-  for (int n = 0; n < ncomp; ++n) {
+  const double oz = (DIM == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
     double v = u0[n] + ox * (s[0][n] * sf[0]);
     v = v + oy * (s[1][n] * sf[1]);
-    if (dim == 3) v = v + oz * (s[2][n] * sf[2]);
+    if constexpr (DIM == 3) v = v + oz * (s[2][n] * sf[2]);
     out[n] = v;
+  }
+}
+
+template <int DIM, int NC>
+__device__ inline void cons_interp(const Window& W, int I, int J, int K, double* out) {
+  if (W.mode == 0) cons_interp_m<DIM, NC, 0>(W, I, J, K, out);
+  else if (W.mode == 1) cons_interp_m<DIM, NC, 1>(W, I, J, K, out);
+  else cons_interp_m<DIM, NC, 2>(W, I, J, K, out);
+}
+
+// the same with the dimension and component count (4 or 6) known at run time
+__device__ inline void cons_interp_rt(const Window& W, int dim, int ncomp, int I, int J, int K, double* out) {
+  if (dim == 3) {
+    if (ncomp == 6) cons_interp<3, 6>(W, I, J, K, out);
+    else cons_interp<3, 4>(W, I, J, K, out);
+  } else {
+    if (ncomp == 6) cons_interp<2, 6>(W, I, J, K, out);
+    else cons_interp<2, 4>(W, I, J, K, out);
   }
 }
 
@@ -96,7 +132,7 @@ struct CfArgs {
   Slabs slabs;
   void* gbuf;          // fine populations; the ghost slabs follow the Q planes
   int ioff[3];         // global index of the fine level's local cell 0
-  int ncomp;           // macro components interpolated together (nmac)
+  int ncomp;           // macro components interpolated together (nmac: 4 or 6)
   Window W;
 };
 
@@ -108,20 +144,10 @@ __global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npad) return;
   int iv[3];
-  {
-    const Slabs& S = a.slabs;
-    int f = 0;
-    while (f + 1 < S.nface && t >= S.face_off[f + 1]) ++f;
-    int64_t r = t - S.face_off[f];
-    for (int e = 0; e < 3; ++e) {
-      if (e == S.face_d[f]) { iv[e] = S.face_side[f] ? a.g.n[e] : -1; continue; }
-      const int n = S.face_n[f][e];
-      iv[e] = S.face_lo[f][e] + (int)(r % n);
-      r /= n;
-    }
-  }
+  slab_cell_of(a.slabs, a.g.n, t, iv);
   double m[6];
-  cons_interp(a.W, LAT::D, a.ncomp, iv[0] + a.ioff[0], iv[1] + a.ioff[1], iv[2] + a.ioff[2], m);
+  if (a.ncomp == 6) cons_interp<LAT::D, 6>(a.W, iv[0] + a.ioff[0], iv[1] + a.ioff[1], iv[2] + a.ioff[2], m);
+  else cons_interp<LAT::D, 4>(a.W, iv[0] + a.ioff[0], iv[1] + a.ioff[1], iv[2] + a.ioff[2], m);
   // FillPatchMesoscopic's equilibrium (AmrCoreLBM.cpp:1605-1620)
   const double r = fmax(m[0], 1.0e-12);
   const double u0 = m[1], v0 = m[2], w0 = m[3];
@@ -283,7 +309,8 @@ __global__ void k_patch_cf(PatchCfArgs a) {
   const int k = a.P.lo[2] + (int)(t / a.P.ks);
   if (i >= 0 && i < a.n[0] && j >= 0 && j < a.n[1] && k >= 0 && k < a.n[2]) return;
   double m[4];
-  cons_interp(a.W, a.dim, 4, i + a.ioff[0], j + a.ioff[1], k + a.ioff[2], m);
+  if (a.dim == 3) cons_interp<3, 4>(a.W, i + a.ioff[0], j + a.ioff[1], k + a.ioff[2], m);
+  else cons_interp<2, 4>(a.W, i + a.ioff[0], j + a.ioff[1], k + a.ioff[2], m);
   for (int c = 0; c < 4; ++c) a.P.p[c * a.P.ns + t] = m[c];
 }
 
@@ -420,7 +447,7 @@ __global__ void k_remake_macro(RemakeArgs a) {
     return;
   }
   double m[6];
-  cons_interp(a.W, a.dim, a.nmac, I, J, K, m);
+  cons_interp_rt(a.W, a.dim, a.nmac, I, J, K, m);
   for (int c = 0; c < a.nmac; ++c) a.M[c * cells + t] = m[c];
 }
 

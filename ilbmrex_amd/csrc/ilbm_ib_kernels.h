@@ -782,6 +782,7 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const doub
 // with k_ib_spread_all's arithmetic.
 constexpr int kGatherTile = 16;
 constexpr int kGatherStage = 256 / kMarkerChunk;   // chunks staged per pass
+static_assert(kGatherTile * kGatherTile == 256, "one 256-thread block per tile, 16 x 4 cells per wavefront");
 
 struct GatherTile {
   int bi, bj, bk;     // band-local cell of this thread
@@ -808,45 +809,67 @@ __device__ inline GatherTile gather_tile_of(const IbArgs& a, int blk) {
   return t;
 }
 
+template <bool D3>
 __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, double tot[3]) {
+  constexpr int kRows = D3 ? 15 : 10;
+  constexpr int kBatch = 8;       // list entries per round of the cell sum
   __shared__ int s_list[kGatherStage];
   __shared__ int s_n, s_next;
   __shared__ int s_base[3][256];
-  __shared__ double s_w[15][256];
-  __shared__ double s_f[4][256];   // F * dv (3 comps), scale
+  __shared__ double s_w[15][256 + 2];   // rows 2 doubles off the bank period
+  __shared__ double s_f[4][256];        // F * dv (3 comps), scale
+  __shared__ int4 s_wl[4][256];         // per wavefront: {entry, base x, y, z}
   tot[0] = tot[1] = tot[2] = 0.0;
   const int nm = a.nm;
   const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
-  const bool d3 = (a.dim == 3);
   const int i = a.band_lo[0] + t.bi, j = a.band_lo[1] + t.bj;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
   if (threadIdx.x == 0) s_next = 0;
   __syncthreads();
   while (true) {
-    if (threadIdx.x == 0) {
-      int n = 0, ch = s_next;
-      for (; ch < nch && n < kGatherStage; ++ch) {
-        if (t.i1 < a.chunk_box[ch] - 2 || t.i0 > a.chunk_box[nch + ch] + 2) continue;
-        if (t.j1 < a.chunk_box[2 * nch + ch] - 2 || t.j0 > a.chunk_box[3 * nch + ch] + 2) continue;
-        if (d3 && (t.k < a.chunk_box[4 * nch + ch] - 2 || t.k > a.chunk_box[5 * nch + ch] + 2)) continue;
-        s_list[n++] = ch;
+    const int next = s_next;
+    __syncthreads();   // every thread has read s_next before the scan moves it
+    if (next >= nch) break;
+    // the first wavefront tests 64 chunks at once; the first kGatherStage
+    // hits, in chunk (= marker) order, are staged this pass
+    if (wv == 0) {
+      const int ch = next + lane;
+      bool hit = ch < nch;
+      if (hit) {
+        hit = !(t.i1 < a.chunk_box[ch] - 2 || t.i0 > a.chunk_box[nch + ch] + 2) &&
+              !(t.j1 < a.chunk_box[2 * nch + ch] - 2 || t.j0 > a.chunk_box[3 * nch + ch] + 2) &&
+              !(D3 && (t.k < a.chunk_box[4 * nch + ch] - 2 || t.k > a.chunk_box[5 * nch + ch] + 2));
       }
-      s_n = n;
-      s_next = ch;
+      const unsigned long long mask = __ballot(hit);
+      const int total = __popcll(mask);
+      const int pos = __popcll(mask & ((1ull << lane) - 1ull));
+      if (hit && pos < kGatherStage) s_list[pos] = ch;
+      if (lane == 0) {
+        s_n = total < kGatherStage ? total : kGatherStage;
+        if (total <= kGatherStage) {
+          s_next = next + 64;
+        } else {
+          unsigned long long rest = mask;
+          for (int q = 0; q < kGatherStage; ++q) rest &= rest - 1ull;   // drop the staged hits
+          s_next = next + __ffsll((long long)rest) - 1;
+        }
+      }
     }
     __syncthreads();
     const int n = s_n;
-    if (n == 0) break;
+    if (n == 0) continue;
     const int slot = threadIdx.x;
     if (slot < n * kMarkerChunk) {
       const int m = s_list[slot / kMarkerChunk] * kMarkerChunk + slot % kMarkerChunk;
       if (m < nm) {
         s_base[0][slot] = a.base[m];
         s_base[1][slot] = a.base[nm + m];
-        s_base[2][slot] = d3 ? a.base[2 * nm + m] : 0;
-        for (int r = 0; r < (d3 ? 15 : 10); ++r) s_w[r][slot] = a.wts[r * nm + m];
+        s_base[2][slot] = D3 ? a.base[2 * nm + m] : 0;
+#pragma unroll
+        for (int r = 0; r < kRows; ++r) s_w[r][slot] = a.wts[r * nm + m];
         s_f[0][slot] = a.F[m] * a.dv;
         s_f[1][slot] = a.F[nm + m] * a.dv;
-        s_f[2][slot] = d3 ? a.F[2 * nm + m] * a.dv : 0.0;
+        s_f[2][slot] = D3 ? a.F[2 * nm + m] * a.dv : 0.0;
         s_f[3][slot] = a.scale[nm + m];
       } else {
         s_base[0][slot] = -(1 << 29);   // no cell is covered
@@ -855,18 +878,59 @@ __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, dou
       }
     }
     __syncthreads();
-    if (t.active) {
-      for (int e = 0; e < n * kMarkerChunk; ++e) {
-        const int ii = i - s_base[0][e] + 2, jj = j - s_base[1][e] + 2;
-        const int kk = d3 ? t.k - s_base[2][e] + 2 : 0;
-        if ((unsigned)ii > 4u || (unsigned)jj > 4u || (unsigned)kk > 4u) continue;
+    // each wavefront lists, in staged (= marker) order, the entries whose
+    // stencil reaches its 16 x 4 cells; every entry that covers one of its
+    // cells is among them, so each cell adds its markers in the same order
+    const int wj0 = t.j0 + wv * (64 / kGatherTile), wj1 = wj0 + 64 / kGatherTile - 1;
+    int cnt = 0;   // wave-uniform
+    for (int e0 = 0; e0 < n * kMarkerChunk; e0 += 64) {
+      const int e = e0 + lane;
+      int bx = 0, by = 0, bz = 0;
+      bool rel = false;
+      if (e < n * kMarkerChunk) {
+        bx = s_base[0][e];
+        by = s_base[1][e];
+        bz = s_base[2][e];
+        rel = bx + 2 >= t.i0 && bx - 2 <= t.i1 && by + 2 >= wj0 && by - 2 <= wj1 &&
+              (!D3 || (bz + 2 >= t.k && bz - 2 <= t.k));
+      }
+      const unsigned long long mask = __ballot(rel);
+      if (rel) s_wl[wv][cnt + __popcll(mask & ((1ull << lane) - 1ull))] = make_int4(e, bx, by, bz);
+      cnt += __popcll(mask);
+    }
+    __syncthreads();
+    // kBatch entries per round: their LDS reads are in flight together, and
+    // the additions stay in entry order (a skipped entry keeps the sum)
+    for (int x = 0; x < cnt; x += kBatch) {
+      double dv[kBatch], f0[kBatch], f1[kBatch], f2[kBatch];
+      bool h[kBatch];
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const int4 en = s_wl[wv][x + u];   // x + u < 256: x < cnt <= 256, x a multiple of kBatch
+        const bool in = x + u < cnt;
+        const int e = in ? en.x : 0;
+        int ii = i - en.y + 2, jj = j - en.z + 2, kk = D3 ? t.k - en.w + 2 : 0;
+        h[u] = in && (unsigned)ii <= 4u && (unsigned)jj <= 4u && (unsigned)kk <= 4u;
+        if (!h[u]) ii = jj = kk = 0;
         const double tU = s_w[ii][e];
         const double tV = s_w[5 + jj][e];
         const double sc = s_f[3][e];
-        const double dval = d3 ? (tU * tV * s_w[10 + kk][e]) * sc : (tU * tV) * sc;
-        tot[0] += dval * s_f[0][e];
-        tot[1] += dval * s_f[1][e];
-        if (d3) tot[2] += dval * s_f[2][e];
+        if constexpr (D3) dv[u] = (tU * tV * s_w[10 + kk][e]) * sc;
+        else dv[u] = (tU * tV) * sc;
+        f0[u] = s_f[0][e];
+        f1[u] = s_f[1][e];
+        f2[u] = D3 ? s_f[2][e] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < kBatch; ++u) {
+        const double n0 = tot[0] + dv[u] * f0[u];
+        const double n1 = tot[1] + dv[u] * f1[u];
+        tot[0] = h[u] ? n0 : tot[0];
+        tot[1] = h[u] ? n1 : tot[1];
+        if constexpr (D3) {
+          const double n2 = tot[2] + dv[u] * f2[u];
+          tot[2] = h[u] ? n2 : tot[2];
+        }
       }
     }
     __syncthreads();
@@ -887,14 +951,19 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   bool active = b < a.band_cells;
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double cell_tot[3] = {0.0, 0.0, 0.0};
+  double mac[4] = {0.0, 0.0, 0.0, 0.0};   // the cell's rho, u, v, w
   if constexpr (MODE == 2) {
     const GatherTile t = gather_tile_of(a, blk);
     b = t.b;
     active = t.active;
-    gather_tile_sum(a, t, cell_tot);
+    // loaded before the gather, so they arrive while it runs
+    if (active)
+      for (int c = 0; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
+    gather_tile_sum<LAT::D == 3>(a, t, cell_tot);
   }
   if (active) {
-    const double rho = a.Mband[b];
+    if constexpr (MODE != 2) mac[0] = a.Mband[b];
+    const double rho = mac[0];
     if constexpr (MODE == 0) {
       const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
       cell_tot[0] = tot[b];
@@ -914,8 +983,9 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
       const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
       const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
       const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
-      const T ux = T(a.Mband[a.band_cells + b]), uy = T(a.Mband[2 * a.band_cells + b]);
-      const T uz = T(a.Mband[3 * a.band_cells + b]);
+      if constexpr (MODE != 2)
+        for (int c = 1; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
+      const T ux = T(mac[1]), uy = T(mac[2]), uz = T(mac[3]);
       const T fac = T(1.0) - T(0.5) / T(tau);
       R* gb = static_cast<R*>(gbuf);
       const int64_t self = g.idx(i, j, k);

@@ -997,20 +997,40 @@ struct GhostArgs {
   Slabs slabs;
 };
 
-// a slab holds under 2^31 cells (checked at set-up): 32-bit divisions
-__device__ inline void slab_cell(const GhostArgs& a, int64_t t, int c[3]) {
-  const Slabs& S = a.slabs;
-  int f = 0;
-  while (f + 1 < S.nface && t >= S.face_off[f + 1]) ++f;
-  uint32_t r = (uint32_t)(t - S.face_off[f]);
+// the ghost cell of slab entry t (a slab holds under 2^31 cells, checked at
+// set-up: 32-bit divisions).  The face is the last one whose offset is at
+// most t; its fields are picked with constant indices, so they stay scalar
+// kernel-argument loads and selects instead of loads indexed per lane.
+__device__ inline void slab_cell_of(const Slabs& S, const int n[3], int64_t t, int c[3]) {
+  int fd = S.face_d[0], fs = S.face_side[0];
+  int flo[3] = {S.face_lo[0][0], S.face_lo[0][1], S.face_lo[0][2]};
+  int fn[3] = {S.face_n[0][0], S.face_n[0][1], S.face_n[0][2]};
+  int64_t foff = S.face_off[0];
+#pragma unroll
+  for (int q = 1; q < 6; ++q) {
+    if (q < S.nface && t >= S.face_off[q]) {
+      fd = S.face_d[q];
+      fs = S.face_side[q];
+      foff = S.face_off[q];
+#pragma unroll
+      for (int e = 0; e < 3; ++e) {
+        flo[e] = S.face_lo[q][e];
+        fn[e] = S.face_n[q][e];
+      }
+    }
+  }
+  uint32_t r = (uint32_t)(t - foff);
+#pragma unroll
   for (int e = 0; e < 3; ++e) {
-    if (e == S.face_d[f]) { c[e] = S.face_side[f] ? a.g.n[e] : -1; continue; }
-    const uint32_t n = (uint32_t)S.face_n[f][e];
-    const uint32_t qn = r / n;
-    c[e] = S.face_lo[f][e] + (int)(r - qn * n);
+    if (e == fd) { c[e] = fs ? n[e] : -1; continue; }
+    const uint32_t nn = (uint32_t)fn[e];
+    const uint32_t qn = r / nn;
+    c[e] = flo[e] + (int)(r - qn * nn);
     r = qn;
   }
 }
+
+__device__ inline void slab_cell(const GhostArgs& a, int64_t t, int c[3]) { slab_cell_of(a.slabs, a.g.n, t, c); }
 
 template <class LAT, class ST>
 __device__ inline void load_face(const GhostArgs& a, const int c[3], typename ST::compute_t* out) {
