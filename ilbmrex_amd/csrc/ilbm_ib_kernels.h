@@ -189,6 +189,29 @@ __device__ inline double ev_value(const IbArgs& a, const int p[3], int comp) {
   return a.EV[comp * a.band_cells + b];
 }
 
+// The three EulerVel components at stencil point p: read from the band
+// directly when p lies inside the domain and the band (the common case, all
+// loads issued together), else through ev_value.  Same values either way.
+__device__ inline void ev_values3(const IbArgs& a, const int p[3], double v[3]) {
+  bool inside = true;
+  for (int d = 0; d < a.dim; ++d)
+    if (p[d] < 0 || p[d] > a.n[d] - 1) inside = false;
+  int64_t b;
+  if (inside && band_index(a, p, &b)) {
+    if (a.ev_from_macro) {
+      v[0] = a.Mband[a.band_cells + b];
+      v[1] = a.Mband[2 * a.band_cells + b];
+      v[2] = (a.dim == 3) ? a.Mband[3 * a.band_cells + b] : 0.0;
+    } else {
+      v[0] = a.EV[b];
+      v[1] = a.EV[a.band_cells + b];
+      v[2] = a.EV[2 * a.band_cells + b];
+    }
+    return;
+  }
+  for (int c = 0; c < 3; ++c) v[c] = ev_value(a, p, c);
+}
+
 __device__ __forceinline__ void ib_begin_body(const IbArgs& a, int64_t b) {
   if (b >= a.band_cells) return;
   a.EV[b] = a.Mband[a.band_cells + b];
@@ -351,10 +374,12 @@ __device__ __forceinline__ void interp_tree_body(const IbArgs& a, int blk) {
     bool ok = !(p[0] < -2 || p[0] > a.n[0] + 1 || p[1] < -2 || p[1] > a.n[1] + 1);
     if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
     if (!ok) continue;
+    double ev[3];
+    ev_values3(a, p, ev);
     s0 += dval * d;
-    s1 += dval * ev_value(a, p, 0) * d;
-    s2 += dval * ev_value(a, p, 1) * d;
-    if (a.dim == 3) s3 += dval * ev_value(a, p, 2) * d;
+    s1 += dval * ev[0] * d;
+    s2 += dval * ev[1] * d;
+    if (a.dim == 3) s3 += dval * ev[2] * d;
   }
   s0 = wave_sum(s0);
   s1 = wave_sum(s1);
@@ -409,10 +434,12 @@ __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
     if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
     valid[wv][t] = ok ? 1 : 0;
     if (!ok) continue;
+    double ev[3];
+    ev_values3(a, p, ev);
     prod[wv][0][t] = dval[r] * d;
-    prod[wv][1][t] = dval[r] * ev_value(a, p, 0) * d;
-    prod[wv][2][t] = dval[r] * ev_value(a, p, 1) * d;
-    prod[wv][3][t] = (a.dim == 3) ? dval[r] * ev_value(a, p, 2) * d : 0.0;
+    prod[wv][1][t] = dval[r] * ev[0] * d;
+    prod[wv][2][t] = dval[r] * ev[1] * d;
+    prod[wv][3][t] = (a.dim == 3) ? dval[r] * ev[2] * d : 0.0;
   }
   __syncthreads();
   if (active && lane < 4) {
@@ -511,30 +538,51 @@ __global__ void k_ib_entry_weights(IbArgs a, int64_t nent) {
 // marker order, added as a whole as SumBoundary adds ghost sums) -- the
 // serial scatter's order.
 __device__ inline void spread_cell_sum(const IbArgs& a, int64_t b, double tot[3]) {
+  // kB entries per round: their list and marker-force loads are in flight
+  // together; the products are added in list order
+  constexpr int kB = 8;
   const int nm = a.nm;
   const int e0 = a.rowptr[b], e1 = a.rowptr[b + 1];
   double gs[3] = {0.0, 0.0, 0.0};
   tot[0] = tot[1] = tot[2] = 0.0;
   int cur = 0;
-  for (int e = e0; e < e1; ++e) {
-    const int m = a.ent_m[e];
-    const double dval = a.ent_w[e];
-    const int grp = a.ent_code[e] >> 9;
-    const double p0 = dval * (a.F[m] * a.dv);
-    const double p1 = dval * (a.F[nm + m] * a.dv);
-    const double p2 = dval * (a.F[2 * nm + m] * a.dv);
-    if (grp != cur) {
-      if (cur != 0)
-        for (int c = 0; c < 3; ++c) tot[c] += gs[c];
-      gs[0] = gs[1] = gs[2] = 0.0;
-      cur = grp;
+  for (int e = e0; e < e1; e += kB) {
+    int mm[kB], gg[kB];
+    double dd[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int ee = (e + u < e1) ? e + u : e0;
+      mm[u] = a.ent_m[ee];
+      dd[u] = a.ent_w[ee];
+      gg[u] = a.ent_code[ee] >> 9;
     }
-    if (grp == 0) {
-      tot[0] += p0; tot[1] += p1;
-      if (a.dim == 3) tot[2] += p2;
-    } else {
-      gs[0] += p0; gs[1] += p1;
-      if (a.dim == 3) gs[2] += p2;
+    double f0[kB], f1[kB], f2[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      f0[u] = a.F[mm[u]];
+      f1[u] = a.F[nm + mm[u]];
+      f2[u] = a.F[2 * nm + mm[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      if (e + u >= e1) break;
+      const int grp = gg[u];
+      const double p0 = dd[u] * (f0[u] * a.dv);
+      const double p1 = dd[u] * (f1[u] * a.dv);
+      const double p2 = dd[u] * (f2[u] * a.dv);
+      if (grp != cur) {
+        if (cur != 0)
+          for (int c = 0; c < 3; ++c) tot[c] += gs[c];
+        gs[0] = gs[1] = gs[2] = 0.0;
+        cur = grp;
+      }
+      if (grp == 0) {
+        tot[0] += p0; tot[1] += p1;
+        if (a.dim == 3) tot[2] += p2;
+      } else {
+        gs[0] += p0; gs[1] += p1;
+        if (a.dim == 3) gs[2] += p2;
+      }
     }
   }
   if (cur != 0)
@@ -952,17 +1000,35 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   double fx = 0.0, fy = 0.0, fz = 0.0;
   double cell_tot[3] = {0.0, 0.0, 0.0};
   double mac[4] = {0.0, 0.0, 0.0, 0.0};   // the cell's rho, u, v, w
+  // Gathered or CSR-summed forces on a 2D level: the cell's macros and
+  // populations are loaded before the sum (they do not depend on it), so
+  // they arrive while it runs.  Elsewhere they are loaded after it, and the
+  // populations only where the force is not zero.
+  constexpr bool kPre = (MODE != 0) && (LAT::D == 2);
+  T gpre[kPre ? LAT::Q : 1];
+  int64_t self = 0;
+  R* gb = static_cast<R*>(gbuf);
   if constexpr (MODE == 2) {
     const GatherTile t = gather_tile_of(a, blk);
     b = t.b;
     active = t.active;
-    // loaded before the gather, so they arrive while it runs
-    if (active)
-      for (int c = 0; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
+  }
+  if (active && (kPre || MODE == 2)) {
+    for (int c = 0; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
+    if constexpr (kPre) {
+      const int bi = (int)(b % a.band_n[0]);
+      const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+      self = g.idx(a.band_lo[0] + bi, a.band_lo[1] + bj, a.band_lo[2]);
+#pragma unroll
+      for (int q = 0; q < LAT::Q; ++q) gpre[q] = ST::load(gb, (int64_t)q * g.qs + self, T(LAT::w[q]));
+    }
+  }
+  if constexpr (MODE == 2) {
+    const GatherTile t = gather_tile_of(a, blk);
     gather_tile_sum<LAT::D == 3>(a, t, cell_tot);
   }
   if (active) {
-    if constexpr (MODE != 2) mac[0] = a.Mband[b];
+    if constexpr (!(kPre || MODE == 2)) mac[0] = a.Mband[b];
     const double rho = mac[0];
     if constexpr (MODE == 0) {
       const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
@@ -979,20 +1045,22 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
     a.Fband[a.band_cells + b] = fy;
     a.Fband[2 * a.band_cells + b] = fz;
     if (!(fx == 0.0 && fy == 0.0 && fz == 0.0)) {
-      const int bi = (int)(b % a.band_n[0]);
-      const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
-      const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
-      const int i = a.band_lo[0] + bi, j = a.band_lo[1] + bj, k = a.band_lo[2] + bk;
-      if constexpr (MODE != 2)
+      if constexpr (!kPre) {
+        const int bi = (int)(b % a.band_n[0]);
+        const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+        const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+        self = g.idx(a.band_lo[0] + bi, a.band_lo[1] + bj, a.band_lo[2] + bk);
+      }
+      if constexpr (!(kPre || MODE == 2))
         for (int c = 1; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
       const T ux = T(mac[1]), uy = T(mac[2]), uz = T(mac[3]);
       const T fac = T(1.0) - T(0.5) / T(tau);
-      R* gb = static_cast<R*>(gbuf);
-      const int64_t self = g.idx(i, j, k);
 #pragma unroll
       for (int q = 0; q < LAT::Q; ++q) {
         const int64_t o = (int64_t)q * g.qs + self;
-        T gq = ST::load(gb, o, T(LAT::w[q]));
+        T gq;
+        if constexpr (kPre) gq = gpre[q];
+        else gq = ST::load(gb, o, T(LAT::w[q]));
         gq += guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
         ST::store(gb, o, gq, T(LAT::w[q]));
       }
@@ -1210,6 +1278,29 @@ __global__ void __launch_bounds__(kSumsBlock) k_ghosts_record(GhostArgs g, unsig
     return;
   }
   sums_record_body(a, rec, nblk);
+}
+
+// A step's stream-collide and the force record of the forcing before it in
+// one launch (levels that step on one stream and need no separate ghost
+// fill): blocks [0, ntiles) are k_step's blocks in linear order (tile t
+// keeps t as its mass partial), block ntiles writes the record.  The record
+// reads the marker forces and the export partials, which the stream-collide
+// does not touch; the next forcing, which rewrites them, follows the launch.
+// The step's blocks are kSumsBlock x 1 threads, so the record's reduction
+// tree is k_sums_record's.
+template <class LAT, class ST, int AM>
+__global__ void __launch_bounds__(kBlock) k_step_record(StepArgs a, uint32_t tx, uint32_t ty, uint32_t ntiles,
+                                                        IbArgs ia, RecArgs rec, int nblk) {
+  const uint32_t t = blockIdx.x;
+  if (t == ntiles) {
+    sums_record_body(ia, rec, nblk);
+    return;
+  }
+  __shared__ double lds[kBlock / 64];
+  const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
+  const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
+  const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
+  step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x) + (int)threadIdx.x, o1 + (int)by, o2 + (int)bz, t, lds);
 }
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of

@@ -288,6 +288,7 @@ using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
 using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 using GhostRecFn = void (*)(unsigned, hipStream_t, const GhostArgs&, unsigned, const IbArgs&, const RecArgs&, int);
+using StepRecFn = void (*)(dim3, hipStream_t, const StepArgs&, const IbArgs&, const RecArgs&, int);
 
 template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
@@ -327,11 +328,21 @@ static void launch_ghost_rec(unsigned grid, hipStream_t s, const GhostArgs& g, u
   hipLaunchKernelGGL((k_ghosts_record<LAT, ST>), dim3(grid), dim3(kSumsBlock), 0, s, g, nghost, a, r, nblk);
 }
 
+// g: k_step's grid (blocks of kSumsBlock x 1 threads); one block more
+template <class LAT, class ST, int AM>
+static void launch_step_rec(dim3 g, hipStream_t s, const StepArgs& a, const IbArgs& ia, const RecArgs& r, int nblk) {
+  const uint32_t nt = g.x * g.y * g.z;
+  hipLaunchKernelGGL((k_step_record<LAT, ST, AM>), dim3(nt + 1), dim3(kSumsBlock), 0, s, a, g.x, g.y, nt, ia, r, nblk);
+}
+
 template <class LAT, class ST>
-static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf, GhostRecFn& grf) {
+static void pick(int am, StepFn& sf, InitFn& inf, FixFn& ff, GhostFn& gf, GhostRecFn& grf, StepRecFn& srf) {
   sf = (am == kAddrWrap) ? launch_step<LAT, ST, kAddrWrap>
      : (am == kAddrWallX32) ? launch_step<LAT, ST, kAddrWallX32>
      : (am == kAddrWall32) ? launch_step<LAT, ST, kAddrWall32> : launch_step<LAT, ST, kAddrWall64>;
+  srf = (am == kAddrWrap) ? launch_step_rec<LAT, ST, kAddrWrap>
+      : (am == kAddrWallX32) ? launch_step_rec<LAT, ST, kAddrWallX32>
+      : (am == kAddrWall32) ? launch_step_rec<LAT, ST, kAddrWall32> : launch_step_rec<LAT, ST, kAddrWall64>;
   inf = launch_init<LAT, ST>;
   ff = launch_fix<LAT, ST>;
   gf = launch_ghost<LAT, ST>;
@@ -429,8 +440,10 @@ struct Level {
   FixFn fixfn = nullptr;
   GhostFn ghostfn = nullptr;
   GhostRecFn ghostrecfn = nullptr;
+  StepRecFn steprecfn = nullptr;
   // one-stream steps: the force record of the forcing is held back
-  // (rec_to_ghost) and launched with the ghost slabs of the same step
+  // (rec_to_ghost) and launched with the ghost slabs of the same step, or,
+  // when the stream-collide kernel writes those slabs, with its launch
   bool rec_to_ghost = false, rec_held = false;
   // k_step writes the x-face ghost slabs of the next step (ghost_write);
   // slabs_fresh: the current buffer's slabs were written by the last step
@@ -732,9 +745,9 @@ struct Level {
 
     auto pk = [&](auto latt) {
       using L = decltype(latt);
-      if (st == 0) pick<L, StF64>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
-      else if (st == 1) pick<L, StF32>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
-      else pick<L, StF16S>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn);
+      if (st == 0) pick<L, StF64>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn, steprecfn);
+      else if (st == 1) pick<L, StF32>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn, steprecfn);
+      else pick<L, StF16S>(am, stepfn, initfn, fixfn, ghostfn, ghostrecfn, steprecfn);
     };
     if (lat == 0) pk(D2Q9{});
     else if (lat == 1) pk(D3Q19{});
@@ -1799,15 +1812,25 @@ struct Level {
         } else {
           ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, ga);
         }
-      } else if (rec_held) {
+      }
+      // a record still held goes with the stream-collide launch as its last
+      // block, unless the launch is timed or its blocks are not one row of
+      // kSumsBlock threads (then on its own, before it)
+      const bool rec_with_step = rec_held && !timing && block.x == (unsigned)kSumsBlock && block.y == 1;
+      if (rec_held && !rec_with_step) {
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream, held_a, held_r, held_nblk);
         rec_held = false;
       }
       const bool last = (s == nsteps - 1);
       StepArgs sa = step_args(want_state && last);
       sa.partial = mring + (int64_t)mring_used * nblocks;
-      // timing: the dispatch records the kernel's own begin / end
-      stepfn(grid, block, stream, sa, timing ? ev_start[s] : nullptr, timing ? ev_stop[s] : nullptr);
+      if (rec_with_step) {
+        steprecfn(grid, stream, sa, held_a, held_r, held_nblk);
+        rec_held = false;
+      } else {
+        // timing: the dispatch records the kernel's own begin / end
+        stepfn(grid, block, stream, sa, timing ? ev_start[s] : nullptr, timing ? ev_stop[s] : nullptr);
+      }
       ring_step(stream, s, !last, false);
       slabs_fresh = gfuse;
       cur ^= 1;
