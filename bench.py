@@ -21,8 +21,9 @@ Also reported:
   against the FP64 run;
 - the periodic D3Q19 256^3 roofline case (F = 0 and F = 1e-6; FP32, FP64,
   FP16-shifted; 200 warm-up and 1000 timed steps);
-- the 512x256 D2Q9 cylinder (FP64, explicit and IVC, and as a one-case
-  graph-replayed batch);
+- the 512x256 D2Q9 cylinder (FP64 explicit and IVC, FP32 explicit, and FP64
+  as a one-case graph-replayed batch; ``--workload cylinder`` runs only the
+  first three);
 - the three-level pitching NACA 0012 hierarchy (FP32, Re = 100, regrid every
   10 coarse steps, 10,000 coarse steps) with force statistics;
 - a plain copy kernel's bandwidth;
@@ -102,6 +103,29 @@ def run_level(case, init, warmup, steps, want_forces=False):
     return {"mlups": mlups, "ms_per_step": 1e3 * wall / steps, "kernel_ms": kms,
             "kernel_launches": nl, "bytes_per_lup": bpl, "kernel_gbs": kgbs,
             "kernel_frac": kgbs / HBM_PEAK_GBS, "bytes_per_launch": cells * bpl}, fh
+
+
+CYLINDER_CASES = (("cylinder512_f64_explicit", cf.COUPLING_EXPLICIT, cf.STORE_F64),
+                  ("cylinder512_f64_ivc", cf.COUPLING_IVC, cf.STORE_F64),
+                  ("cylinder512_f32_explicit", cf.COUPLING_EXPLICIT, cf.STORE_F32))
+
+
+def cylinder_variants(steps, only=None):
+    """The reference's CPU-runnable case, D2Q9 cylinder 512x256, D = 32 at
+    (128, 128), 101 markers, Re = 100: explicit and IVC forcing in FP64 and
+    explicit forcing in FP32, each on one stream (below 2^20 cells)."""
+    out = {}
+    for name, coup, st in CYLINDER_CASES:
+        if only and name not in only:
+            continue
+        case = cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0,
+                              coupling=coup, storage=st)
+        log(f"[bench] {name}: {steps} steps")
+        r, fh = run_level(case, cf.uniform_init(case, u=(0.05, 0.0, 0.0)), 100, steps, want_forces=True)
+        r["us_per_step"] = round(1e3 * r["ms_per_step"], 2)
+        r["cd_last"] = float(fh[-1, 4])
+        out[name] = r
+    return out
 
 
 def accuracy(nsteps):
@@ -305,8 +329,12 @@ def main():
                     help="arrival rate of the config-5 stream served by both the batched and the plain FP64 path")
     ap.add_argument("--no-overlap", action="store_true",
                     help="IB levels step on one stream (profiler runs of the whole-level kernel)")
-    ap.add_argument("--workload", choices=["sphere", "ensemble", "amr"], default="sphere",
-                    help="ensemble / amr: time only that workload (profiler runs) and print its summary")
+    ap.add_argument("--cylinder-steps", type=int, default=2000,
+                    help="timed steps of each 512x256 cylinder case")
+    ap.add_argument("--cylinder-cases", nargs="+", choices=[n for n, _, _ in CYLINDER_CASES], default=None,
+                    help="--workload cylinder: run only these cases")
+    ap.add_argument("--workload", choices=["sphere", "ensemble", "amr", "cylinder"], default="sphere",
+                    help="ensemble / amr / cylinder: time only that workload (profiler runs) and print its summary")
     ap.add_argument("--probe", action="store_true",
                     help="also run the 1 GiB copy kernel (profiler counter calibration)")
     args = ap.parse_args()
@@ -330,6 +358,9 @@ def main():
         return
     if args.workload == "amr":
         print(json.dumps(amr_variant(args.amr_steps)), flush=True)
+        return
+    if args.workload == "cylinder":
+        print(json.dumps(cylinder_variants(args.cylinder_steps, args.cylinder_cases)), flush=True)
         return
 
     # headline: sphere, FP32 storage
@@ -359,15 +390,8 @@ def main():
             r, _ = run_level(case, init, args.warmup, args.steps)
             variants[name] = r
         variants["accuracy"] = accuracy(args.accuracy_steps)
-        # the reference's CPU-runnable case: D2Q9 cylinder 512x256, FP64
-        for name, coup in (("cylinder512_f64_explicit", cf.COUPLING_EXPLICIT),
-                           ("cylinder512_f64_ivc", cf.COUPLING_IVC)):
-            case = cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0,
-                                  coupling=coup, storage=cf.STORE_F64)
-            log(f"[bench] {name}: 2000 steps")
-            r, fh = run_level(case, cf.uniform_init(case, u=(0.05, 0.0, 0.0)), 100, 2000, want_forces=True)
-            r["cd_last"] = float(fh[-1, 4])
-            variants[name] = r
+        # the reference's CPU-runnable case: D2Q9 cylinder 512x256
+        variants.update(cylinder_variants(args.cylinder_steps))
         # the same case as a one-case batch: hipGraph replay of the step sequence
         case = cf.cylinder_2d(nx=512, ny=256, D=32.0, xc=128.0, yc=128.0, U=0.05, Re=100.0,
                               storage=cf.STORE_F64)

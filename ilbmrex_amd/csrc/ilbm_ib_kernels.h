@@ -823,41 +823,51 @@ __device__ void write_force_record(const IbArgs& a, const RecArgs& r, const doub
 // term of the coming collision on the same cell (collide_forced.H:22-50),
 // which the stream-collide kernel left out because F was not yet known.
 // Each block also sums its exported force for the step's force record.
-// Moving body: the band is covered by 16 x 16 tiles of (i, j) cells per k
-// plane, one block each.  The block stages the marker chunks whose stencil
-// bounds reach its tile into LDS (chunk order = marker order) and every
-// cell sums the markers whose 5-point stencil covers it, in marker order,
-// with k_ib_spread_all's arithmetic.
+// Moving body: the band is covered by tiles of 16 x 16 (i, j) cells per k
+// plane, one block each, or of 16 x 4 cells when the level's IB sums may
+// use a tree order (reduced storage).  The block stages the marker chunks
+// whose stencil bounds reach its tile into LDS (chunk order = marker order);
+// every cell sums the markers whose 5-point stencil covers it with
+// k_ib_spread_all's arithmetic -- in marker order on a 16 x 16 tile (a
+// wavefront per 16 x 4 cells), or on a 16 x 4 tile with the four wavefronts
+// taking every fourth batch of the tile's marker list and their partial
+// sums added in wavefront order.
 constexpr int kGatherTile = 16;
 constexpr int kGatherStage = 256 / kMarkerChunk;   // chunks staged per pass
 static_assert(kGatherTile * kGatherTile == 256, "one 256-thread block per tile, 16 x 4 cells per wavefront");
 
+// tile height in cells: 16 (serial order) or 4 (tree order)
+__host__ __device__ inline int gather_tile_h(bool tree) { return tree ? 64 / kGatherTile : kGatherTile; }
+
 struct GatherTile {
   int bi, bj, bk;     // band-local cell of this thread
-  bool active;        // inside the band
+  bool active;        // inside the band (tree order: wavefront 0 only)
   int64_t b;          // band cell index (valid when active)
   int i0, i1, j0, j1, k;   // the tile's cells in level-local indices (inclusive)
 };
 
+template <bool TREE>
 __device__ inline GatherTile gather_tile_of(const IbArgs& a, int blk) {
+  constexpr int th = TREE ? 64 / kGatherTile : kGatherTile;
   GatherTile t;
   const int tx = (a.band_n[0] + kGatherTile - 1) / kGatherTile;
-  const int ty = (a.band_n[1] + kGatherTile - 1) / kGatherTile;
+  const int ty = (a.band_n[1] + th - 1) / th;
   const int cx = blk % tx, cy = (blk / tx) % ty;
+  const int cell = TREE ? (int)(threadIdx.x & 63) : (int)threadIdx.x;
   t.bk = blk / (tx * ty);
-  t.bi = cx * kGatherTile + (int)(threadIdx.x % kGatherTile);
-  t.bj = cy * kGatherTile + (int)(threadIdx.x / kGatherTile);
-  t.active = t.bi < a.band_n[0] && t.bj < a.band_n[1];
+  t.bi = cx * kGatherTile + cell % kGatherTile;
+  t.bj = cy * th + cell / kGatherTile;
+  t.active = t.bi < a.band_n[0] && t.bj < a.band_n[1] && (!TREE || threadIdx.x < 64);
   t.b = ((int64_t)t.bk * a.band_n[1] + t.bj) * a.band_n[0] + t.bi;
   t.i0 = a.band_lo[0] + cx * kGatherTile;
   t.i1 = t.i0 + kGatherTile - 1;
-  t.j0 = a.band_lo[1] + cy * kGatherTile;
-  t.j1 = t.j0 + kGatherTile - 1;
+  t.j0 = a.band_lo[1] + cy * th;
+  t.j1 = t.j0 + th - 1;
   t.k = a.band_lo[2] + t.bk;
   return t;
 }
 
-template <bool D3>
+template <bool D3, bool TREE>
 __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, double tot[3]) {
   constexpr int kRows = D3 ? 15 : 10;
   constexpr int kBatch = 8;       // list entries per round of the cell sum
@@ -867,6 +877,7 @@ __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, dou
   __shared__ double s_w[15][256 + 2];   // rows 2 doubles off the bank period
   __shared__ double s_f[4][256];        // F * dv (3 comps), scale
   __shared__ int4 s_wl[4][256];         // per wavefront: {entry, base x, y, z}
+  __shared__ double s_part[TREE ? 3 : 1][TREE ? 3 * 64 : 1];   // tree order: wavefronts 1-3's partial sums
   tot[0] = tot[1] = tot[2] = 0.0;
   const int nm = a.nm;
   const int nch = (nm + kMarkerChunk - 1) / kMarkerChunk;
@@ -928,8 +939,10 @@ __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, dou
     __syncthreads();
     // each wavefront lists, in staged (= marker) order, the entries whose
     // stencil reaches its 16 x 4 cells; every entry that covers one of its
-    // cells is among them, so each cell adds its markers in the same order
-    const int wj0 = t.j0 + wv * (64 / kGatherTile), wj1 = wj0 + 64 / kGatherTile - 1;
+    // cells is among them, so each cell adds its markers in the same order.
+    // Tree order: all four wavefronts hold the tile's 16 x 4 cells and build
+    // the same list.
+    const int wj0 = TREE ? t.j0 : t.j0 + wv * (64 / kGatherTile), wj1 = wj0 + 64 / kGatherTile - 1;
     int cnt = 0;   // wave-uniform
     for (int e0 = 0; e0 < n * kMarkerChunk; e0 += 64) {
       const int e = e0 + lane;
@@ -949,7 +962,9 @@ __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, dou
     __syncthreads();
     // kBatch entries per round: their LDS reads are in flight together, and
     // the additions stay in entry order (a skipped entry keeps the sum)
-    for (int x = 0; x < cnt; x += kBatch) {
+    // tree order: wavefront w takes batches w, w + 4, ...
+    const int x0 = TREE ? wv * kBatch : 0, xs = TREE ? 4 * kBatch : kBatch;
+    for (int x = x0; x < cnt; x += xs) {
       double dv[kBatch], f0[kBatch], f1[kBatch], f2[kBatch];
       bool h[kBatch];
 #pragma unroll
@@ -983,12 +998,22 @@ __device__ inline void gather_tile_sum(const IbArgs& a, const GatherTile& t, dou
     }
     __syncthreads();
   }
+  if constexpr (TREE) {
+    // partial sums of wavefronts 1-3 to wavefront 0, added in wavefront order
+    const int lane = threadIdx.x & 63;
+    if (wv > 0)
+      for (int c = 0; c < 3; ++c) s_part[wv - 1][c * 64 + lane] = tot[c];
+    __syncthreads();
+    if (wv == 0)
+      for (int w = 0; w < 3; ++w)
+        for (int c = 0; c < 3; ++c) tot[c] += s_part[w][c * 64 + lane];
+  }
 }
 
 // Export of the force of band cell b: MODE 0 reads EulerForceIter /
 // EulerForceTotal, MODE 1 sums the cell's CSR spreading list first, MODE 2
-// gathers a moving body's markers (single-iteration forcing, where the
-// iteration's force is the total).
+// gathers a moving body's markers in marker order and MODE 4 in tree order
+// (single-iteration forcing, where the iteration's force is the total).
 template <class LAT, class ST, int MODE>
 __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
                                                   const RecArgs& rec, int blk, int nblk) {
@@ -1004,16 +1029,17 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
   // populations are loaded before the sum (they do not depend on it), so
   // they arrive while it runs.  Elsewhere they are loaded after it, and the
   // populations only where the force is not zero.
+  constexpr bool kGather = (MODE == 2 || MODE == 4);
   constexpr bool kPre = (MODE != 0) && (LAT::D == 2);
   T gpre[kPre ? LAT::Q : 1];
   int64_t self = 0;
   R* gb = static_cast<R*>(gbuf);
-  if constexpr (MODE == 2) {
-    const GatherTile t = gather_tile_of(a, blk);
+  if constexpr (kGather) {
+    const GatherTile t = gather_tile_of<MODE == 4>(a, blk);
     b = t.b;
     active = t.active;
   }
-  if (active && (kPre || MODE == 2)) {
+  if (active && (kPre || kGather)) {
     for (int c = 0; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
     if constexpr (kPre) {
       const int bi = (int)(b % a.band_n[0]);
@@ -1023,12 +1049,9 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
       for (int q = 0; q < LAT::Q; ++q) gpre[q] = ST::load(gb, (int64_t)q * g.qs + self, T(LAT::w[q]));
     }
   }
-  if constexpr (MODE == 2) {
-    const GatherTile t = gather_tile_of(a, blk);
-    gather_tile_sum<LAT::D == 3>(a, t, cell_tot);
-  }
+  if constexpr (kGather) gather_tile_sum<LAT::D == 3, MODE == 4>(a, gather_tile_of<MODE == 4>(a, blk), cell_tot);
   if (active) {
-    if constexpr (!(kPre || MODE == 2)) mac[0] = a.Mband[b];
+    if constexpr (!(kPre || kGather)) mac[0] = a.Mband[b];
     const double rho = mac[0];
     if constexpr (MODE == 0) {
       const double* tot = a.ev_from_macro ? a.EFI : a.EFT;
@@ -1051,7 +1074,7 @@ __device__ __forceinline__ void export_fixup_body(const IbArgs& a, const GridGeo
         const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
         self = g.idx(a.band_lo[0] + bi, a.band_lo[1] + bj, a.band_lo[2] + bk);
       }
-      if constexpr (!(kPre || MODE == 2))
+      if constexpr (!(kPre || kGather))
         for (int c = 1; c < 4; ++c) mac[c] = a.Mband[c * a.band_cells + b];
       const T ux = T(mac[1]), uy = T(mac[2]), uz = T(mac[3]);
       const T fac = T(1.0) - T(0.5) / T(tau);

@@ -307,14 +307,16 @@ static void launch_init(dim3 g, dim3 b, hipStream_t s, const InitArgs& a) {
   hipLaunchKernelGGL((k_init<LAT, ST>), g, b, 0, s, a);
 }
 // mode 0: export of EulerForceIter / Total; 1: fused CSR spreading (thread
-// per band cell); 2: fused gather spreading of a moving body; 3: fused CSR
-// spreading, one wavefront per non-empty band cell
+// per band cell); 2: fused gather spreading of a moving body (marker order);
+// 3: fused CSR spreading, one wavefront per non-empty band cell; 4: as 2 in
+// tree order (16 x 4 tiles, four wavefronts per tile)
 template <class LAT, class ST>
 static void launch_fix(int mode, dim3 g, dim3 b, hipStream_t s, const IbArgs& a, const GridGeom& gg, void* buf,
                        double tau, const RecArgs& r) {
   if (mode == 3) hipLaunchKernelGGL((k_ib_spread_export_w<LAT, ST>), g, b, 0, s, a, gg, buf, tau, r);
   else if (mode == 1) hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 1>), g, b, 0, s, a, gg, buf, tau, r);
   else if (mode == 2) hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 2>), g, b, 0, s, a, gg, buf, tau, r);
+  else if (mode == 4) hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 4>), g, b, 0, s, a, gg, buf, tau, r);
   else hipLaunchKernelGGL((k_ib_export_fixup<LAT, ST, 0>), g, b, 0, s, a, gg, buf, tau, r);
 }
 template <class LAT, class ST>
@@ -441,6 +443,9 @@ struct Level {
   GhostFn ghostfn = nullptr;
   GhostRecFn ghostrecfn = nullptr;
   StepRecFn steprecfn = nullptr;
+  // 3D single-iteration export: one wavefront per non-empty band cell (true)
+  // or one thread per band cell
+  bool spread_wave = true;
   // one-stream steps: the force record of the forcing is held back
   // (rec_to_ghost) and launched with the ghost slabs of the same step, or,
   // when the stream-collide kernel writes those slabs, with its launch
@@ -721,6 +726,8 @@ struct Level {
     gfuse = am == kAddrWallX32 && !cf && fusable(c.bc_lo[0]) && fusable(c.bc_hi[0]) && !(gsep && gsep[0] == '1');
     const char* ibx = std::getenv("ILBM_IB_EXACT");
     ib_tree = st != 0 && !(ibx && ibx[0] == '1');
+    const char* swv = std::getenv("ILBM_SPREAD_WAVE");
+    spread_wave = !(swv && swv[0] == '0');
     for (int b = 0; b < 2; ++b) {
       // Q population planes followed by the Q ghost slabs
       const size_t bytes = popbytes * ((size_t)g.pop_n + (size_t)npad * Q);
@@ -858,7 +865,7 @@ struct Level {
     HIPCHK(hipMemsetAsync(Mm_d, 0, sizeof(double) * 3 * nm, stream));
     rowptr_d = dev_alloc<int>(band_cells + 1);
     const int64_t tiles = (int64_t)((band_n[0] + kGatherTile - 1) / kGatherTile) *
-                          ((band_n[1] + kGatherTile - 1) / kGatherTile) * band_n[2];
+                          ((band_n[1] + gather_tile_h(true) - 1) / gather_tile_h(true)) * band_n[2];
     fpartial_n = 3 * ((band_cells + 255) / 256 + band_cells / kSpreadWaves + tiles + 2);
     fpartial = dev_alloc<double>(fpartial_n);
     upload_markers();
@@ -1050,14 +1057,15 @@ struct Level {
   int fix_mode() const {
     const int nloop = (c.coupling_method == 1) ? 1 : std::max(c.loop_ns, 1);
     if (nloop > 1) return 0;
-    if (dev_kin) return 2;
-    return (c.dim == 3) ? 3 : 1;
+    if (dev_kin) return ib_tree ? 4 : 2;
+    return (c.dim == 3 && spread_wave) ? 3 : 1;
   }
   unsigned fix_blocks() const {
     if (fix_mode() == 3) return (unsigned)((n_ne + kSpreadWaves - 1) / kSpreadWaves);
-    if (fix_mode() == 2)
-      return (unsigned)(((band_n[0] + kGatherTile - 1) / kGatherTile) * ((band_n[1] + kGatherTile - 1) / kGatherTile) *
-                        band_n[2]);
+    if (fix_mode() == 2 || fix_mode() == 4) {
+      const int th = gather_tile_h(fix_mode() == 4);
+      return (unsigned)(((band_n[0] + kGatherTile - 1) / kGatherTile) * ((band_n[1] + th - 1) / th) * band_n[2]);
+    }
     return (unsigned)((band_cells + 255) / 256);
   }
 

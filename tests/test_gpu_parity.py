@@ -312,18 +312,25 @@ def test_fused_ghost_slabs_bitexact(lat, bx, monkeypatch):
         assert_bitexact(out[0], out[1], f"storage {st} fused vs separate")
 
 
-@pytest.mark.parametrize("shape", ["cylinder", "sphere"])
+@pytest.mark.parametrize("shape", ["cylinder", "sphere", "pitching"])
 def test_reduced_storage_ib_tree_sums_match_serial_order(shape, monkeypatch):
     """Levels with reduced storage sum the interpolation and spreading
-    products in a fixed wave tree; ILBM_IB_EXACT=1 keeps the reference's
-    serial order.  Both are FP64 arithmetic, so after 40 FP32-storage steps
-    the marker velocities and forces agree to 1e-12 relative, the force
-    record to 1e-10 and the FP32 populations to 1e-5 (a rounding flip of
-    the exported force can move an FP32 population by one ulp)."""
+    products in a fixed wave tree (a moving body: 16 x 4 gather tiles whose
+    four wavefronts split the marker list); ILBM_IB_EXACT=1 keeps the
+    reference's serial order.  Both are FP64 arithmetic, so after 40
+    FP32-storage steps the marker velocities and forces agree to 1e-12
+    relative, the force record to 1e-10 and the FP32 populations to 1e-5 (a
+    rounding flip of the exported force can move an FP32 population by one
+    ulp)."""
     from ilbmrex_amd.native import Level
     if shape == "cylinder":
         c = cf.cylinder_2d(nx=192, ny=96, D=24.0, xc=48.0, yc=47.7, U=0.05, Re=100.0, storage=cf.STORE_F32)
         u0 = (0.05, 0.0, 0.0)
+    elif shape == "pitching":
+        c = cf.cylinder_2d(nx=192, ny=128, D=40.0, xc=60.0, yc=64.3, U=0.03, Re=200.0, storage=cf.STORE_F32).replace(
+            geometry_type=cf.GEOM_USER, airfoil_chord=40.0, pitch_frequency=0.01, pivot_x=10.0,
+            force_eval_method=cf.FORCE_EVAL_MARKER, name="pitching")
+        u0 = (0.03, 0.0, 0.0)
     else:
         c = cf.sphere_3d(nx=64, ny=48, nz=48, D=12.0, center=(20.0, 24.2, 23.9), U=0.04, Re=50.0,
                          storage=cf.STORE_F32)
