@@ -1116,10 +1116,124 @@ __global__ void __launch_bounds__(256) k_ib_export_fixup(IbArgs a, GridGeom g, v
 // entries keep F = 0.  Block partials of F in wave order for the record.
 // The cell's own inputs -- its entry range, macros and populations -- do not
 // depend on the spreading sum and are loaded first, so the dependent chain
-// is entries -> marker forces -> sum -> store.
+// is entries -> marker forces -> sum -> store.  Levels whose IB sums may use
+// a tree order (reduced storage) take spread_export_group_body instead.
+//
+// Tree order: kGroupLanes lanes per non-empty band cell, kGroupCells cells
+// per block.  Lane s of a cell takes the entries e0 + s, e0 + s + 8, ...
+// (four per round, their loads in flight together), the cell's lanes add
+// their sums in a fixed butterfly (periodic images in the same sum), and
+// each lane adds the Guo term to populations s, s + 8, s + 16.  Block
+// partials: the cells' forces in the wave tree, then the waves in order.
+constexpr int kGroupLanes = 8;
+constexpr int kGroupCells = 256 / kGroupLanes;
+template <class LAT, class ST>
+__device__ __forceinline__ void spread_export_group_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
+                                                         const RecArgs& rec, int blk, int nblk) {
+  using T = typename ST::compute_t;
+  using R = typename ST::raw;
+  constexpr int kQP = (LAT::Q + kGroupLanes - 1) / kGroupLanes;
+  constexpr int kE = 4;   // entries per lane per round
+  __shared__ double wsum[4][3];
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, sub = lane & (kGroupLanes - 1);
+  const int ci = blk * kGroupCells + (int)threadIdx.x / kGroupLanes;
+  const bool active = ci < a.n_ne;
+  const int nm = a.nm;
+  const int b = active ? a.ne_cells[ci] : 0;
+  const int e0 = active ? a.ne_e[ci] : 0, e1 = active ? a.ne_e[a.n_ne + ci] : 0;
+  R* gb = static_cast<R*>(gbuf);
+  int64_t self = 0;
+  T gq[kQP];
+  T ux = T(0), uy = T(0), uz = T(0);
+  double rho = 0.0;
+  if (active) {
+    const int bi = (int)(b % a.band_n[0]);
+    const int bj = (int)((b / a.band_n[0]) % a.band_n[1]);
+    const int bk = (int)(b / ((int64_t)a.band_n[0] * a.band_n[1]));
+    self = g.idx(a.band_lo[0] + bi, a.band_lo[1] + bj, a.band_lo[2] + bk);
+#pragma unroll
+    for (int p = 0; p < kQP; ++p) {
+      const int q = sub + p * kGroupLanes;
+      gq[p] = (q < LAT::Q) ? ST::load(gb, (int64_t)q * g.qs + self, T(LAT::w[q])) : T(0);
+    }
+    rho = a.Mband[b];
+    ux = T(a.Mband[a.band_cells + b]);
+    uy = T(a.Mband[2 * a.band_cells + b]);
+    uz = T(a.Mband[3 * a.band_cells + b]);
+  }
+  double tot[3] = {0.0, 0.0, 0.0};
+  for (int e = e0 + sub; e < e1; e += kE * kGroupLanes) {
+    int mm[kE];
+    double dd[kE];
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      const int ee = (e + u * kGroupLanes < e1) ? e + u * kGroupLanes : e;
+      mm[u] = a.ent_m[ee];
+      dd[u] = a.ent_w[ee];
+    }
+    double f0[kE], f1[kE], f2[kE];
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      f0[u] = a.F[mm[u]];
+      f1[u] = a.F[nm + mm[u]];
+      f2[u] = a.F[2 * nm + mm[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kE; ++u) {
+      if (e + u * kGroupLanes >= e1) break;
+      tot[0] += dd[u] * (f0[u] * a.dv);
+      tot[1] += dd[u] * (f1[u] * a.dv);
+      if (LAT::D == 3) tot[2] += dd[u] * (f2[u] * a.dv);
+    }
+  }
+  // a + b == b + a: every lane of the cell ends with the same sums
+#pragma unroll
+  for (int off = kGroupLanes / 2; off > 0; off >>= 1)
+    for (int c = 0; c < 3; ++c) tot[c] += __shfl_xor(tot[c], off, kGroupLanes);
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  if (active) {
+    fx = a.dt * rho * tot[0];
+    fy = a.dt * rho * tot[1];
+    fz = a.dt * rho * tot[2];
+    if (sub == 0) {
+      a.Fband[b] = fx;
+      a.Fband[a.band_cells + b] = fy;
+      a.Fband[2 * a.band_cells + b] = fz;
+    }
+    if (!(fx == 0.0 && fy == 0.0 && fz == 0.0)) {
+      const T fac = T(1.0) - T(0.5) / T(tau);
+#pragma unroll
+      for (int p = 0; p < kQP; ++p) {
+        const int q = sub + p * kGroupLanes;
+        if (q < LAT::Q) {
+          const T v = gq[p] + guo_q<LAT, T>(q, fac, ux, uy, uz, T(fx), T(fy), T(fz));
+          ST::store(gb, (int64_t)q * g.qs + self, v, T(LAT::w[q]));
+        }
+      }
+    }
+  }
+  const double v[3] = {fx, fy, fz};
+  for (int c = 0; c < 3; ++c) {
+    const double r = wave_sum(sub == 0 ? v[c] : 0.0);
+    if (lane == 0) wsum[wv][c] = r;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int c = 0; c < 3; ++c) {
+      double sum = 0.0;
+      for (int w2 = 0; w2 < 4; ++w2) sum += wsum[w2][c];
+      rec.partial[(int64_t)c * nblk + blk] = sum;
+    }
+  }
+}
+
 template <class LAT, class ST>
 __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const GridGeom& g, void* gbuf, double tau,
                                                         const RecArgs& rec, int blk, int nblk) {
+  if (a.tree) {
+    spread_export_group_body<LAT, ST>(a, g, gbuf, tau, rec, blk, nblk);
+    return;
+  }
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   __shared__ double pr[kSpreadWaves][3][kSpreadChunk];
@@ -1153,21 +1267,7 @@ __device__ __forceinline__ void spread_export_wave_body(const IbArgs& a, const G
   }
   double tot[3] = {0.0, 0.0, 0.0}, gs[3] = {0.0, 0.0, 0.0};
   int cur = 0;
-  if (a.tree) {
-    // the cell's contributions over the lanes, then a fixed tree (periodic
-    // images included in the same sum)
-    for (int e = e0 + lane; e < e1; e += 64) {
-      const int m = a.ent_m[e];
-      const double dval = a.ent_w[e];
-      tot[0] += dval * (a.F[m] * a.dv);
-      tot[1] += dval * (a.F[nm + m] * a.dv);
-      if (a.dim == 3) tot[2] += dval * (a.F[2 * nm + m] * a.dv);
-    }
-    tot[0] = wave_sum(tot[0]);
-    tot[1] = wave_sum(tot[1]);
-    tot[2] = wave_sum(tot[2]);
-  }
-  if (lane == 0) rounds[wv] = a.tree ? 0 : (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
+  if (lane == 0) rounds[wv] = (e1 - e0 + kSpreadChunk - 1) / kSpreadChunk;
   __syncthreads();
   int nr = 0;
   for (int w2 = 0; w2 < kSpreadWaves; ++w2) nr = rounds[w2] > nr ? rounds[w2] : nr;

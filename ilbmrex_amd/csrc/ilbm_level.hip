@@ -1061,7 +1061,8 @@ struct Level {
     return (c.dim == 3 && spread_wave) ? 3 : 1;
   }
   unsigned fix_blocks() const {
-    if (fix_mode() == 3) return (unsigned)((n_ne + kSpreadWaves - 1) / kSpreadWaves);
+    if (fix_mode() == 3)
+      return (unsigned)(ib_tree ? (n_ne + kGroupCells - 1) / kGroupCells : (n_ne + kSpreadWaves - 1) / kSpreadWaves);
     if (fix_mode() == 2 || fix_mode() == 4) {
       const int th = gather_tile_h(fix_mode() == 4);
       return (unsigned)(((band_n[0] + kGatherTile - 1) / kGatherTile) * ((band_n[1] + th - 1) / th) * band_n[2]);
