@@ -570,7 +570,7 @@ struct Batch {
       if (L.ib) {
         const IbArgs& a = ia[0][ci].ib;
         for (int64_t b = 0; b < (L.band_cells + 255) / 256; ++b) maps[BatchGroup::M_BAND].push_back({c, (int)b});
-        for (int b = 0; b < (L.nm + kInterpWaves - 1) / kInterpWaves; ++b)
+        for (int b = 0; b < interp_blocks(L.nm, a.tree != 0, L.c.dim); ++b)
           maps[BatchGroup::M_INTERP].push_back({c, b});
         if (a.nloop > 1)
           for (int b = 0; b < (L.n_ne + 255) / 256; ++b) maps[BatchGroup::M_SPREAD].push_back({c, b});

@@ -346,47 +346,102 @@ __device__ inline void explicit_force(const IbArgs& a, int m, double Ux_m, doubl
 // before the cell index arrives, so the chain of dependent memory accesses
 // is cell index -> band velocities -> marker U / F.
 constexpr int kInterpWaves = 4;
-// Interpolation of one marker per wavefront with the sums of the stencil
-// products in a fixed tree over the lanes (wave_sum), not in the serial
-// order of the reference's loop: deterministic, and equal to the serial sum
-// to rounding.  No block barrier: a wave past the last marker leaves.
-__device__ __forceinline__ void interp_tree_body(const IbArgs& a, int blk) {
-  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int m = blk * kInterpWaves + wv;
-  if (m >= a.nm) return;
+// Interpolation in a tree order (levels with reduced storage): G lanes per
+// marker, 256 / G markers per block -- a wavefront per marker in 3D (125
+// points, the shortest chain on the step's critical path), 16 lanes in 2D
+// (25 points).  Lane s takes the stencil points s, s + G, ..., adds its
+// products in that order, and the marker's lanes add their sums in a fixed
+// butterfly: deterministic, and equal to the serial sum to rounding.
+__host__ __device__ constexpr int interp_group(int dim) { return dim == 3 ? 64 : 16; }
+// blocks of a k_ib_interp launch over nm markers
+__host__ __device__ inline int interp_blocks(int nm, bool tree, int dim) {
+  const int per = tree ? 64 * kInterpWaves / interp_group(dim) : kInterpWaves;
+  return (nm + per - 1) / per;
+}
+template <int DIM>
+__device__ __forceinline__ void interp_group_body(const IbArgs& a, int blk) {
+  constexpr int kInterpGroup = interp_group(DIM);
+  constexpr int kInterpGroupMarkers = 64 * kInterpWaves / kInterpGroup;
+  constexpr int npts = (DIM == 3) ? 125 : 25;
+  constexpr int kP = (npts + kInterpGroup - 1) / kInterpGroup;   // points per lane
+  const int sub = (int)threadIdx.x & (kInterpGroup - 1);
+  const int m = blk * kInterpGroupMarkers + (int)threadIdx.x / kInterpGroup;
+  const bool act = m < a.nm;
   const int nm = a.nm;
-  const double d = (a.dim == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
-  const int npts = (a.dim == 3) ? 125 : 25;
-  const int b0 = a.base[m], b1 = a.base[nm + m], b2 = (a.dim == 3) ? a.base[2 * nm + m] : 0;
+  const int mc = act ? m : 0;
+  const double d = (DIM == 3) ? a.dx[0] * a.dx[1] * a.dx[2] : a.dx[0] * a.dx[1];
+  const int b0 = a.base[mc], b1 = a.base[nm + mc], b2 = (DIM == 3) ? a.base[2 * nm + mc] : 0;
   double tvx = 0.0, tvy = 0.0, tvz = 0.0;
-  if (lane == 0 && a.fuse_explicit) target_velocity(a, m, &tvx, &tvy, &tvz);
+  if (act && sub == 0 && a.fuse_explicit) target_velocity(a, m, &tvx, &tvy, &tvz);
   double s0 = 0.0, s1 = 0.0, s2 = 0.0, s3 = 0.0;
-  for (int r = 0; r < 2; ++r) {
-    const int t = lane + 64 * r;
-    if (t >= npts) continue;
-    int ii, jj, kk;
-    if (a.dim == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
-    else { ii = t / 5; jj = t % 5; kk = 0; }
-    const double tU = a.wts[(0 * 5 + ii) * nm + m];
-    const double tV = a.wts[(1 * 5 + jj) * nm + m];
-    const double dval = (a.dim == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
-    const int p[3] = {b0 + ii - 2, b1 + jj - 2, (a.dim == 3) ? b2 + kk - 2 : 0};
-    bool ok = !(p[0] < -2 || p[0] > a.n[0] + 1 || p[1] < -2 || p[1] > a.n[1] + 1);
-    if (a.dim == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) ok = false;
-    if (!ok) continue;
-    double ev[3];
-    ev_values3(a, p, ev);
-    s0 += dval * d;
-    s1 += dval * ev[0] * d;
-    s2 += dval * ev[1] * d;
-    if (a.dim == 3) s3 += dval * ev[2] * d;
+  // A marker whose whole stencil lies inside the domain and the band reads
+  // its EulerVel values straight from the band, every load of the lane's
+  // points issued first; any other marker goes point by point through
+  // ev_value.  Same values, same order of the additions.
+  bool inner = act;
+  for (int e = 0; e < DIM; ++e) {
+    const int be = (e == 0) ? b0 : (e == 1) ? b1 : b2;
+    inner = inner && be - 2 >= 0 && be + 2 <= a.n[e] - 1 && be - 2 >= a.band_lo[e] &&
+            be + 2 <= a.band_lo[e] + a.band_n[e] - 1;
   }
-  s0 = wave_sum(s0);
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
-  s3 = wave_sum(s3);
-  if (lane != 0) return;
-  double u = s1, v = s2, w = (a.dim == 3) ? s3 : 0.0;
+  const double* src = a.ev_from_macro ? a.Mband + a.band_cells : a.EV;
+  if (inner) {
+    double dval[kP], ev[kP][3];
+#pragma unroll
+    for (int r = 0; r < kP; ++r) {
+      const int t = sub + kInterpGroup * r;
+      const int tc = t < npts ? t : 0;
+      int ii, jj, kk;
+      if (DIM == 3) { ii = tc / 25; jj = (tc / 5) % 5; kk = tc % 5; }
+      else { ii = tc / 5; jj = tc % 5; kk = 0; }
+      const double tU = a.wts[(0 * 5 + ii) * nm + m];
+      const double tV = a.wts[(1 * 5 + jj) * nm + m];
+      dval[r] = (DIM == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
+      const int64_t bc = ((int64_t)((DIM == 3) ? b2 + kk - 2 - a.band_lo[2] : 0) * a.band_n[1] +
+                          (b1 + jj - 2 - a.band_lo[1])) * a.band_n[0] + (b0 + ii - 2 - a.band_lo[0]);
+      ev[r][0] = src[bc];
+      ev[r][1] = src[a.band_cells + bc];
+      ev[r][2] = (DIM == 3) ? src[2 * a.band_cells + bc] : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < kP; ++r) {
+      if (sub + kInterpGroup * r >= npts) continue;
+      s0 += dval[r] * d;
+      s1 += dval[r] * ev[r][0] * d;
+      s2 += dval[r] * ev[r][1] * d;
+      if (DIM == 3) s3 += dval[r] * ev[r][2] * d;
+    }
+  } else if (act) {
+#pragma unroll 1
+    for (int t = sub; t < npts; t += kInterpGroup) {
+      int ii, jj, kk;
+      if (DIM == 3) { ii = t / 25; jj = (t / 5) % 5; kk = t % 5; }
+      else { ii = t / 5; jj = t % 5; kk = 0; }
+      const int p[3] = {b0 + ii - 2, b1 + jj - 2, (DIM == 3) ? b2 + kk - 2 : 0};
+      bool o = !(p[0] < -2 || p[0] > a.n[0] + 1 || p[1] < -2 || p[1] > a.n[1] + 1);
+      if (DIM == 3 && (p[2] < -2 || p[2] > a.n[2] + 1)) o = false;
+      if (!o) continue;
+      const double tU = a.wts[(0 * 5 + ii) * nm + m];
+      const double tV = a.wts[(1 * 5 + jj) * nm + m];
+      const double dv = (DIM == 3) ? tU * tV * a.wts[(2 * 5 + kk) * nm + m] : tU * tV;
+      double ev[3];
+      ev_values3(a, p, ev);
+      s0 += dv * d;
+      s1 += dv * ev[0] * d;
+      s2 += dv * ev[1] * d;
+      if (DIM == 3) s3 += dv * ev[2] * d;
+    }
+  }
+  // a + b == b + a: every lane of the marker ends with the same sums
+#pragma unroll
+  for (int off = kInterpGroup / 2; off > 0; off >>= 1) {
+    s0 += __shfl_xor(s0, off, kInterpGroup);
+    s1 += __shfl_xor(s1, off, kInterpGroup);
+    s2 += __shfl_xor(s2, off, kInterpGroup);
+    s3 += __shfl_xor(s3, off, kInterpGroup);
+  }
+  if (!act || sub != 0) return;
+  double u = s1, v = s2, w = (DIM == 3) ? s3 : 0.0;
   if (a.renorm_interp && s0 > 0.0) { u /= s0; v /= s0; w /= s0; }
   a.U[m] = u; a.U[nm + m] = v; a.U[2 * nm + m] = w;
   double F0 = 0.0, F1 = 0.0, F2 = 0.0;
@@ -396,7 +451,8 @@ __device__ __forceinline__ void interp_tree_body(const IbArgs& a, int blk) {
 
 __device__ __forceinline__ void interp_body(const IbArgs& a, int blk) {
   if (a.tree) {
-    interp_tree_body(a, blk);
+    if (a.dim == 3) interp_group_body<3>(a, blk);
+    else interp_group_body<2>(a, blk);
     return;
   }
   __shared__ double prod[kInterpWaves][4][128];

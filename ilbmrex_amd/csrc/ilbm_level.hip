@@ -1112,7 +1112,7 @@ struct Level {
     if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
     for (int it = 0; it < nloop; ++it) {
       a.first_iter = (it == 0) ? 1 : 0;
-      hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
+      hipLaunchKernelGGL(k_ib_interp, dim3(interp_blocks(nm, a.tree != 0, c.dim)), dim3(64 * kInterpWaves), 0,
                          stream, a);
       if (ib_mark == 1 && nloop == 1 && !use_ivc) ib_sig = xsignal(stream, X_IB);
       if (use_ivc) {
@@ -1187,7 +1187,7 @@ struct Level {
     if (!a.ev_from_macro) hipLaunchKernelGGL(k_ib_begin, dim3(gb), dim3(tb), 0, stream, a);
     for (int it = 0; it < a.nloop; ++it) {
       a.first_iter = (it == 0) ? 1 : 0;
-      hipLaunchKernelGGL(k_ib_interp, dim3((nm + kInterpWaves - 1) / kInterpWaves), dim3(64 * kInterpWaves), 0,
+      hipLaunchKernelGGL(k_ib_interp, dim3(interp_blocks(nm, a.tree != 0, c.dim)), dim3(64 * kInterpWaves), 0,
                          stream, a);
       if (a.use_ivc) {
         hipLaunchKernelGGL(k_ib_ivc_rhs, dim3((nm + tb - 1) / tb), dim3(tb), 0, stream, a);
