@@ -279,6 +279,25 @@ static void lu_solve(const std::vector<double>& a, const std::vector<int>& piv, 
   }
 }
 
+// Flags of the events that order one step's kernels across the streams of
+// an overlapped level on one device.  Such an event needs no system-scope
+// fence: the kernels' own completion makes their writes visible to kernels
+// that start after them on the device, and nothing on the host or another
+// device reads through these events (the events before host copies keep
+// HIP's default).  Without the fence the recorded signal is cheaper and the
+// waiting stream starts sooner (sphere 256x128x128 FP32 overlapped step:
+// 0.133 -> 0.129 ms; a batch's lane events gained nothing and keep the
+// default).  ILBM_XEV_FENCE=system | device restores the default
+// system-scope fence or takes a device-scope release instead.
+static unsigned step_event_flags() {
+  unsigned f = hipEventDisableTiming | hipEventDisableSystemFence;
+  if (const char* xf = std::getenv("ILBM_XEV_FENCE")) {
+    if (std::string(xf) == "system") f = hipEventDisableTiming;
+    else if (std::string(xf) == "device") f = hipEventDisableTiming | hipEventReleaseToDevice;
+  }
+  return f;
+}
+
 // ---------------------------------------------------------------------------
 // Level
 // ---------------------------------------------------------------------------
@@ -1459,7 +1478,7 @@ struct Level {
         HIPCHK(hipStreamCreateWithFlags(&stream_d, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_d, hipEventDisableTiming));
       }
-      for (hipEvent_t& e : xev) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+      for (hipEvent_t& e : xev) HIPCHK(hipEventCreateWithFlags(&e, step_event_flags()));
       if (async_record()) {
         HIPCHK(hipStreamCreateWithFlags(&stream_c, hipStreamNonBlocking));
         HIPCHK(hipEventCreateWithFlags(&ev_c, hipEventDisableTiming));
