@@ -29,23 +29,40 @@
 namespace ilbm {
 
 using CfFn = void (*)(dim3, hipStream_t, const CfArgs&);
+using CfMoveFn = void (*)(unsigned, unsigned, hipStream_t, const CfArgs&, const MoveArgs&, const IbArgs&);
 using AvgFn = void (*)(dim3, hipStream_t, const AvgArgs&);
 
 template <class LAT, class ST>
 static void launch_cf(dim3 g, hipStream_t s, const CfArgs& a) {
   hipLaunchKernelGGL((k_cf_fill<LAT, ST>), g, dim3(kBlock), 0, s, a);
 }
+// A refined level's coarse-fine ghost fill and the marker kinematics and
+// weights of its moving body in one launch: blocks [0, ncf) fill ghost
+// cells, the rest run k_ib_move_weights's blocks.  Neither reads what the
+// other writes, and the forcing and stream-collide that follow need both.
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_cf_fill_move(CfArgs a, unsigned ncf, MoveArgs mv, IbArgs ia) {
+  static_assert(kBlock == 256, "ghost and marker blocks share the launch's block size");
+  if (blockIdx.x < ncf) cf_fill_body<LAT, ST>(a, (int64_t)blockIdx.x * kBlock + threadIdx.x);
+  else move_weights_body(mv, ia, (int)(blockIdx.x - ncf));
+}
+template <class LAT, class ST>
+static void launch_cf_move(unsigned ncf, unsigned nmv, hipStream_t s, const CfArgs& a, const MoveArgs& mv,
+                           const IbArgs& ia) {
+  hipLaunchKernelGGL((k_cf_fill_move<LAT, ST>), dim3(ncf + nmv), dim3(kBlock), 0, s, a, ncf, mv, ia);
+}
+
 template <class LAT, class ST>
 static void launch_avg(dim3 g, hipStream_t s, const AvgArgs& a) {
   hipLaunchKernelGGL((k_average_down<LAT, ST>), g, dim3(256), 0, s, a);
 }
 
-static void pick_amr(int lat, int st, CfFn& cf, AvgFn& av) {
+static void pick_amr(int lat, int st, CfFn& cf, AvgFn& av, CfMoveFn& cm) {
   auto pk = [&](auto latt) {
     using L = decltype(latt);
-    if (st == 0) { cf = launch_cf<L, StF64>; av = launch_avg<L, StF64>; }
-    else if (st == 1) { cf = launch_cf<L, StF32>; av = launch_avg<L, StF32>; }
-    else { cf = launch_cf<L, StF16S>; av = launch_avg<L, StF16S>; }
+    if (st == 0) { cf = launch_cf<L, StF64>; av = launch_avg<L, StF64>; cm = launch_cf_move<L, StF64>; }
+    else if (st == 1) { cf = launch_cf<L, StF32>; av = launch_avg<L, StF32>; cm = launch_cf_move<L, StF32>; }
+    else { cf = launch_cf<L, StF16S>; av = launch_avg<L, StF16S>; cm = launch_cf_move<L, StF16S>; }
   };
   if (lat == 0) pk(D2Q9{});
   else if (lat == 1) pk(D3Q19{});
@@ -65,6 +82,7 @@ struct Amr {
   int64_t dblocks[kMaxLev] = {0, 0, 0};
   bool exported[kMaxLev] = {false, false, false};
   CfFn cffn = nullptr;
+  CfMoveFn cfmovefn = nullptr;
   AvgFn avgfn = nullptr;
   int istep = 0;
   double step_end = 0.0;
@@ -214,7 +232,7 @@ struct Amr {
     }
     const Level& F = *L[nlev - 1];
     ncol = F.ib ? F.ncol : 0;
-    pick_amr(L[0]->lat, L[0]->st, cffn, avgfn);
+    pick_amr(L[0]->lat, L[0]->st, cffn, avgfn, cfmovefn);
     HIPCHK(hipStreamSynchronize(stream));
   }
 
@@ -271,7 +289,9 @@ struct Amr {
     W.b = (time - C.t_old) / (C.t_new - C.t_old);
   }
 
-  void cf_fill(int l, double time) {
+  // with_move: the level's moving-body kinematics at `time` go in the same
+  // launch (k_cf_fill_move) and its next forcing skips them
+  void cf_fill(int l, double time, bool with_move = false) {
     Level& V = *L[l];
     CfArgs a{};
     a.g = V.g;
@@ -281,7 +301,13 @@ struct Amr {
     a.ncomp = V.c.nmac;
     window_at(l - 1, time, a.W);
     const int64_t npad = V.slabs.face_off[V.slabs.nface];
-    cffn(dim3((unsigned)((npad + kBlock - 1) / kBlock)), stream, a);
+    const unsigned ncf = (unsigned)((npad + kBlock - 1) / kBlock);
+    if (with_move) {
+      cfmovefn(ncf, (unsigned)((V.nm + kBlock - 1) / kBlock), stream, a, V.move_args(time), V.ib_args());
+      V.moved_ext = true;
+      return;
+    }
+    cffn(dim3(ncf), stream, a);
   }
 
   void average_down(int l) {
@@ -636,7 +662,7 @@ struct Amr {
     V.t_old = V.t_new;
     V.t_new += V.dt;
     if (V.window) V.wsel ^= 1;  // swap(macro_new, macro_old)
-    if (l > 0) cf_fill(l, time);
+    if (l > 0) cf_fill(l, time, V.ib && V.dev_kin);
     if (V.ib) {
       V.rec_enable = last_chain;
       V.rec_override = true;

@@ -137,11 +137,10 @@ struct CfArgs {
 };
 
 template <class LAT, class ST>
-__global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
+__device__ __forceinline__ void cf_fill_body(const CfArgs& a, int64_t t) {
   using T = typename ST::compute_t;
   using R = typename ST::raw;
   const int64_t npad = a.slabs.face_off[a.slabs.nface];
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= npad) return;
   int iv[3];
   slab_cell_of(a.slabs, a.g.n, t, iv);
@@ -159,6 +158,11 @@ __global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
     const double fe = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
     ST::store(out, (int64_t)q * npad + t, T(fe), T(LAT::w[q]));
   }
+}
+
+template <class LAT, class ST>
+__global__ void __launch_bounds__(kBlock) k_cf_fill(CfArgs a) {
+  cf_fill_body<LAT, ST>(a, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 struct AvgArgs {

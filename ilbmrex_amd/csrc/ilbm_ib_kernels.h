@@ -770,8 +770,8 @@ struct MoveArgs {
 // (k_ib_move), stencil cells and weights (k_ib_weights), and the cell bounds
 // of each kMarkerChunk-marker chunk (k_ib_chunk_bounds) from a 32-lane
 // min / max; blocks hold whole chunks.
-__global__ void __launch_bounds__(256) k_ib_move_weights(MoveArgs mv, IbArgs a) {
-  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void move_weights_body(const MoveArgs& mv, const IbArgs& a, int blk) {
+  const int m = blk * (int)blockDim.x + (int)threadIdx.x;
   const int nm = a.nm;
   const bool act = m < nm;
   int b[3] = {0, 0, 0};
@@ -803,6 +803,8 @@ __global__ void __launch_bounds__(256) k_ib_move_weights(MoveArgs mv, IbArgs a) 
     }
   }
 }
+
+__global__ void __launch_bounds__(256) k_ib_move_weights(MoveArgs mv, IbArgs a) { move_weights_body(mv, a, blockIdx.x); }
 
 // Block size of the one-block reductions (marker sums, force record): every
 // launch of a level, a batch or a hierarchy uses it, so the trees are the

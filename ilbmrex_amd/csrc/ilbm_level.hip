@@ -462,6 +462,9 @@ struct Level {
   GhostFn ghostfn = nullptr;
   GhostRecFn ghostrecfn = nullptr;
   StepRecFn steprecfn = nullptr;
+  // the step's marker kinematics were launched with another kernel (an AMR
+  // level's coarse-fine ghost fill); the next forcing skips its own launch
+  bool moved_ext = false;
   // 3D single-iteration export: one wavefront per non-empty band cell (true)
   // or one thread per band cell
   bool spread_wave = true;
@@ -1099,6 +1102,21 @@ struct Level {
     return b;
   }
 
+  // Positions and weights of the pitching section at `time` on the device;
+  // the angle terms as marker_positions computes them.
+  MoveArgs move_args(double time) const {
+    const double twopi = 2.0 * kPi;
+    const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
+    const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
+    MoveArgs mv{};
+    mv.xref = xref_d; mv.xm = xm_d; mv.ut = ut_d; mv.nm = nm;
+    mv.omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
+    mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
+    mv.cs = std::cos(angle); mv.sn = std::sin(angle);
+    mv.z = 0.0;
+    return mv;
+  }
+
   // IBMarkerDF::update_forcing (IBMarkerDF.cpp:1522-1917) on the band, ending
   // with the export of F fused with the Guo term of the coming collision and
   // with the ComputeIBForce record of the step (row `row` of the history).
@@ -1106,19 +1124,13 @@ struct Level {
     // the slot's previous record must be done before its inputs are rewritten
     if (ring_slot >= 0 && rec_seq[ring_slot]) xwait(stream, X_REC0 + ring_slot, rec_seq[ring_slot]);
     if (dev_kin) {
-      // the pitching kinematics on the device; the angle terms as
-      // marker_positions computes them
-      const double twopi = 2.0 * kPi;
-      const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
-      const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
-      MoveArgs mv{};
-      mv.xref = xref_d; mv.xm = xm_d; mv.ut = ut_d; mv.nm = nm;
-      mv.omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
-      mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
-      mv.cs = std::cos(angle); mv.sn = std::sin(angle);
-      mv.z = 0.0;
-      const IbArgs aw = ib_args();
-      hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, mv, aw);
+      // the pitching kinematics on the device, unless the caller launched
+      // them with another kernel of the step (moved_ext)
+      if (!moved_ext) {
+        const IbArgs aw = ib_args();
+        hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, move_args(time), aw);
+      }
+      moved_ext = false;
     } else if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
@@ -1186,16 +1198,7 @@ struct Level {
   // (Mband) are the caller's.
   void ib_stages(double time) {
     if (dev_kin) {
-      const double twopi = 2.0 * kPi;
-      const double phase = twopi * c.pitch_frequency * time + c.pitch_phase;
-      const double angle = c.pitch_mean + c.pitch_amplitude * std::sin(phase);
-      MoveArgs mv{};
-      mv.xref = xref_d; mv.xm = xm_d; mv.ut = ut_d; mv.nm = nm;
-      mv.omega = twopi * c.pitch_frequency * c.pitch_amplitude * std::cos(phase);
-      mv.px = c.x0 + c.pivot_x; mv.py = c.y0; mv.pivot_x = c.pivot_x;
-      mv.cs = std::cos(angle); mv.sn = std::sin(angle);
-      mv.z = 0.0;
-      hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, mv, ib_args());
+      hipLaunchKernelGGL(k_ib_move_weights, dim3((nm + 255) / 256), dim3(256), 0, stream, move_args(time), ib_args());
     } else if (moving) {
       marker_positions(c, mg, time, xm_h, ut_h);
       upload_markers();
