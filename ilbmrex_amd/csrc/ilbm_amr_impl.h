@@ -337,6 +337,9 @@ struct Amr {
     for (int d = 0; d < 3; ++d) a.Fcol[d] = sa.Fcol[d];
     a.dpartial = dpartial[l];
     avgfn(dim3((unsigned)dblocks[l]), stream, a);
+    // slab values that the stream-collide kernel wrote from a boundary cell
+    // this average-down has overwritten are stale
+    if (a.clo[0] <= 0 || a.clo[0] + a.cn[0] >= C.c.n[0]) C.slabs_fresh = false;
   }
 
   // ---------------- regrid ----------------
@@ -671,12 +674,16 @@ struct Amr {
       V.ib_update_forcing(time, row);
     }
     if (!V.cf && !V.wrap) {
-      GhostArgs ga = V.ghost_args();
+      // a stream-collide kernel that wrote the slabs of this step (level 0
+      // walled in x only) leaves nothing to fill
       const int64_t npad = V.slabs.face_off[V.slabs.nface];
-      if (npad) V.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), V.stream, ga);
+      if (npad && !V.slabs_fresh)
+        V.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), V.stream,
+                  V.ghost_args());
     }
     StepArgs sa = V.step_args(export_state);
     V.stepfn(V.grid, V.block, stream, sa, nullptr, nullptr);
+    V.slabs_fresh = V.gfuse;
     exported[l] = export_state;
     V.cur ^= 1;
     const int step_lev = V.istep;
