@@ -26,10 +26,17 @@ using GhostBFn = void (*)(dim3, hipStream_t, const GhostArgs*, const BlockRef*);
 using FixBFn = void (*)(dim3, hipStream_t, const IbCase*, const BlockRef*);
 using GhostRecBFn = void (*)(unsigned, hipStream_t, const GhostArgs*, const BlockRef*, unsigned, const IbCase*,
                              const BlockRef*);
+using StepRecBFn = void (*)(unsigned, hipStream_t, const StepArgs*, const BlockRef*, unsigned, const IbCase*,
+                            const BlockRef*);
 
 template <class LAT, class ST, int AM>
 static void launch_step_b(dim3 g, hipStream_t s, const StepArgs* a, const BlockRef* m) {
   hipLaunchKernelGGL((k_step_b<LAT, ST, AM>), g, dim3(kBlock), 0, s, a, m);
+}
+template <class LAT, class ST, int AM>
+static void launch_step_rec_b(unsigned grid, hipStream_t s, const StepArgs* a, const BlockRef* m, unsigned nstep,
+                              const IbCase* ia, const BlockRef* rm) {
+  hipLaunchKernelGGL((k_step_record_b<LAT, ST, AM>), dim3(grid), dim3(kBlock), 0, s, a, m, nstep, ia, rm);
 }
 template <class LAT, class ST>
 static void launch_ghost_b(dim3 g, hipStream_t s, const GhostArgs* a, const BlockRef* m) {
@@ -65,21 +72,26 @@ static GhostRecBFn pick_ghost_rec_b(int lat, int st) {
 }
 
 template <class LAT, class ST>
-static void pick_b(int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
+static void pick_b(int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw, StepRecBFn* sr) {
   sf = (am == kAddrWrap) ? launch_step_b<LAT, ST, kAddrWrap>
      : (am == kAddrWallX32) ? launch_step_b<LAT, ST, kAddrWallX32>
      : (am == kAddrWall32) ? launch_step_b<LAT, ST, kAddrWall32> : launch_step_b<LAT, ST, kAddrWall64>;
+  if (sr)
+    *sr = (am == kAddrWrap) ? launch_step_rec_b<LAT, ST, kAddrWrap>
+        : (am == kAddrWallX32) ? launch_step_rec_b<LAT, ST, kAddrWallX32>
+        : (am == kAddrWall32) ? launch_step_rec_b<LAT, ST, kAddrWall32> : launch_step_rec_b<LAT, ST, kAddrWall64>;
   gf = launch_ghost_b<LAT, ST>;
   ff = launch_fix_b<LAT, ST>;
   fw = launch_fixw_b<LAT, ST>;
 }
 
-static void pick_class(int lat, int st, int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw) {
+static void pick_class(int lat, int st, int am, StepBFn& sf, GhostBFn& gf, FixBFn& ff, FixBFn& fw,
+                       StepRecBFn* sr = nullptr) {
   auto pk = [&](auto latt) {
     using L = decltype(latt);
-    if (st == 0) pick_b<L, StF64>(am, sf, gf, ff, fw);
-    else if (st == 1) pick_b<L, StF32>(am, sf, gf, ff, fw);
-    else pick_b<L, StF16S>(am, sf, gf, ff, fw);
+    if (st == 0) pick_b<L, StF64>(am, sf, gf, ff, fw, sr);
+    else if (st == 1) pick_b<L, StF32>(am, sf, gf, ff, fw, sr);
+    else pick_b<L, StF16S>(am, sf, gf, ff, fw, sr);
   };
   if (lat == 0) pk(D2Q9{});
   else if (lat == 1) pk(D3Q19{});
@@ -111,6 +123,7 @@ struct BatchGroup {
   StepBFn step = nullptr;
   GhostBFn ghost = nullptr;
   GhostRecBFn ghost_rec = nullptr;  // slabs and force records in one launch
+  StepRecBFn step_rec = nullptr;    // stream-collide and force records in one launch (no slab launch)
   FixBFn fix = nullptr;
   FixBFn fixw = nullptr;
   std::vector<int> members;  // indices into Batch::cases
@@ -512,7 +525,7 @@ struct Batch {
       if (gi < 0) {
         BatchGroup g;
         g.lat = L.lat; g.st = L.st; g.am = L.am; g.lane = lane_of[i];
-        pick_class(g.lat, g.st, g.am, g.step, g.ghost, g.fix, g.fixw);
+        pick_class(g.lat, g.st, g.am, g.step, g.ghost, g.fix, g.fixw, &g.step_rec);
         g.ghost_rec = pick_ghost_rec_b(g.lat, g.st);
         groups.push_back(g);
         gi = (int)groups.size() - 1;
@@ -656,10 +669,8 @@ struct Batch {
       if (G.mcnt[BatchGroup::M_FIX] > 0) G.fix(grid(BatchGroup::M_FIX), stream, A, m + G.moff[BatchGroup::M_FIX]);
       if (G.mcnt[BatchGroup::M_FIXW] > 0)
         G.fixw(grid(BatchGroup::M_FIXW), stream, A, m + G.moff[BatchGroup::M_FIXW]);
-      // with slabs to fill, the records go with them (one launch fewer)
-      if (G.mcnt[BatchGroup::M_GHOST] == 0)
-        hipLaunchKernelGGL(k_force_record_b, grid(BatchGroup::M_IBCASE), dim3(kSumsBlock), 0, stream, A,
-                           m + G.moff[BatchGroup::M_IBCASE]);
+      // the records go with the slab launch, or without slabs to fill with
+      // the stream-collide launch (enqueue_group_lbm)
       if (G.lane == 0 && phase_pending && !lane0_ib) {
         HIPCHK(hipEventRecord(ev_phase, stream));
         lane0_ib = true;
@@ -667,7 +678,8 @@ struct Batch {
     }
   }
 
-  // the slabs (with the records when the group has slabs), stream-collide,
+  // the slabs, stream-collide (the force records ride with the slab launch,
+  // or with the stream-collide launch when there are no slabs to fill),
   // Sum(Rho) and the clock
   void enqueue_group_lbm(BatchGroup& G, int p, hipStream_t stream) {
     const BlockRef* m = G.dmap;
@@ -678,7 +690,12 @@ struct Batch {
                   m + G.moff[BatchGroup::M_IBCASE]);
     else if (G.mcnt[BatchGroup::M_GHOST] > 0)
       G.ghost(grid(BatchGroup::M_GHOST), stream, G.dghost[p], m + G.moff[BatchGroup::M_GHOST]);
-    G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
+    if (G.mcnt[BatchGroup::M_GHOST] == 0 && G.mcnt[BatchGroup::M_IBCASE] > 0)
+      G.step_rec((unsigned)(G.mcnt[BatchGroup::M_STEP] + G.mcnt[BatchGroup::M_IBCASE]), stream, G.dstep[p],
+                 m + G.moff[BatchGroup::M_STEP], (unsigned)G.mcnt[BatchGroup::M_STEP], G.dib[p],
+                 m + G.moff[BatchGroup::M_IBCASE]);
+    else
+      G.step(grid(BatchGroup::M_STEP), stream, G.dstep[p], m + G.moff[BatchGroup::M_STEP]);
     hipLaunchKernelGGL(k_case_tail_b, grid(BatchGroup::M_TAIL), dim3(1024), 0, stream, G.dtail);
   }
 

@@ -1624,9 +1624,8 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_export_w_b(cons
   spread_export_wave_body<LAT, ST>(c.ib, c.g, c.gbuf, c.tau, c.rec, r.b, c.nblk_fix);
 }
 
-__global__ void __launch_bounds__(kSumsBlock) k_force_record_b(const IbCase* __restrict__ A,
-                                                         const BlockRef* __restrict__ map) {
-  const IbCase& c = A[map[blockIdx.x].c];
+// one batched case's force record (a block of kSumsBlock threads)
+__device__ __forceinline__ void case_record_body(const IbCase& c) {
   if (c.ib.nloop == 1) sums_record_body(c.ib, c.rec, c.nblk_fix);
   else force_record_body(c.ib, c.rec, c.nblk_fix);
 }
@@ -1646,9 +1645,25 @@ __global__ void __launch_bounds__(kSumsBlock) k_ghosts_record_b(const GhostArgs*
     ghost_body<LAT, ST>(GA[r.c], (int64_t)r.b * kBlock + threadIdx.x);
     return;
   }
-  const IbCase& c = A[rmap[blockIdx.x - nghost].c];
-  if (c.ib.nloop == 1) sums_record_body(c.ib, c.rec, c.nblk_fix);
-  else force_record_body(c.ib, c.rec, c.nblk_fix);
+  case_record_body(A[rmap[blockIdx.x - nghost].c]);
+}
+
+// A batched step's stream-collide and the force records of its IB cases in
+// one launch (classes whose stream-collide kernel writes the slabs, so no
+// slab launch carries the records): blocks [0, nstep) are k_step_b's, the
+// rest write one case's record each.  Neither reads what the other writes,
+// and the tail that advances the cases' clocks follows the launch.
+template <class LAT, class ST, int AM>
+__global__ void __launch_bounds__(kBlock) k_step_record_b(const StepArgs* __restrict__ A,
+                                                          const BlockRef* __restrict__ map, unsigned nstep,
+                                                          const IbCase* __restrict__ IA,
+                                                          const BlockRef* __restrict__ rmap) {
+  if (blockIdx.x < nstep) {
+    __shared__ double lds[kBlock / 64];
+    step_b_body<LAT, ST, AM>(A, map, blockIdx.x, lds);
+    return;
+  }
+  case_record_body(IA[rmap[blockIdx.x - nstep].c]);
 }
 
 }  // namespace ilbm

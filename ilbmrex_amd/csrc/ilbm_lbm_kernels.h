@@ -876,9 +876,9 @@ __device__ inline void batch_tile(const GridGeom& g, int bx, const BlockRef& r, 
 // thread (nine loads per cell keep too few bytes in flight); 3D lattices
 // take one, which keeps their register budget.
 template <class LAT, class ST, int AM>
-__global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map) {
-  __shared__ double lds[kBlock / 64];
-  const BlockRef r = map[blockIdx.x];
+__device__ __forceinline__ void step_b_body(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map,
+                                            unsigned blk, double* lds) {
+  const BlockRef r = map[blk];
   const StepArgs& a = A[r.c];
   if constexpr (LAT::D == 2) {
     if (a.tile_nc == 2) {
@@ -892,6 +892,12 @@ __global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ 
   int i, j, k;
   batch_tile(a.g, a.tile_bx, r, 0, &i, &j, &k);
   step_body<LAT, ST, AM>(a, i, j, k, r.b, lds);
+}
+
+template <class LAT, class ST, int AM>
+__global__ void __launch_bounds__(kBlock) k_step_b(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map) {
+  __shared__ double lds[kBlock / 64];
+  step_b_body<LAT, ST, AM>(A, map, blockIdx.x, lds);
 }
 
 // ---------------------------------------------------------------------------
