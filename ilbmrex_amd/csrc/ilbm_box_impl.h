@@ -694,17 +694,23 @@ constexpr int kMaxInterpComp = 8;
 
 // AMReX cell_cons_interp at fine cell (I, J, K): centred slopes limited per
 // direction (minmod of the one-sided differences), one factor per direction
-// shared by the ncomp components, offsets of +-1/4 coarse cell.
-__device__ inline void cons_interp_fab(const Fab& Co, const Fab& Cn, const TimeMix& tm, int dim, int ncomp, int I,
-                                       int J, int K, double* out) {
-  const int ic = floordiv2(I), jc = floordiv2(J), kc = (dim == 3) ? floordiv2(K) : K;
-  double u0[kMaxInterpComp], s[3][kMaxInterpComp];
-  double sf[3] = {1.0, 1.0, 1.0};
-  for (int n = 0; n < ncomp; ++n) u0[n] = crse_at(Co, Cn, tm, ic, jc, kc, n);
-  for (int d = 0; d < dim; ++d) {
+// shared by the NC components, offsets of +-1/4 coarse cell.  NC is a
+// template parameter so that the per-component arrays stay in registers.
+template <int DIM, int NC>
+__device__ inline void cons_interp_fab(const Fab& Co, const Fab& Cn, const TimeMix& tm, int I, int J, int K,
+                                       double* out) {
+  const int ic = floordiv2(I), jc = floordiv2(J), kc = (DIM == 3) ? floordiv2(K) : K;
+  double u0[NC], s[DIM][NC];
+  double sf[DIM];
+#pragma unroll
+  for (int n = 0; n < NC; ++n) u0[n] = crse_at(Co, Cn, tm, ic, jc, kc, n);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    sf[d] = 1.0;
     const int pi = ic + (d == 0), pj = jc + (d == 1), pk = kc + (d == 2);
     const int mi = ic - (d == 0), mj = jc - (d == 1), mk = kc - (d == 2);
-    for (int n = 0; n < ncomp; ++n) {
+#pragma unroll
+    for (int n = 0; n < NC; ++n) {
       const double up = crse_at(Co, Cn, tm, pi, pj, pk, n);
       const double um = crse_at(Co, Cn, tm, mi, mj, mk, n);
       const double dc = 0.5 * (up - um);
@@ -718,18 +724,40 @@ __device__ inline void cons_interp_fab(const Fab& Co, const Fab& Cn, const TimeM
   }
   const double ox = ((double)(I - ic * 2) + 0.5) / 2.0 - 0.5;
   const double oy = ((double)(J - jc * 2) + 0.5) / 2.0 - 0.5;
-  const double oz = (dim == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
-  for (int n = 0; n < ncomp; ++n) {
+  const double oz = (DIM == 3) ? ((double)(K - kc * 2) + 0.5) / 2.0 - 0.5 : 0.0;
+#pragma unroll
+  for (int n = 0; n < NC; ++n) {
     double v = u0[n] + ox * (s[0][n] * sf[0]);
     v = v + oy * (s[1][n] * sf[1]);
-    if (dim == 3) v = v + oz * (s[2][n] * sf[2]);
+    if constexpr (DIM == 3) v = v + oz * (s[2][n] * sf[2]);
     out[n] = v;
+  }
+}
+
+// one cell of k_box_cf_fill with NC interpolated components
+template <class LAT, int NC>
+__device__ inline void box_cf_cell(const Fab& Co, const Fab& Cn, const TimeMix& tm, const Fab& M, const Fab& Fq,
+                                   const int iv[3]) {
+  double m[NC < 4 ? 4 : NC] = {};
+  cons_interp_fab<LAT::D, NC>(Co, Cn, tm, iv[0], iv[1], iv[2], m);
+  if (M.p)
+#pragma unroll
+    for (int n = 0; n < NC; ++n) M.at(iv[0], iv[1], iv[2], n) = m[n];
+  if (!Fq.p) return;
+  // FillPatchMesoscopic's equilibrium of the ghost cells (:1605-1620)
+  const double r = fmax(m[0], 1.0e-12);
+  const double u0 = m[1], v0 = m[2], w0 = m[3];
+  const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
+#pragma unroll
+  for (int q = 0; q < LAT::Q; ++q) {
+    const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
+    Fq.at(iv[0], iv[1], iv[2], q) = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
   }
 }
 
 // Interpolated macros on the cells of `dom` (the f fab's box when f is
 // given, else the macro fab's) inside (ghosts = 0) or outside (ghosts = 1)
-// the valid box; f = feq of them where f is given.
+// the valid box; f = feq of them where f is given (ncomp >= 4 then).
 template <class LAT>
 __global__ void k_box_cf_fill(Fab Co, Fab Cn, TimeMix tm, Fab M, Fab Fq, Fab dom, int ncomp, BoxGeom G, int ghosts) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -740,19 +768,15 @@ __global__ void k_box_cf_fill(Fab Co, Fab Cn, TimeMix tm, Fab M, Fab Fq, Fab dom
   for (int d = 0; d < LAT::D; ++d)
     if (iv[d] < G.vlo[d] || iv[d] > G.vhi[d]) in_valid = false;
   if (in_valid == (ghosts != 0)) return;
-  double m[kMaxInterpComp];
-  cons_interp_fab(Co, Cn, tm, LAT::D, ncomp, iv[0], iv[1], iv[2], m);
-  if (M.p)
-    for (int n = 0; n < ncomp; ++n) M.at(iv[0], iv[1], iv[2], n) = m[n];
-  if (!Fq.p) return;
-  // FillPatchMesoscopic's equilibrium of the ghost cells (:1605-1620)
-  const double r = fmax(m[0], 1.0e-12);
-  const double u0 = m[1], v0 = m[2], w0 = m[3];
-  const double u2 = u0 * u0 + v0 * v0 + w0 * w0;
-#pragma unroll
-  for (int q = 0; q < LAT::Q; ++q) {
-    const double cu = double(LAT::cx[q]) * u0 + double(LAT::cy[q]) * v0 + double(LAT::cz[q]) * w0;
-    Fq.at(iv[0], iv[1], iv[2], q) = LAT::w[q] * r * (1.0 + 3.0 * cu + 4.5 * cu * cu - 1.5 * u2);
+  switch (ncomp) {
+    case 1: box_cf_cell<LAT, 1>(Co, Cn, tm, M, Fq, iv); break;
+    case 2: box_cf_cell<LAT, 2>(Co, Cn, tm, M, Fq, iv); break;
+    case 3: box_cf_cell<LAT, 3>(Co, Cn, tm, M, Fq, iv); break;
+    case 4: box_cf_cell<LAT, 4>(Co, Cn, tm, M, Fq, iv); break;
+    case 5: box_cf_cell<LAT, 5>(Co, Cn, tm, M, Fq, iv); break;
+    case 6: box_cf_cell<LAT, 6>(Co, Cn, tm, M, Fq, iv); break;
+    case 7: box_cf_cell<LAT, 7>(Co, Cn, tm, M, Fq, iv); break;
+    default: box_cf_cell<LAT, kMaxInterpComp>(Co, Cn, tm, M, Fq, iv); break;
   }
 }
 
