@@ -671,7 +671,9 @@ struct Amr {
       V.rec_override = true;
       V.rec_time = step_end;
       V.rec_step = (double)(istep + 1);
+      V.rec_to_ghost = true;   // a single-iteration record rides with the stream-collide launch
       V.ib_update_forcing(time, row);
+      V.rec_to_ghost = false;
     }
     if (!V.cf && !V.wrap) {
       // a stream-collide kernel that wrote the slabs of this step (level 0
@@ -682,7 +684,12 @@ struct Amr {
                   V.ghost_args());
     }
     StepArgs sa = V.step_args(export_state);
-    V.stepfn(V.grid, V.block, stream, sa, nullptr, nullptr);
+    if (V.rec_held) {
+      V.steprecfn(V.grid, V.block, stream, sa, V.held_a, V.held_r, V.held_nblk);
+      V.rec_held = false;
+    } else {
+      V.stepfn(V.grid, V.block, stream, sa, nullptr, nullptr);
+    }
     V.slabs_fresh = V.gfuse;
     exported[l] = export_state;
     V.cur ^= 1;

@@ -816,13 +816,19 @@ constexpr int kSumsBlock = 256;
 // Marker force integral sum_m F_m dv (:1781-1819, accumulated over MDF
 // iterations) and the moments r x (F dv) (:1500-1503), with their sum as
 // the body torque.  One block, fixed reduction tree.
+// Thread index and count of a block in linear order (x fastest), so that the
+// reductions below form the same wavefronts and trees in a block of
+// kSumsBlock x 1 threads and in one of the stream-collide kernel's 2D shapes.
+__device__ inline int lin_tid() { return (int)(threadIdx.x + threadIdx.y * blockDim.x); }
+__device__ inline int lin_nt() { return (int)(blockDim.x * blockDim.y); }
+
 __device__ __forceinline__ void sums_body(const IbArgs& a, int first_iter) {
   __shared__ double lds[6][16];
   const int nm = a.nm;
-  const int t = threadIdx.x;
+  const int t = lin_tid(), nt = lin_nt();
   double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   const double z0 = (a.dim == 3) ? a.z0 : 0.0;
-  for (int m = t; m < nm; m += blockDim.x) {
+  for (int m = t; m < nm; m += nt) {
     const double fx = a.F[m] * a.dv, fy = a.F[nm + m] * a.dv, fz = a.F[2 * nm + m] * a.dv;
     const double rx = a.xm[m] - a.x0, ry = a.xm[nm + m] - a.y0;
     const double rz = (a.dim == 3) ? a.xm[2 * nm + m] - z0 : 0.0;
@@ -838,7 +844,7 @@ __device__ __forceinline__ void sums_body(const IbArgs& a, int first_iter) {
   __syncthreads();
   if (t < 6) {
     double s = 0.0;
-    for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) s += lds[t][wv];
+    for (int wv = 0; wv < (nt + 63) / 64; ++wv) s += lds[t][wv];
     if (t < 3) a.sums[t] = first_iter ? s : a.sums[t] + s;
     else a.sums[t] = s;
   }
@@ -1403,15 +1409,16 @@ __global__ void __launch_bounds__(64 * kSpreadWaves) k_ib_spread_export_w(IbArgs
 __device__ __forceinline__ void force_record_body(const IbArgs& a, const RecArgs& rec, int nblk) {
   __shared__ double lds[3][4];
   double acc[3] = {0.0, 0.0, 0.0};
-  if (threadIdx.x < 256)
-    for (int t = threadIdx.x; t < nblk; t += 256)
+  const int tid = lin_tid();
+  if (tid < 256)
+    for (int t = tid; t < nblk; t += 256)
       for (int c = 0; c < 3; ++c) acc[c] += rec.partial[(int64_t)c * nblk + t];
   for (int c = 0; c < 3; ++c) {
     const double r = wave_sum(acc[c]);
-    if ((threadIdx.x & 63) == 0 && threadIdx.x < 256) lds[c][threadIdx.x >> 6] = r;
+    if ((tid & 63) == 0 && tid < 256) lds[c][tid >> 6] = r;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double sum[3] = {0.0, 0.0, 0.0};
     for (int c = 0; c < 3; ++c)
       for (int wv = 0; wv < 4; ++wv) sum[c] += lds[c][wv];
@@ -1427,18 +1434,19 @@ __device__ __forceinline__ void sums_record_body(const IbArgs& a, const RecArgs&
   __shared__ double lds[3][16];
   sums_body(a, 1);
   __syncthreads();
+  const int tid = lin_tid(), nt = lin_nt();
   double acc[3] = {0.0, 0.0, 0.0};
-  for (int t = threadIdx.x; t < nblk; t += blockDim.x)
+  for (int t = tid; t < nblk; t += nt)
     for (int c = 0; c < 3; ++c) acc[c] += rec.partial[(int64_t)c * nblk + t];
   for (int c = 0; c < 3; ++c) {
     const double r = wave_sum(acc[c]);
-    if ((threadIdx.x & 63) == 0) lds[c][threadIdx.x >> 6] = r;
+    if ((tid & 63) == 0) lds[c][tid >> 6] = r;
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
+  if (tid == 0) {
     double sum[3] = {0.0, 0.0, 0.0};
     for (int c = 0; c < 3; ++c)
-      for (int wv = 0; wv < (int)(blockDim.x + 63) / 64; ++wv) sum[c] += lds[c][wv];
+      for (int wv = 0; wv < (nt + 63) / 64; ++wv) sum[c] += lds[c][wv];
     write_force_record(a, rec, sum);
   }
 }
@@ -1467,8 +1475,8 @@ __global__ void __launch_bounds__(kSumsBlock) k_ghosts_record(GhostArgs g, unsig
 // keeps t as its mass partial), block ntiles writes the record.  The record
 // reads the marker forces and the export partials, which the stream-collide
 // does not touch; the next forcing, which rewrites them, follows the launch.
-// The step's blocks are kSumsBlock x 1 threads, so the record's reduction
-// tree is k_sums_record's.
+// The block (k_step's shape) holds kSumsBlock threads and the record's
+// reductions number them linearly, so its tree is k_sums_record's.
 template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step_record(StepArgs a, uint32_t tx, uint32_t ty, uint32_t ntiles,
                                                         IbArgs ia, RecArgs rec, int nblk) {
@@ -1481,7 +1489,8 @@ __global__ void __launch_bounds__(kBlock) k_step_record(StepArgs a, uint32_t tx,
   const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
   const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
   const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
-  step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x) + (int)threadIdx.x, o1 + (int)by, o2 + (int)bz, t, lds);
+  step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x), o1 + (int)(by * blockDim.y + threadIdx.y),
+                         o2 + (int)bz, t, lds);
 }
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of

@@ -307,7 +307,7 @@ using InitFn = void (*)(dim3, dim3, hipStream_t, const InitArgs&);
 using FixFn = void (*)(int, dim3, dim3, hipStream_t, const IbArgs&, const GridGeom&, void*, double, const RecArgs&);
 using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 using GhostRecFn = void (*)(unsigned, hipStream_t, const GhostArgs&, unsigned, const IbArgs&, const RecArgs&, int);
-using StepRecFn = void (*)(dim3, hipStream_t, const StepArgs&, const IbArgs&, const RecArgs&, int);
+using StepRecFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&, const IbArgs&, const RecArgs&, int);
 
 template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
@@ -349,11 +349,12 @@ static void launch_ghost_rec(unsigned grid, hipStream_t s, const GhostArgs& g, u
   hipLaunchKernelGGL((k_ghosts_record<LAT, ST>), dim3(grid), dim3(kSumsBlock), 0, s, g, nghost, a, r, nblk);
 }
 
-// g: k_step's grid (blocks of kSumsBlock x 1 threads); one block more
+// g, b: k_step's grid and block (kSumsBlock threads); one block more
 template <class LAT, class ST, int AM>
-static void launch_step_rec(dim3 g, hipStream_t s, const StepArgs& a, const IbArgs& ia, const RecArgs& r, int nblk) {
+static void launch_step_rec(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, const IbArgs& ia, const RecArgs& r,
+                            int nblk) {
   const uint32_t nt = g.x * g.y * g.z;
-  hipLaunchKernelGGL((k_step_record<LAT, ST, AM>), dim3(nt + 1), dim3(kSumsBlock), 0, s, a, g.x, g.y, nt, ia, r, nblk);
+  hipLaunchKernelGGL((k_step_record<LAT, ST, AM>), dim3(nt + 1), b, 0, s, a, g.x, g.y, nt, ia, r, nblk);
 }
 
 template <class LAT, class ST>
@@ -468,9 +469,10 @@ struct Level {
   // 3D single-iteration export: one wavefront per non-empty band cell (true)
   // or one thread per band cell
   bool spread_wave = true;
-  // one-stream steps: the force record of the forcing is held back
-  // (rec_to_ghost) and launched with the ghost slabs of the same step, or,
-  // when the stream-collide kernel writes those slabs, with its launch
+  // one-stream steps (levels and AMR substeps): the force record of the
+  // forcing is held back (rec_to_ghost) and launched with the ghost slabs of
+  // the same step or, with no slab launch, as the last block of the
+  // stream-collide launch
   bool rec_to_ghost = false, rec_held = false;
   // k_step writes the x-face ghost slabs of the next step (ghost_write);
   // slabs_fresh: the current buffer's slabs were written by the last step
@@ -1829,7 +1831,7 @@ struct Level {
       }
       const int64_t npad = wrap ? 0 : slabs.face_off[slabs.nface];
       if (ib) {
-        rec_to_ghost = npad > 0;
+        rec_to_ghost = true;
         ib_update_forcing(time, s);
         rec_to_ghost = false;
       }
@@ -1845,9 +1847,8 @@ struct Level {
         }
       }
       // a record still held goes with the stream-collide launch as its last
-      // block, unless the launch is timed or its blocks are not one row of
-      // kSumsBlock threads (then on its own, before it)
-      const bool rec_with_step = rec_held && !timing && block.x == (unsigned)kSumsBlock && block.y == 1;
+      // block, unless the launch is timed (then on its own, before it)
+      const bool rec_with_step = rec_held && !timing && block.x * block.y == (unsigned)kSumsBlock;
       if (rec_held && !rec_with_step) {
         hipLaunchKernelGGL(k_sums_record, dim3(1), dim3(kSumsBlock), 0, stream, held_a, held_r, held_nblk);
         rec_held = false;
@@ -1856,7 +1857,7 @@ struct Level {
       StepArgs sa = step_args(want_state && last);
       sa.partial = mring + (int64_t)mring_used * nblocks;
       if (rec_with_step) {
-        steprecfn(grid, stream, sa, held_a, held_r, held_nblk);
+        steprecfn(grid, block, stream, sa, held_a, held_r, held_nblk);
         rec_held = false;
       } else {
         // timing: the dispatch records the kernel's own begin / end
