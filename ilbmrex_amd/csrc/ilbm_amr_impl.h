@@ -337,9 +337,9 @@ struct Amr {
     for (int d = 0; d < 3; ++d) a.Fcol[d] = sa.Fcol[d];
     a.dpartial = dpartial[l];
     avgfn(dim3((unsigned)dblocks[l]), stream, a);
-    // slab values that the stream-collide kernel wrote from a boundary cell
-    // this average-down has overwritten are stale
-    if (a.clo[0] <= 0 || a.clo[0] + a.cn[0] >= C.c.n[0]) C.slabs_fresh = false;
+    // the covered cells keep 3 coarse cells inside the coarse box
+    // (nest_geometry), so no boundary cell of level 0 is averaged down and
+    // the slabs its stream-collide kernel wrote stay current
   }
 
   // ---------------- regrid ----------------

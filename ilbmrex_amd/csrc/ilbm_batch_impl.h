@@ -381,12 +381,14 @@ struct Batch {
   }
 
   // A case whose stream-collide kernel writes its own ghost slabs gets them
-  // from the boundary layer once, for its first batched step.
+  // from the boundary layer once, for its first batched step, unless they
+  // are current (restored from a checkpoint taken after a step).
   void prime_slabs(Level& L) {
-    if (L.wrap || !L.gfuse) return;
+    if (L.wrap || !L.gfuse || L.slabs_fresh) return;
     const int64_t npad = L.slabs.face_off[L.slabs.nface];
     L.ghostfn(dim3((unsigned)((npad + kGhostBlock - 1) / kGhostBlock)), dim3(kGhostBlock), stream, L.ghost_args());
     HIPCHK(hipGetLastError());
+    L.slabs_fresh = true;
   }
 
   // ---------------- admission ----------------
@@ -811,6 +813,7 @@ struct Batch {
       }
       L.istep += k;
       L.cur ^= (k & 1);
+      if (k > 0) L.slabs_fresh = L.gfuse;
       L.state_valid = false;
       bc.done += k;
       lups += (double)L.ncells * k;

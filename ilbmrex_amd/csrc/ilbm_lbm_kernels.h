@@ -681,11 +681,17 @@ __device__ __forceinline__ double step_update(const StepArgs& a, int i, int j, i
 #pragma unroll
     for (int q = 0; q < LAT::Q; ++q) a.fout[(int64_t)q * ncell + vcell] = (double)f[q] + (SH ? LAT::w[q] : 0.0);
   }
-  // pre-collision boundary layer for the next ghost fill
+  // pre-collision boundary layer for the next ghost fill.  A level whose
+  // stream-collide kernel writes its x-face slabs itself (gfuse) keeps the
+  // layer it had: its slabs are rebuilt from the layer only when they are
+  // not current, after the initial state (whose kernel writes the layer)
+  // or a restore of a checkpoint taken before the first step.  The stores
+  // are lone 2- to 8-byte writes of one lane per wavefront (FP32 sphere
+  // 256x128x128: 3 % of the kernel's time, FP16-shifted 8 %).
   const int cc[3] = {i, j, k};
 #pragma unroll
   for (int d = 0; d < LAT::D; ++d) {
-    if (!a.nonper[d]) continue;
+    if (!a.nonper[d] || (XO && a.gfuse)) continue;
     int side = -1;
     if (cc[d] == 0) side = 0;
     else if (cc[d] == g.n[d] - 1) side = 1;

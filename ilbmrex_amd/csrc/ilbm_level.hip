@@ -476,6 +476,7 @@ struct Level {
   bool rec_to_ghost = false, rec_held = false;
   // k_step writes the x-face ghost slabs of the next step (ghost_write);
   // slabs_fresh: the current buffer's slabs were written by the last step
+  // (or restored with it); a gfuse level's boundary layer is then stale
   bool gfuse = false, slabs_fresh = false;
   // IB sums in a fixed wave tree: levels with reduced storage, unless
   // ILBM_IB_EXACT=1 keeps the reference's serial order (FP64 storage always
@@ -1291,7 +1292,8 @@ struct Level {
     int abi, q, storage, ib;
     int64_t pop_bytes, face_bytes, band_cells;
     double t_new, t_old;
-    int istep, pad;
+    int istep;
+    int slabs_fresh;   // the saved slabs are current (else rebuilt from the boundary layer)
   };
 
   int64_t face_bytes_total() const {
@@ -1323,6 +1325,7 @@ struct Level {
     h.abi = ILBM_ABI_VERSION; h.q = c.q; h.storage = st; h.ib = ib ? 1 : 0;
     h.pop_bytes = pop_bytes_total(); h.face_bytes = face_bytes_total(); h.band_cells = ib ? band_cells : 0;
     h.t_new = t_new; h.t_old = t_old; h.istep = istep;
+    h.slabs_fresh = slabs_fresh ? 1 : 0;
     std::memcpy(out, &h, sizeof(h));
     char* p = out + sizeof(h);
     HIPCHK(hipMemcpy(p, gbuf[cur], h.pop_bytes, hipMemcpyDeviceToHost));
@@ -1375,7 +1378,7 @@ struct Level {
       HIPCHK(hipMemcpy(frstate_d, p, sizeof(double) * 4, hipMemcpyHostToDevice));
     }
     t_new = h.t_new; t_old = h.t_old; istep = h.istep;
-    slabs_fresh = false;
+    slabs_fresh = h.slabs_fresh != 0;
     const double clk[4] = {t_new, (double)istep, (double)istep, 0.0};
     HIPCHK(hipMemcpy(clock_d, clk, sizeof(clk), hipMemcpyHostToDevice));
     state_valid = false;
