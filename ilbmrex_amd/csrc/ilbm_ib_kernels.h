@@ -1489,8 +1489,8 @@ __global__ void __launch_bounds__(kBlock) k_step_record(StepArgs a, uint32_t tx,
   const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
   const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
   const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
-  step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x), o1 + (int)(by * blockDim.y + threadIdx.y),
-                         o2 + (int)bz, t, lds);
+  step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x),
+                         wave_row(o1 + (int)(by * blockDim.y + threadIdx.y)), o2 + (int)bz, t, lds);
 }
 
 // The ComputeIBForce record (AmrCoreLBM.cpp:456-631) from the band sum of

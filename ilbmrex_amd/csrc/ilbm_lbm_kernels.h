@@ -811,14 +811,21 @@ __device__ __forceinline__ void step_body(const StepArgs& a, int i, int j, int k
   finish_mass(a, s, blk);
 }
 
+// Row of a thread of a level's stream-collide blocks: they are bx x by
+// threads with bx a multiple of 64 (the launchers refuse other shapes), so
+// every wavefront lies in one row, and the row and the offsets derived from
+// it stay in scalar registers (FP16-shifted sphere: fewer vector
+// instructions per wavefront).
+__device__ __forceinline__ int wave_row(int j) { return __builtin_amdgcn_readfirstlane(j); }
+
 template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   __shared__ double lds[kBlock / 64];
   const int64_t blk = ((int64_t)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
   const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
   const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
-  step_body<LAT, ST, AM>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x, o1 + blockIdx.y * blockDim.y + threadIdx.y,
-                           o2 + blockIdx.z, blk, lds);
+  step_body<LAT, ST, AM>(a, o0 + blockIdx.x * blockDim.x + threadIdx.x,
+                         wave_row(o1 + blockIdx.y * blockDim.y + threadIdx.y), o2 + blockIdx.z, blk, lds);
 }
 
 // k_step with a fixed number of resident blocks, each striding over the
@@ -834,8 +841,8 @@ __global__ void __launch_bounds__(kBlock) k_step_p(StepArgs a, uint32_t tx, uint
   const int o2 = (a.sub_mode >= 2) ? a.sub_lo[2] : 0;
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
-    step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x), o1 + (int)(by * blockDim.y + threadIdx.y),
-                           o2 + (int)bz, (int64_t)t, lds);
+    step_body<LAT, ST, AM>(a, o0 + (int)(bx * blockDim.x + threadIdx.x),
+                           wave_row(o1 + (int)(by * blockDim.y + threadIdx.y)), o2 + (int)bz, (int64_t)t, lds);
   }
 }
 

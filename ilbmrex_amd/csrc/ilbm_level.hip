@@ -309,8 +309,15 @@ using GhostFn = void (*)(dim3, dim3, hipStream_t, const GhostArgs&);
 using GhostRecFn = void (*)(unsigned, hipStream_t, const GhostArgs&, unsigned, const IbArgs&, const RecArgs&, int);
 using StepRecFn = void (*)(dim3, dim3, hipStream_t, const StepArgs&, const IbArgs&, const RecArgs&, int);
 
+// the stream-collide kernels take a wavefront's row as uniform (wave_row)
+static void check_step_block(dim3 b) {
+  if (b.x % 64 != 0 || b.x * b.y > (unsigned)kBlock || b.z != 1)
+    throw StatusError{ILBM_ERR_INVALID, "stream-collide block must be bx x by <= 256 threads, bx a multiple of 64"};
+}
+
 template <class LAT, class ST, int AM>
 static void launch_step(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, hipEvent_t e0, hipEvent_t e1) {
+  check_step_block(b);
   const uint32_t nt = g.x * g.y * g.z;
   if (a.persist > 0 && nt > (uint32_t)a.persist) {
     const dim3 pg((unsigned)a.persist);
@@ -353,6 +360,7 @@ static void launch_ghost_rec(unsigned grid, hipStream_t s, const GhostArgs& g, u
 template <class LAT, class ST, int AM>
 static void launch_step_rec(dim3 g, dim3 b, hipStream_t s, const StepArgs& a, const IbArgs& ia, const RecArgs& r,
                             int nblk) {
+  check_step_block(b);
   const uint32_t nt = g.x * g.y * g.z;
   hipLaunchKernelGGL((k_step_record<LAT, ST, AM>), dim3(nt + 1), b, 0, s, a, g.x, g.y, nt, ia, r, nblk);
 }
