@@ -857,9 +857,11 @@ __global__ void __launch_bounds__(kBlock) k_step_p(StepArgs a, uint32_t tx, uint
 // below cell 0; bx = 0, cells [r, r + nc * 256) in x-fastest order, cell c
 // of thread t at 256 c + t (no row padding; a cell past the end gets
 // j = n[1], inactive).
+// ROW: bx > 0, so every wavefront lies in one row of the tile (wave_row).
+template <bool ROW>
 __device__ inline void batch_tile(const GridGeom& g, int bx, const BlockRef& r, int c, int* i, int* j, int* k) {
   const int t = threadIdx.x;
-  if (bx == 0) {
+  if (!ROW) {
     const int nx = g.n[0], ny = g.n[1];
     // the chunk's first cell (uniform), then the lane's: each steps over at
     // most kBlock / n[0] + 1 row ends
@@ -880,7 +882,7 @@ __device__ inline void batch_tile(const GridGeom& g, int bx, const BlockRef& r, 
   }
   const int sh = __builtin_ctz((unsigned)bx);   // bx is 64, 128 or 256
   *i = r.i0 + (t & (bx - 1));
-  *j = r.j0 + c * (kBlock >> sh) + (t >> sh);
+  *j = r.j0 + c * (kBlock >> sh) + wave_row(t >> sh);
   *k = r.k0;
 }
 
@@ -888,23 +890,29 @@ __device__ inline void batch_tile(const GridGeom& g, int bx, const BlockRef& r, 
 // a case's tile_nc (uniform over its blocks) selects one or two cells per
 // thread (nine loads per cell keep too few bytes in flight); 3D lattices
 // take one, which keeps their register budget.
-template <class LAT, class ST, int AM>
-__device__ __forceinline__ void step_b_body(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map,
-                                            unsigned blk, double* lds) {
-  const BlockRef r = map[blk];
-  const StepArgs& a = A[r.c];
+template <class LAT, class ST, int AM, bool ROW>
+__device__ __forceinline__ void step_b_tile(const StepArgs& a, const BlockRef& r, double* lds) {
   if constexpr (LAT::D == 2) {
     if (a.tile_nc == 2) {
       int i[2], j[2], k[2];
-      batch_tile(a.g, a.tile_bx, r, 0, &i[0], &j[0], &k[0]);
-      batch_tile(a.g, a.tile_bx, r, 1, &i[1], &j[1], &k[1]);
+      batch_tile<ROW>(a.g, a.tile_bx, r, 0, &i[0], &j[0], &k[0]);
+      batch_tile<ROW>(a.g, a.tile_bx, r, 1, &i[1], &j[1], &k[1]);
       step_body_n<LAT, ST, AM, 2>(a, i, j, k, r.b, lds);
       return;
     }
   }
   int i, j, k;
-  batch_tile(a.g, a.tile_bx, r, 0, &i, &j, &k);
+  batch_tile<ROW>(a.g, a.tile_bx, r, 0, &i, &j, &k);
   step_body<LAT, ST, AM>(a, i, j, k, r.b, lds);
+}
+
+template <class LAT, class ST, int AM>
+__device__ __forceinline__ void step_b_body(const StepArgs* __restrict__ A, const BlockRef* __restrict__ map,
+                                            unsigned blk, double* lds) {
+  const BlockRef r = map[blk];
+  const StepArgs& a = A[r.c];
+  if (a.tile_bx != 0) step_b_tile<LAT, ST, AM, true>(a, r, lds);
+  else step_b_tile<LAT, ST, AM, false>(a, r, lds);
 }
 
 template <class LAT, class ST, int AM>
