@@ -1456,35 +1456,42 @@ __global__ void __launch_bounds__(kSumsBlock) k_sums_record(IbArgs a, RecArgs re
 }
 
 // A step's ghost slabs and the force record of its forcing in one launch
-// (levels that step on one stream): blocks [0, nghost) fill kSumsBlock slab
-// cells each, the last block writes the record.  Neither reads what the
-// other writes, and the stream-collide launch that follows needs both.
+// (levels that step on one stream): block 0 writes the record, blocks
+// [1, nghost + 1) fill kSumsBlock slab cells each.  Neither reads what the
+// other writes, and the stream-collide launch that follows needs both.  The
+// record, a chain of dependent reductions, goes first: dispatched last it
+// would start once every other block had been placed and end the launch.
 template <class LAT, class ST>
 __global__ void __launch_bounds__(kSumsBlock) k_ghosts_record(GhostArgs g, unsigned nghost, IbArgs a, RecArgs rec,
                                                                int nblk) {
-  if (blockIdx.x < nghost) {
-    ghost_body<LAT, ST>(g, (int64_t)blockIdx.x * kSumsBlock + threadIdx.x);
+  (void)nghost;
+  if (blockIdx.x == 0) {
+    sums_record_body(a, rec, nblk);
     return;
   }
-  sums_record_body(a, rec, nblk);
+  ghost_body<LAT, ST>(g, (int64_t)(blockIdx.x - 1) * kSumsBlock + threadIdx.x);
 }
 
 // A step's stream-collide and the force record of the forcing before it in
 // one launch (levels that step on one stream and need no separate ghost
-// fill): blocks [0, ntiles) are k_step's blocks in linear order (tile t
-// keeps t as its mass partial), block ntiles writes the record.  The record
-// reads the marker forces and the export partials, which the stream-collide
-// does not touch; the next forcing, which rewrites them, follows the launch.
-// The block (k_step's shape) holds kSumsBlock threads and the record's
-// reductions number them linearly, so its tree is k_sums_record's.
+// fill): block 0 writes the record, blocks [1, ntiles + 1) are k_step's
+// blocks in linear order (tile t = block - 1 keeps t as its mass partial).
+// The record reads the marker forces and the export partials, which the
+// stream-collide does not touch; the next forcing, which rewrites them,
+// follows the launch.  The record goes first so that it runs beside the
+// stream-collide blocks (last, it ended the launch: sphere 256x128x128
+// 112.1 against 107.4 us).  The block (k_step's shape) holds kSumsBlock
+// threads and the record's reductions number them linearly, so its tree is
+// k_sums_record's.
 template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step_record(StepArgs a, uint32_t tx, uint32_t ty, uint32_t ntiles,
                                                         IbArgs ia, RecArgs rec, int nblk) {
-  const uint32_t t = blockIdx.x;
-  if (t == ntiles) {
+  (void)ntiles;
+  if (blockIdx.x == 0) {
     sums_record_body(ia, rec, nblk);
     return;
   }
+  const uint32_t t = blockIdx.x - 1;
   __shared__ double lds[kBlock / 64];
   const uint32_t bz = t / (tx * ty), rem = t - bz * tx * ty, by = rem / tx, bx = rem - by * tx;
   const int o0 = (a.sub_mode >= 2) ? a.sub_lo[0] : 0, o1 = (a.sub_mode >= 2) ? a.sub_lo[1] : 0;
@@ -1640,39 +1647,44 @@ __device__ __forceinline__ void case_record_body(const IbCase& c) {
 }
 
 // The ghost slabs and the force records of a batched step in one launch:
-// blocks [0, nghost) fill the slab cells of their cases (kBlock each), the
-// rest write one case's record each.  Neither reads what the other writes,
-// and the stream-collide launch that follows needs both.
+// the first blocks write one case's record each (gridDim.x - nghost of
+// them, first for the reason k_ghosts_record's goes first), the other
+// nghost fill the slab cells of their cases (kBlock each).  Neither reads
+// what the other writes, and the stream-collide launch that follows needs
+// both.
 static_assert(kBlock == kSumsBlock, "slab and record blocks share the launch's block size");
 template <class LAT, class ST>
 __global__ void __launch_bounds__(kSumsBlock) k_ghosts_record_b(const GhostArgs* __restrict__ GA,
                                                                const BlockRef* __restrict__ gmap, unsigned nghost,
                                                                const IbCase* __restrict__ A,
                                                                const BlockRef* __restrict__ rmap) {
-  if (blockIdx.x < nghost) {
-    const BlockRef r = gmap[blockIdx.x];
-    ghost_body<LAT, ST>(GA[r.c], (int64_t)r.b * kBlock + threadIdx.x);
+  const unsigned nrec = gridDim.x - nghost;
+  if (blockIdx.x < nrec) {
+    case_record_body(A[rmap[blockIdx.x].c]);
     return;
   }
-  case_record_body(A[rmap[blockIdx.x - nghost].c]);
+  const BlockRef r = gmap[blockIdx.x - nrec];
+  ghost_body<LAT, ST>(GA[r.c], (int64_t)r.b * kBlock + threadIdx.x);
 }
 
 // A batched step's stream-collide and the force records of its IB cases in
 // one launch (classes whose stream-collide kernel writes the slabs, so no
-// slab launch carries the records): blocks [0, nstep) are k_step_b's, the
-// rest write one case's record each.  Neither reads what the other writes,
-// and the tail that advances the cases' clocks follows the launch.
+// slab launch carries the records): the first gridDim.x - nstep blocks
+// write one case's record each (first, as in k_step_record), the other
+// nstep are k_step_b's.  Neither reads what the other writes, and the tail
+// that advances the cases' clocks follows the launch.
 template <class LAT, class ST, int AM>
 __global__ void __launch_bounds__(kBlock) k_step_record_b(const StepArgs* __restrict__ A,
                                                           const BlockRef* __restrict__ map, unsigned nstep,
                                                           const IbCase* __restrict__ IA,
                                                           const BlockRef* __restrict__ rmap) {
-  if (blockIdx.x < nstep) {
-    __shared__ double lds[kBlock / 64];
-    step_b_body<LAT, ST, AM>(A, map, blockIdx.x, lds);
+  const unsigned nrec = gridDim.x - nstep;
+  if (blockIdx.x < nrec) {
+    case_record_body(IA[rmap[blockIdx.x].c]);
     return;
   }
-  case_record_body(IA[rmap[blockIdx.x - nstep].c]);
+  __shared__ double lds[kBlock / 64];
+  step_b_body<LAT, ST, AM>(A, map, blockIdx.x - nrec, lds);
 }
 
 }  // namespace ilbm
