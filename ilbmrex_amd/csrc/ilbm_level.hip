@@ -438,16 +438,18 @@ static int64_t plane_stride(int64_t cells, int64_t pop_bytes) {
   return ((cells * pop_bytes + 16383) / 16384 * 16384 + 8192) / pop_bytes;
 }
 
-// Overlapped step of an IB level: the far cells start once this step's
-// interpolation is done (1; 2: the export; 0: at once) as a launch of one
+// Overlapped step of an IB level: the far cells start at once (0), or once
+// this step's interpolation (1) or export (2) is done, as a launch of one
 // block per tile (0) or a persistent launch of that many blocks per CU.
-// The interpolation then runs on an idle device, and the spreading kernel
-// is dispatched beside the far cells' launch instead of behind it.
+// Waiting for the interpolation lets it run on an idle device, but the
+// cross-stream wait then sits between the far launches of two steps.
 // Measured on the 256x128x128 sphere (GLUPS, FP32 / FP16-shifted / FP64):
 // at once with 4 blocks per CU 29.0 / 28.3 / 12.9; after the interpolation,
-// full grid 30.7 / 46.1 / 16.3.
+// full grid 30.7 / 46.1 / 16.3 with the IB and stream-collide kernels of
+// that time, 33.3 / 54.4 / 17.0 with the later ones; at once, full grid,
+// 34.5 / 56.7 / 17.2.
 constexpr int kFarBlocksPerCu = 0;
-constexpr int kFarAfter = 1;
+constexpr int kFarAfter = 0;
 constexpr int kRingStream = 0;   // Level::RING_NEAR
 
 struct Level {

@@ -76,7 +76,7 @@ def test_overlap_sphere_off_walls_split_advances():
     b.close()
 
 
-@pytest.mark.parametrize("regions", ["three", "two", "three_at_once", "three_after_export",
+@pytest.mark.parametrize("regions", ["three", "two", "three_after_interp", "three_after_export",
                                      "three_persistent", "three_separate_slabs", "three_ring_on_far",
                                      "three_ring_on_near", "three_ring_own_stream"])
 def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
@@ -84,9 +84,9 @@ def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
     twice, the alternating far / ring signals run through every parity)
     against the single-stream step: bit-identical state and force history.
     "three": far cells, near box and the ring between them as three
-    launches, the far cells after the step's interpolation; "two":
-    ILBM_OVERLAP_TWO_REGIONS=1 keeps far + near; the far cells at once or
-    after the step's export (ILBM_FAR_AFTER), as a persistent launch of 4
+    launches, the far cells at once; "two": ILBM_OVERLAP_TWO_REGIONS=1
+    keeps far + near; the far cells after the step's interpolation or
+    export (ILBM_FAR_AFTER), as a persistent launch of 4
     blocks per CU (ILBM_FAR_BLOCKS_PER_CU=4), and with the separate ghost
     slab kernel (ILBM_GHOST_SEPARATE=1); the ring on the far cells', the
     near box's or its own stream (ILBM_RING_STREAM)."""
@@ -96,8 +96,8 @@ def test_overlap_long_run_crosses_mass_ring_halves(regions, monkeypatch):
         monkeypatch.delenv(v, raising=False)
     if regions == "two":
         monkeypatch.setenv("ILBM_OVERLAP_TWO_REGIONS", "1")
-    if regions == "three_at_once":
-        monkeypatch.setenv("ILBM_FAR_AFTER", "none")
+    if regions == "three_after_interp":
+        monkeypatch.setenv("ILBM_FAR_AFTER", "interp")
     if regions == "three_after_export":
         monkeypatch.setenv("ILBM_FAR_AFTER", "export")
     if regions == "three_persistent":
